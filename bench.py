@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Headline benchmark: path-traced samples per second on one MI355X.
+
+Workload (a "step" = one full frame): the seeded displaced sphere of
+1,048,576 triangles (7 material regions: diffuse, rough dielectric, rough
+conductor, smooth plastic, rough plastic, smooth coating, rough coating) on a
+ground quad under a Hosek sky, 1920x1080 at 64 samples per pixel, max depth
+16, Russian roulette from depth 5.  Scene data is resident in HBM before the
+timed region; every step renders the same frame (same seed and sample
+indices) so each step does identical work.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "ray-tracing_amd"))
+
+METRIC = "Msamples/s (1080p, 1M tris); time to relMSE≤1e-2: denoised vs plain"
+W, H, SPP, MAX_DEPTH, RR_DEPTH, SEED = 1920, 1080, 64, 16, 5, 1
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(gen, budget_s=15.0):
+    """The oracle's CPU restatement (tests/oracle_lib.py) on this host's cores:
+    a band of full-width rows of the same frame at 64 spp, sized to about
+    budget_s seconds of render-loop time (scene build timed apart)."""
+    sys.path.insert(0, str(REPO / "tests"))
+    import oracle_lib  # noqa: E402
+    if oracle_lib.oracle is None:
+        return None
+    threads = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    sc = oracle_lib.oracle.scene(gen.desc_ptr)
+    build_s = time.perf_counter() - t0
+    mid = H // 2
+    t0 = time.perf_counter()
+    sc.render(W, H, 4, MAX_DEPTH, RR_DEPTH, seed=SEED, threads=threads, rows=(mid, mid + 1))
+    probe = time.perf_counter() - t0
+    rows = max(1, min(H // 2, int(budget_s / max(probe * 16, 1e-3))))
+    t0 = time.perf_counter()
+    r = sc.render(W, H, SPP, MAX_DEPTH, RR_DEPTH, seed=SEED, threads=threads, rows=(mid, mid + rows))
+    dt = time.perf_counter() - t0
+    samples = W * rows * SPP
+    return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
+            "sample": f"rows {mid}..{mid + rows - 1} of the 1920x1080 frame at 64 spp ({samples} samples, "
+                      f"{dt:.1f} s render loop; kd build {build_s:.1f} s not timed)",
+            "extension_rays": int(r["ext"]), "shadow_rays": int(r["shadow"])}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--spp", type=int, default=SPP)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by pixel/sample ranges "
+              f"across GPUs but no multi-GPU run is part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import linrt
+
+    if linrt.device_count() < 1 or not torch.cuda.is_available():
+        log("no GPU visible")
+        sys.exit(1)
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    gen = linrt.displaced_sphere(seed=SEED)
+    gen_s = time.perf_counter() - t0
+    scene = linrt.Scene(gen.desc_ptr, device=0)
+    info = scene.info
+    log(f"scene: {info.n_triangles} tris, {info.n_nodes} nodes, {info.n_refs} refs, depth {info.max_tree_depth}, "
+        f"{info.device_bytes / 2**20:.1f} MiB, gen {gen_s:.2f}s build {info.build_seconds:.2f}s")
+    acc = torch.zeros((H, W, 3), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    # untimed counting pass over the same frame: algorithmic work per ray
+    st_c = scene.render_device(W, H, args.spp, MAX_DEPTH, RR_DEPTH, acc.data_ptr(), seed=SEED, count=True)
+    scene.sync()
+    log(f"count pass: ext {st_c.extension_rays} shadow {st_c.shadow_rays} nodes {st_c.nodes_visited} "
+        f"refs {st_c.refs_read} tris {st_c.tris_tested}")
+    for _ in range(args.warmup):
+        scene.render_device(W, H, args.spp, MAX_DEPTH, RR_DEPTH, acc.data_ptr(), seed=SEED)
+    scene.sync()
+    torch.cuda.synchronize()
+    scene.kernel_timing(linrt.KERNEL_CLOSEST, reset=True)
+    scene.kernel_timing(linrt.KERNEL_OCCLUDED, reset=True)
+    scene.kernel_timing(linrt.KERNEL_SHADE, reset=True)
+
+    stats = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        stats.append(scene.render_device(W, H, args.spp, MAX_DEPTH, RR_DEPTH, acc.data_ptr(), seed=SEED))
+        log(f"step {k}: {stats[-1].seconds:.3f}s")
+    scene.sync()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+
+    samples = sum(s.samples for s in stats)
+    ext = sum(s.extension_rays for s in stats)
+    shadow = sum(s.shadow_rays for s in stats)
+    ms_c, n_c, _ = scene.kernel_timing(linrt.KERNEL_CLOSEST)
+    ms_o, n_o, _ = scene.kernel_timing(linrt.KERNEL_OCCLUDED)
+    ms_s, n_s, _ = scene.kernel_timing(linrt.KERNEL_SHADE)
+    value = samples / elapsed / 1e6
+
+    # roofline of the dominant kernel (closest-hit traversal of extension rays):
+    # B_ray = 32 (ray) + 8 n_node + 4 n_ref + 36 n_tri + 16 (hit) bytes
+    per_frame_bytes = (32 + 16) * st_c.extension_rays + 8 * st_c.nodes_visited + 4 * st_c.refs_read + \
+        36 * st_c.tris_tested
+    launches_per_frame = n_c / max(args.steps, 1)
+    bytes_per_launch = per_frame_bytes / launches_per_frame
+    avg_ms = ms_c / max(n_c, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tf = REPO / "profiles" / "traffic_extend_displaced_sphere.json"
+    if tf.exists():
+        traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+
+    cpu = None if args.no_cpu else cpu_baseline(gen)
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded scene generator)",
+        "config": {"workload": "displaced sphere 1,048,576 tris + ground, 7 BSDF regions, Hosek sky; "
+                               f"{W}x{H}, {args.spp} spp, max depth {MAX_DEPTH}, RR from depth {RR_DEPTH}",
+                   "width": W, "height": H, "spp": args.spp, "max_depth": MAX_DEPTH, "rr_depth": RR_DEPTH,
+                   "triangles": int(info.n_triangles), "parallelism": "single GPU"},
+        "mrays_per_s": round((ext + shadow) / elapsed / 1e6, 2),
+        "rays_per_sample": round((ext + shadow) / max(samples, 1), 3),
+        "kernel_ms_per_step": {"extend": round(ms_c / args.steps, 2), "shadow": round(ms_o / args.steps, 2),
+                               "shade": round(ms_s / args.steps, 2)},
+        "roofline": {"bound": "hbm", "kernel": "k_extend (closest-hit kd traversal)", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(avg_ms, 4),
+                     "launches": int(n_c)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

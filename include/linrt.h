@@ -1,0 +1,292 @@
+/*
+ * linrt.h - C ABI of liblinrt_hip.so, the MI355X (gfx950) render hot path of
+ * LinRT (Lincacc/Ray-Tracing).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (LRT_OK == 0, negative on error) and never lets a C++ exception
+ * cross the boundary.  lrt_last_error() returns a thread-local message for the
+ * most recent failure on the calling thread.
+ *
+ * Threading: a lrt_scene is externally synchronised (one host thread at a time
+ * per scene).  All GPU work of a scene is ordered on the scene's own HIP stream
+ * and every call blocks until its results are complete, unless stated.
+ *
+ * Reference interfaces that these entry points stand in for are cited per
+ * declaration as path:line of the reference repository.
+ */
+#ifndef LINRT_H
+#define LINRT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    LRT_OK = 0,
+    LRT_ERR_ARG = -1,         /* invalid argument (null pointer, bad index, size) */
+    LRT_ERR_HIP = -2,         /* a HIP runtime call failed */
+    LRT_ERR_ALLOC = -3,       /* host or device allocation failed */
+    LRT_ERR_UNSUPPORTED = -4, /* valid request this build does not implement */
+    LRT_ERR_INTERNAL = -5
+};
+
+/* ---- BSDF flags (bsdf.h:18-28) ----------------------------------------- */
+enum {
+    LRT_FLAG_NONE = 0,
+    LRT_FLAG_DIFFUSE = 1u << 0,
+    LRT_FLAG_SPECULAR = 1u << 1,
+    LRT_FLAG_REFLECTIVE = 1u << 2,
+    LRT_FLAG_TRANSPARENT = 1u << 3,
+    LRT_FLAG_GLOSSY = 1u << 4,
+    LRT_FLAG_METALLIC = 1u << 5,
+    LRT_FLAG_DIELECTRIC = 1u << 6,
+    LRT_FLAG_REFRACTIVE = 1u << 7
+};
+
+/* measure (bsdf.h:9-16): only these two are used on the path */
+enum { LRT_MEASURE_SOLID_ANGLE = 1, LRT_MEASURE_DISCRETE = 5 };
+/* transport mode (bsdf.h:31-34) */
+enum { LRT_MODE_RADIANCE = 0, LRT_MODE_IMPORTANCE = 1 };
+
+/* ---- materials: one record per reference BSDF class ------------------- */
+enum {
+    LRT_BSDF_NONE = 0,            /* surface without BSDF: paths stop here */
+    LRT_BSDF_DIFFUSE = 1,         /* diffuse_bsdf.h */
+    LRT_BSDF_CONDUCTOR = 2,       /* rough_conductor_bsdf.h (alpha <= 0: smooth mirror) */
+    LRT_BSDF_DIELECTRIC = 3,      /* dielectric_bsdf.h */
+    LRT_BSDF_ROUGH_DIELECTRIC = 4,/* rough_dielectric_bsdf.h */
+    LRT_BSDF_PLASTIC = 5,         /* plastic_bsdf.h */
+    LRT_BSDF_ROUGH_PLASTIC = 6,   /* rough_plastic_bsdf.h */
+    LRT_BSDF_COATING = 7,         /* coating_bsdf.h, over materials[nested] */
+    LRT_BSDF_ROUGH_COATING = 8    /* rough_coating_bsdf.h, over materials[nested] */
+};
+
+/* Host-side precomputed data a material needs is carried inline:
+ *   fdr_int    plastic: fresnel_diffuse_reflectance(1/eta) (fresnel.h:128-133),
+ *              fill with lrt_fresnel_diffuse_reflectance().
+ *   table_ext/table_int  rough plastic, rough coating: index of a 101-float
+ *              rough-transmittance slice (100 theta samples + 1 diffuse value,
+ *              rough_transmittance.h:129-185) in lrt_scene_desc.tables, or -1.
+ * Constructor semantics follow the reference: eta = int_ior/ext_ior, GGX alpha
+ * clamped at 1e-4 inside the distribution (microfacet.h:12-13). */
+typedef struct lrt_material {
+    int32_t type;
+    int32_t nested;                 /* coatings: index of the inner material, else -1 */
+    float int_ior, ext_ior;
+    float alpha;
+    int32_t nonlinear;              /* plastic / rough plastic */
+    float diffuse[3];               /* diffuse albedo, plastic diffuse reflectance */
+    float specular_reflectance[3];
+    float specular_transmittance[3];
+    float cond_eta[3], cond_k[3];   /* conductor complex IOR */
+    float sigma_a[3];
+    float thickness;                /* coatings */
+    float fdr_int;                  /* plastic */
+    int32_t table_ext, table_int;   /* rough plastic: both; rough coating: table_ext */
+} lrt_material;
+
+/* ---- scene description (host arrays the caller owns; they are copied) -- */
+enum { LRT_SHAPE_MESH = 0, LRT_SHAPE_RECTANGLE = 1, LRT_SHAPE_SPHERE = 2 };
+
+typedef struct lrt_object_desc {
+    int32_t shape;
+    int32_t material;               /* -1: no BSDF (rectangles never carry one, rectangle.h:50) */
+    int32_t emitter;                /* index into lights (area light on this shape) or -1 */
+    uint32_t n_vertices, n_triangles;
+    const float* positions;         /* 3*n_vertices */
+    const uint32_t* indices;        /* 3*n_triangles */
+    const float* corner_normals;    /* 9*n_triangles, one normal per corner (triangle_mesh.h:13) */
+    const uint16_t* triangle_material; /* optional per-triangle material, NULL: use .material */
+    float corner[3], edge_u[3], edge_v[3]; /* rectangle (rectangle.h:9-16) */
+    float center[3];
+    float radius;                   /* sphere (sphere.h:10-13) */
+} lrt_object_desc;
+
+enum { LRT_LIGHT_AREA = 0, LRT_LIGHT_SKY = 1 };
+
+typedef struct lrt_light_desc {
+    int32_t kind;
+    int32_t object;                 /* area light: emitting object */
+    float radiance[3];              /* area light */
+    float sun_dir[3];               /* sky: sun direction (normalised by the library) */
+    float turbidity, ground_albedo; /* sky: model parameters (informative) */
+    float scale;                    /* sky: exposure (env_emitter.h:13) */
+    double config[3][9];            /* sky: cooked Hosek-Wilkie RGB coefficients */
+    double channel_radiance[3];     /* sky: cooked per-channel radiance */
+} lrt_light_desc;
+
+typedef struct lrt_camera_desc {
+    float origin[3], look_at[3], up[3];
+    float fov_deg, aspect;          /* vertical field of view (camera.h:20-44) */
+} lrt_camera_desc;
+
+typedef struct lrt_scene_desc {
+    uint32_t n_objects;
+    const lrt_object_desc* objects;
+    uint32_t n_materials;
+    const lrt_material* materials;
+    uint32_t n_lights;
+    const lrt_light_desc* lights;   /* emitter list order is the reference's emitter order */
+    uint32_t n_tables;
+    const float* tables;            /* n_tables * 101 floats */
+    lrt_camera_desc camera;
+} lrt_scene_desc;
+
+typedef struct lrt_scene lrt_scene; /* opaque, library-owned */
+
+/* ---- errors / device ---------------------------------------------------- */
+const char* lrt_last_error(void);
+int lrt_version(void);
+/* number of visible HIP devices (0 without a GPU; never fails) */
+int lrt_device_count(void);
+
+/* ---- accelerator: SAH kd-tree (sah_kdtree.h:35-57, 175-347) ------------ *
+ * Builds the reference's SAH kd-tree over n boxes (6 floats each:
+ * min xyz, max xyz) and returns it in the reference's own node numbering,
+ * before flattening: per node leaf flag, axis, split, left, right, first,
+ * count, plus the ordered primitive references.  Call with null outputs to
+ * get the sizes (*n_nodes, *n_refs, *depth_limit), then again with arrays. */
+int lrt_kdtree_build_export(const float* boxes, uint32_t n,
+                            uint32_t* n_nodes, uint32_t* n_refs, int32_t* depth_limit,
+                            uint8_t* leaf, int32_t* axis, float* split,
+                            int32_t* left, int32_t* right, int32_t* first, int32_t* count,
+                            uint32_t* refs);
+
+/* ---- scene upload (scene.h:19-34, triangle_mesh.h:14-28) --------------- */
+int lrt_scene_create(int hip_device, const lrt_scene_desc* desc, lrt_scene** out);
+int lrt_scene_release(lrt_scene* scene);
+/* statistics of the uploaded acceleration structures */
+typedef struct lrt_scene_info {
+    uint64_t n_triangles, n_nodes, n_refs, device_bytes;
+    int32_t max_tree_depth;
+    double build_seconds, upload_seconds;
+} lrt_scene_info;
+int lrt_scene_info_get(const lrt_scene* scene, lrt_scene_info* info);
+
+/* ---- ray queries (scene.h:36-38 -> hittable_list.h:19-34 ->
+ *      triangle_mesh.h:30-99 / sah_kdtree.h:61-121) ------------------------
+ * rays: SoA, 8 arrays of n floats back to back: ox oy oz dx dy dz tmin tmax.
+ * hit_obj_prim: 2*n int32 (object, primitive; -1,-1 on a miss);
+ * hit_t: n floats; hit_uv: 2*n floats (barycentric u, v; may be NULL).
+ * device_ptrs != 0: every pointer is device memory on the scene's device and
+ * the call is asynchronous on the scene's stream (use lrt_scene_sync).      */
+int lrt_trace_closest(lrt_scene* scene, const float* rays, uint32_t n,
+                      int32_t* hit_obj_prim, float* hit_t, float* hit_uv, int device_ptrs);
+/* occlusion query: occluded[i] = 1 if any surface other than those of
+ * emitter ignore_emitter[i] (-1: none) lies in [tmin, tmax]
+ * (path_integrator.h:113-116). ignore_emitter may be NULL. */
+int lrt_trace_occluded(lrt_scene* scene, const float* rays, uint32_t n,
+                       const int32_t* ignore_emitter, uint8_t* occluded, int device_ptrs);
+/* algorithmic-work counters of the closest-hit traversal: per ray, interior
+ * and leaf nodes visited, leaf references read, triangle tests run */
+int lrt_trace_count(lrt_scene* scene, const float* rays, uint32_t n,
+                    uint32_t* nodes, uint32_t* refs, uint32_t* tris, int device_ptrs);
+int lrt_scene_sync(lrt_scene* scene);
+/* mean duration (ms) of the closest-hit kernel launches since the last reset,
+ * timed with HIP events on the scene's stream */
+int lrt_scene_kernel_timing(lrt_scene* scene, int kernel, int reset,
+                            double* total_ms, uint64_t* launches, uint64_t* items);
+enum { LRT_KERNEL_CLOSEST = 0, LRT_KERNEL_OCCLUDED = 1, LRT_KERNEL_SHADE = 2, LRT_KERNEL_COUNT_ = 3 };
+
+/* ---- BSDF batch evaluation (bsdf.h:78-88) -----------------------------
+ * For n tuples i: material index mat[i] into the scene's (or the given)
+ * material table, local wi/wo (3 floats each), front_face; outputs f*cos
+ * (3 floats) and pdf for the given measure and transport mode.
+ * component = -1 and type_mask = 0xffffffff reproduce the path integrator. */
+int lrt_bsdf_eval_pdf(int hip_device, const lrt_material* materials, uint32_t n_materials,
+                      const float* tables, uint32_t n_tables,
+                      const int32_t* mat, const float* wi, const float* wo,
+                      const uint8_t* front_face, uint32_t n,
+                      int measure, int mode, int component, uint32_t type_mask,
+                      float* f_cos, float* pdf);
+/* bsdf::sample for n tuples with sample u (2 floats) and lobe sample
+ * (ctx.lobe_sample); outputs weight (3), wo (3), pdf, eta, sampled_type,
+ * sampled_component. */
+int lrt_bsdf_sample(int hip_device, const lrt_material* materials, uint32_t n_materials,
+                    const float* tables, uint32_t n_tables,
+                    const int32_t* mat, const float* wi, const uint8_t* front_face,
+                    const float* u, const float* lobe_u, uint32_t n,
+                    int mode, int component, uint32_t type_mask,
+                    float* weight, float* wo, float* pdf, float* eta,
+                    uint32_t* sampled_type, int32_t* sampled_component);
+/* bsdf::flags / component_count for materials[index] (bsdf.h:58-62) */
+int lrt_bsdf_flags(const lrt_material* materials, uint32_t n_materials, int32_t index,
+                   uint32_t* flags, int32_t* component_count);
+
+/* ---- host precompute the reference runs in material constructors ------- */
+/* fresnel_diffuse_reflectance(eta): adaptive Gauss-Lobatto, FP32
+ * (fresnel.h:128-133, gauss_lobatto.h:23-121) */
+int lrt_fresnel_diffuse_reflectance(float eta, float* out);
+/* rough_transmittance(eta, alpha) slice from a ggx.dat image held in memory
+ * (rough_transmittance.h:17-24, 67-185): out[0..99] theta samples,
+ * out[100] diffuse transmittance */
+int lrt_rough_transmittance_slice(const void* table_file, size_t table_bytes,
+                                  float eta, float alpha, float* out101);
+
+/* ---- sky emitter evaluation (env_emitter.h:71-88) ---------------------
+ * rgb radiance of n world directions for light lights[0] of a scene
+ * description (device evaluation) */
+int lrt_sky_eval(int hip_device, const lrt_light_desc* sky, const float* dirs, uint32_t n, float* rgb);
+
+/* ---- path tracer (path_integrator.h:18-170 over src/main.cpp:103-147) --- */
+typedef struct lrt_render_params {
+    uint32_t width, height;
+    uint32_t spp;          /* samples per pixel rendered by this call */
+    uint32_t spp_offset;   /* index of the first sample (progressive / resume) */
+    uint32_t max_depth;    /* path_integrator.h:173 (reference: 50) */
+    uint32_t rr_depth;     /* path_integrator.h:19  (reference: 5) */
+    uint64_t seed;         /* Philox-4x32-10 key */
+    uint32_t row_begin, row_end; /* rows [begin, end) of the image; 0,0 = all */
+    uint32_t flags;        /* LRT_RENDER_COUNT: also count traversal work (stats) */
+} lrt_render_params;
+enum { LRT_RENDER_COUNT = 1 };
+
+typedef struct lrt_render_stats {
+    uint64_t samples, extension_rays, shadow_rays, segments;
+    double seconds;        /* wall time of the render loop */
+    double trace_ms, occluded_ms, shade_ms; /* summed kernel time, HIP events */
+    uint64_t trace_launches, occluded_launches;
+    /* with LRT_RENDER_COUNT: extension-ray traversal work (nodes visited,
+       leaf references read, triangle tests), from a separate counting kernel */
+    uint64_t nodes_visited, refs_read, tris_tested;
+} lrt_render_stats;
+
+/* Renders and ADDS the per-pixel sum of sample radiance into sum_rgb
+ * (3*W*H doubles, row y=0 at the bottom like src/main.cpp:122) and, if
+ * sumsq_rgb is non-null, the per-pixel sum of squares (for standard errors).
+ * device_ptrs: both buffers are device memory. */
+int lrt_render(lrt_scene* scene, const lrt_render_params* params,
+               double* sum_rgb, double* sumsq_rgb, int device_ptrs,
+               lrt_render_stats* stats);
+
+/* ---- seeded scene generators for the benchmark configurations --------- */
+/* Generated geometry lives in a library-owned lrt_generated; its desc()
+ * pointer stays valid until lrt_generated_free. */
+typedef struct lrt_generated lrt_generated;
+enum {
+    LRT_GEN_BOX_ICOSPHERE = 0,   /* closed box room + icosphere L5 + ceiling light */
+    LRT_GEN_DISPLACED_SPHERE = 1 /* displaced sphere, 7 material regions, ground, sky */
+};
+typedef struct lrt_gen_params {
+    int32_t kind;
+    uint64_t seed;
+    uint32_t segments, rings;    /* displaced sphere tessellation (1024 x 512 -> 1,048,576 tris) */
+    uint32_t subdivisions;       /* icosphere level */
+    float aspect;                /* camera aspect ratio */
+} lrt_gen_params;
+int lrt_generate_scene(const lrt_gen_params* params, const float* tables, uint32_t n_tables,
+                       const lrt_light_desc* sky, lrt_generated** out);
+const lrt_scene_desc* lrt_generated_desc(const lrt_generated* g);
+int lrt_generated_free(lrt_generated* g);
+
+/* ---- Philox-4x32-10 (the counter-based sampler of every kernel) -------- */
+int lrt_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LINRT_H */

@@ -1,0 +1,873 @@
+// liblinrt_hip.so: the C ABI (include/linrt.h) over the HIP kernels.
+// Host work here: validating descriptions, kd-tree build and flattening,
+// device uploads, launch sequencing and HIP-event timing.
+#include <chrono>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "host/host.hpp"
+#include "kernels.hip"
+
+namespace lrt {
+
+struct Generated;
+Generated* generate(const lrt_gen_params& p, const float* tables, uint32_t n_tables, const lrt_light_desc* sky);
+void free_generated(Generated* g);
+const lrt_scene_desc* generated_desc(const Generated* g);
+
+namespace {
+thread_local std::string g_error;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(x)                                                                                     \
+    do {                                                                                                \
+        const hipError_t e_ = (x);                                                                      \
+        if (e_ != hipSuccess) throw Error(LRT_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename F>
+int guarded(F&& f) {
+    try {
+        f();
+        return LRT_OK;
+    } catch (const Error& e) {
+        g_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_error = "host allocation failed";
+        return LRT_ERR_ALLOC;
+    } catch (const std::exception& e) {
+        g_error = e.what();
+        return LRT_ERR_ARG;
+    } catch (...) {
+        g_error = "unknown error";
+        return LRT_ERR_INTERNAL;
+    }
+}
+void require(bool c, const std::string& msg, int code = LRT_ERR_ARG) {
+    if (!c) throw Error(code, msg);
+}
+
+// device buffer owned by RAII
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t b) {
+        if (b <= bytes && p) return;
+        release();
+        if (b == 0) return;
+        const hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) throw Error(LRT_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+        bytes = b;
+    }
+    template <typename T>
+    T* as() const { return (T*)p; }
+};
+
+void upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s) {
+    b.alloc(bytes);
+    if (bytes) HIPCHECK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s));
+}
+
+int cu_count(int dev) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return 256;
+    return prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+}
+
+void check_materials(const lrt_material* m, uint32_t nm, uint32_t n_tables) {
+    for (uint32_t i = 0; i < nm; ++i) {
+        const lrt_material& M = m[i];
+        require(M.type >= LRT_BSDF_NONE && M.type <= LRT_BSDF_ROUGH_COATING, "material " + std::to_string(i) + ": unknown type");
+        if (M.type == LRT_BSDF_COATING || M.type == LRT_BSDF_ROUGH_COATING) {
+            require(M.nested >= 0 && (uint32_t)M.nested < nm, "material " + std::to_string(i) + ": nested index out of range");
+            const int nt = m[M.nested].type;
+            require(nt != LRT_BSDF_COATING && nt != LRT_BSDF_ROUGH_COATING && nt != LRT_BSDF_NONE,
+                    "material " + std::to_string(i) + ": a coating must nest a base BSDF", LRT_ERR_UNSUPPORTED);
+        }
+        if (M.type == LRT_BSDF_ROUGH_PLASTIC)
+            require(M.table_ext >= 0 && (uint32_t)M.table_ext < n_tables && M.table_int >= 0 && (uint32_t)M.table_int < n_tables,
+                    "material " + std::to_string(i) + ": rough plastic needs table_ext and table_int");
+        if (M.type == LRT_BSDF_ROUGH_COATING)
+            require(M.table_ext >= 0 && (uint32_t)M.table_ext < n_tables,
+                    "material " + std::to_string(i) + ": rough coating needs table_ext");
+    }
+}
+
+DevLight make_light(const lrt_light_desc& d, const std::vector<lrt_object_desc>& objs) {
+    DevLight L;
+    std::memset(&L, 0, sizeof(L));
+    L.kind = d.kind;
+    L.object = d.object;
+    for (int c = 0; c < 3; ++c) L.radiance[c] = d.radiance[c];
+    if (d.kind == LRT_LIGHT_SKY) {
+        // env_emitter.h:13-30
+        f3 s = ld3(d.sun_dir);
+        const float l = std::sqrt(s.x * s.x + s.y * s.y + s.z * s.z);
+        s = f3{s.x / l, s.y / l, s.z / l};
+        L.sun[0] = s.x; L.sun[1] = s.y; L.sun[2] = s.z;
+        const float l2 = std::sqrt(s.x * s.x + s.y * s.y + s.z * s.z);
+        const f3 n{s.x / l2, s.y / l2, s.z / l2};
+        const f3 a = (std::fabs(n.x) > 0.9f) ? f3{0.0f, 1.0f, 0.0f} : f3{1.0f, 0.0f, 0.0f};
+        f3 t = cross(n, a);
+        const float lt = std::sqrt(t.x * t.x + t.y * t.y + t.z * t.z);
+        t = f3{t.x / lt, t.y / lt, t.z / lt};
+        const f3 sx = cross(n, t);
+        L.fs[0] = sx.x; L.fs[1] = sx.y; L.fs[2] = sx.z;
+        L.ft[0] = t.x; L.ft[1] = t.y; L.ft[2] = t.z;
+        L.fn[0] = n.x; L.fn[1] = n.y; L.fn[2] = n.z;
+        L.sun_active = s.y > 0.0f;
+        const double radius = 0.255 * 3.14159265358979323846 / 180.0;
+        const double cosine = std::cos(radius);
+        L.cos_cutoff = (float)cosine;
+        L.cone_pdf = (float)(1.0 / (2.0 * 3.14159265358979323846 * (1.0 - cosine)));
+        L.scale = d.scale;
+        std::memcpy(L.cfg, d.config, sizeof(L.cfg));
+        std::memcpy(L.rad, d.channel_radiance, sizeof(L.rad));
+    } else {
+        require(d.object >= 0 && (size_t)d.object < objs.size(), "area light object out of range");
+        require(objs[(size_t)d.object].shape == LRT_SHAPE_RECTANGLE,
+                "area lights are supported on rectangle shapes only", LRT_ERR_UNSUPPORTED);
+    }
+    return L;
+}
+
+// HIP-event timing of one kernel kind
+struct Timer {
+    std::vector<hipEvent_t> pool;
+    size_t used = 0;
+    double total_ms = 0.0;
+    uint64_t launches = 0, items = 0;
+    ~Timer() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+    std::pair<hipEvent_t, hipEvent_t> next() {
+        while (pool.size() < used + 2) {
+            hipEvent_t e;
+            HIPCHECK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        used += 2;
+        return {pool[used - 2], pool[used - 1]};
+    }
+    void collect() {  // after a stream sync
+        for (size_t i = 0; i + 1 < used; i += 2) {
+            float ms = 0.0f;
+            HIPCHECK(hipEventElapsedTime(&ms, pool[i], pool[i + 1]));
+            total_ms += ms;
+            launches++;
+        }
+        used = 0;
+    }
+};
+
+}  // namespace
+}  // namespace lrt
+
+using namespace lrt;
+
+struct lrt_scene {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cus = 256;
+    DevScene S;
+    DevLights Ls;
+    Mats M;
+    CameraDev cam;
+    DevBuf nodes, refs, tris, cnormals, trimat, mats, tables;
+    lrt_scene_info info;
+    std::vector<lrt_object_desc> objects;  // shapes only (pointers not kept)
+    // path-tracer work buffers
+    size_t cap = 0;
+    DevBuf pstate, hits, shadow, queues, counters, stats, accum, accum2, totals;
+    // ray-query staging
+    DevBuf q_rays, q_out0, q_out1, q_out2, q_aux;
+    Timer timers[LRT_KERNEL_COUNT_];
+    ~lrt_scene() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
+    require(d.n_objects <= (uint32_t)kMaxObjects, "too many objects (max " + std::to_string(kMaxObjects) + ")", LRT_ERR_UNSUPPORTED);
+    require(d.n_lights <= (uint32_t)kMaxLights, "too many lights (max " + std::to_string(kMaxLights) + ")", LRT_ERR_UNSUPPORTED);
+    require(d.n_objects == 0 || d.objects, "objects is null");
+    require(d.n_materials == 0 || d.materials, "materials is null");
+    require(d.n_tables == 0 || d.tables, "tables is null");
+    check_materials(d.materials, d.n_materials, d.n_tables);
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<uint32_t> words, refs;
+    std::vector<float> tris, cn;
+    std::vector<uint16_t> tmat;
+    bool any_tmat = false;
+    int max_depth = 0;
+    std::memset(&sc.S, 0, sizeof(sc.S));
+    sc.objects.assign(d.objects, d.objects + d.n_objects);
+    for (uint32_t o = 0; o < d.n_objects; ++o) {
+        const lrt_object_desc& od = d.objects[o];
+        DevObject& ob = sc.S.obj[o];
+        std::memset(&ob, 0, sizeof(ob));
+        ob.shape = od.shape;
+        ob.material = od.shape == LRT_SHAPE_RECTANGLE ? -1 : od.material;
+        ob.emitter = od.emitter;
+        require(ob.material < (int)d.n_materials, "object " + std::to_string(o) + ": material out of range");
+        require(od.emitter < (int)d.n_lights, "object " + std::to_string(o) + ": emitter out of range");
+        sc.objects[o].positions = nullptr;
+        sc.objects[o].indices = nullptr;
+        sc.objects[o].corner_normals = nullptr;
+        sc.objects[o].triangle_material = nullptr;
+        if (od.shape == LRT_SHAPE_MESH) {
+            require(od.n_triangles == 0 || (od.positions && od.indices), "object " + std::to_string(o) + ": mesh arrays are null");
+            const uint32_t nt = od.n_triangles;
+            ob.tri_base = (uint32_t)(tris.size() / 9);
+            ob.n_tris = nt;
+            ob.per_tri_material = od.triangle_material ? 1 : 0;
+            std::vector<float> boxes(6 * (size_t)nt);
+            for (uint32_t t = 0; t < nt; ++t) {
+                const uint32_t i0 = od.indices[3 * t], i1 = od.indices[3 * t + 1], i2 = od.indices[3 * t + 2];
+                require(i0 < od.n_vertices && i1 < od.n_vertices && i2 < od.n_vertices,
+                        "object " + std::to_string(o) + ": vertex index out of range");
+                const f3 p0 = ld3(od.positions + 3 * (size_t)i0), p1 = ld3(od.positions + 3 * (size_t)i1),
+                         p2 = ld3(od.positions + 3 * (size_t)i2);
+                f3 e1 = p1 - p0, e2 = p2 - p0;
+                if (len2(cross(e1, e2)) <= 0.0f) e1 = e2 = splat(0.0f);  // triangle_mesh.h:52-54
+                const float rec[9] = {p0.x, p0.y, p0.z, e1.x, e1.y, e1.z, e2.x, e2.y, e2.z};
+                tris.insert(tris.end(), rec, rec + 9);
+                for (int k = 0; k < 9; ++k) cn.push_back(od.corner_normals ? od.corner_normals[9 * (size_t)t + k] : 0.0f);
+                float* b = &boxes[6 * (size_t)t];
+                for (int a = 0; a < 3; ++a) {
+                    const float v0 = comp(p0, a), v1 = comp(p1, a), v2 = comp(p2, a);
+                    b[a] = std::min(std::min(v0, v1), v2);
+                    b[3 + a] = std::max(std::max(v0, v1), v2);
+                }
+                if (od.triangle_material) {
+                    require(od.triangle_material[t] < d.n_materials, "object " + std::to_string(o) + ": triangle material out of range");
+                    tmat.push_back(od.triangle_material[t]);
+                    any_tmat = true;
+                } else {
+                    tmat.push_back((uint16_t)std::max(0, ob.material));
+                }
+            }
+            KdTree kt;
+            kd_build(boxes.data(), nt, kt);
+            FlatTree ft;
+            kd_flatten(kt, ft);
+            require(ft.max_depth <= kMaxStack, "kd-tree deeper than the traversal stack");
+            max_depth = std::max(max_depth, ft.max_depth);
+            ob.node_base = (uint32_t)(words.size() / 2);
+            const uint32_t ref_base = (uint32_t)refs.size();
+            for (size_t k = 0; k < ft.words.size(); k += 2) {
+                uint32_t w0 = ft.words[k];
+                if ((ft.words[k + 1] & 3u) == 3u) w0 += ref_base;
+                words.push_back(w0);
+                words.push_back(ft.words[k + 1]);
+            }
+            for (uint32_t r : ft.refs) refs.push_back(ob.tri_base + r);
+            if (nt == 0) {  // empty mesh: one empty leaf
+                ob.node_base = (uint32_t)(words.size() / 2);
+                words.push_back(0);
+                words.push_back(3u);
+            }
+        } else if (od.shape == LRT_SHAPE_RECTANGLE) {
+            const f3 eu = ld3(od.edge_u), ev = ld3(od.edge_v);
+            const f3 n = cross(eu, ev);
+            const float area = std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+            for (int c = 0; c < 3; ++c) {
+                ob.corner[c] = od.corner[c];
+                ob.eu[c] = od.edge_u[c];
+                ob.ev[c] = od.edge_v[c];
+                ob.rn[c] = area > 0.0f ? comp(n, c) / area : 0.0f;
+            }
+            ob.area = area;
+        } else if (od.shape == LRT_SHAPE_SPHERE) {
+            for (int c = 0; c < 3; ++c) ob.center[c] = od.center[c];
+            ob.radius = od.radius;
+            ob.area = 4.0f * kPiF * od.radius * od.radius;
+        } else {
+            throw Error(LRT_ERR_ARG, "object " + std::to_string(o) + ": unknown shape");
+        }
+    }
+    const auto t1 = std::chrono::steady_clock::now();
+    if (words.empty()) { words.push_back(0); words.push_back(3u); }
+    if (refs.empty()) refs.push_back(0);
+    if (tris.empty()) tris.assign(9, 0.0f);
+    if (cn.empty()) cn.assign(9, 0.0f);
+    if (tmat.empty()) tmat.push_back(0);
+    upload(sc.nodes, words.data(), words.size() * 4, sc.stream);
+    upload(sc.refs, refs.data(), refs.size() * 4, sc.stream);
+    upload(sc.tris, tris.data(), tris.size() * 4, sc.stream);
+    upload(sc.cnormals, cn.data(), cn.size() * 4, sc.stream);
+    upload(sc.trimat, tmat.data(), tmat.size() * 2, sc.stream);
+    std::vector<lrt_material> mats(d.materials, d.materials + d.n_materials);
+    if (mats.empty()) { lrt_material z; std::memset(&z, 0, sizeof(z)); mats.push_back(z); }
+    upload(sc.mats, mats.data(), mats.size() * sizeof(lrt_material), sc.stream);
+    std::vector<float> tb(d.tables ? d.tables : nullptr, d.tables ? d.tables + (size_t)d.n_tables * 101 : nullptr);
+    if (tb.empty()) tb.assign(101, 0.0f);
+    upload(sc.tables, tb.data(), tb.size() * 4, sc.stream);
+    HIPCHECK(hipStreamSynchronize(sc.stream));
+    (void)any_tmat;
+    sc.S.nodes = sc.nodes.as<uint2>();
+    sc.S.refs = sc.refs.as<uint32_t>();
+    sc.S.tris = sc.tris.as<float>();
+    sc.S.cnormals = sc.cnormals.as<float>();
+    sc.S.tri_mat = sc.trimat.as<uint16_t>();
+    sc.S.n_objects = (int32_t)d.n_objects;
+    sc.S.stack_depth = max_depth;
+    sc.M.m = sc.mats.as<lrt_material>();
+    sc.M.tables = sc.tables.as<float>();
+    std::memset(&sc.Ls, 0, sizeof(sc.Ls));
+    sc.Ls.n = (int32_t)d.n_lights;
+    for (uint32_t i = 0; i < d.n_lights; ++i) sc.Ls.l[i] = make_light(d.lights[i], sc.objects);
+    for (uint32_t o = 0; o < d.n_objects; ++o)
+        if (d.objects[o].emitter >= 0) {
+            const lrt_light_desc& L = d.lights[d.objects[o].emitter];
+            require(L.kind == LRT_LIGHT_AREA && L.object == (int)o, "object " + std::to_string(o) + ": emitter does not name this object");
+        }
+    const CameraBasis cb = camera_basis(d.camera);
+    std::memcpy(&sc.cam, &cb, sizeof(sc.cam));
+    const auto t2 = std::chrono::steady_clock::now();
+    std::memset(&sc.info, 0, sizeof(sc.info));
+    sc.info.n_triangles = tris.size() / 9;
+    sc.info.n_nodes = words.size() / 2;
+    sc.info.n_refs = refs.size();
+    sc.info.device_bytes = sc.nodes.bytes + sc.refs.bytes + sc.tris.bytes + sc.cnormals.bytes + sc.trimat.bytes;
+    sc.info.max_tree_depth = max_depth;
+    sc.info.build_seconds = std::chrono::duration<double>(t1 - t0).count();
+    sc.info.upload_seconds = std::chrono::duration<double>(t2 - t1).count();
+}
+
+uint32_t grid_for(const lrt_scene& sc, uint64_t n, int per_cu) {
+    const uint64_t need = (n + kBlock - 1) / kBlock;
+    const uint64_t cap = (uint64_t)sc.cus * (uint64_t)per_cu;
+    return (uint32_t)std::max<uint64_t>(1, std::min(need, cap));
+}
+
+template <typename T>
+T* alloc_carve(char*& p, size_t n) {
+    T* r = (T*)p;
+    p += ((n * sizeof(T) + 255) / 256) * 256;
+    return r;
+}
+
+void stage_rays(lrt_scene& sc, const float* rays, uint32_t n, int device_ptrs, const float*& d_rays) {
+    if (device_ptrs) { d_rays = rays; return; }
+    upload(sc.q_rays, rays, (size_t)n * 8 * 4, sc.stream);
+    d_rays = sc.q_rays.as<float>();
+}
+
+// ---- BSDF and sky batches (host arrays) ---------------------------------------
+struct BatchDev {
+    hipStream_t s = nullptr;
+    std::vector<void*> bufs;
+    explicit BatchDev(int dev) {
+        HIPCHECK(hipSetDevice(dev));
+        HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    }
+    ~BatchDev() {
+        if (s) (void)hipStreamSynchronize(s);
+        for (void* p : bufs) (void)hipFree(p);
+        if (s) (void)hipStreamDestroy(s);
+    }
+    template <typename T>
+    T* put(const T* src, size_t count) {
+        void* p = nullptr;
+        const size_t bytes = std::max<size_t>(count * sizeof(T), 4);
+        if (hipMalloc(&p, bytes) != hipSuccess) throw Error(LRT_ERR_ALLOC, "hipMalloc failed");
+        bufs.push_back(p);
+        if (src && count) HIPCHECK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+        return (T*)p;
+    }
+    template <typename T>
+    void get(T* dst, const T* src, size_t count) {
+        HIPCHECK(hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* lrt_last_error(void) { return g_error.c_str(); }
+int lrt_version(void) { return 1; }
+int lrt_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int lrt_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    return guarded([&] {
+        require(ctr && key && out, "null pointer");
+        philox(ctr, key, out);
+    });
+}
+
+int lrt_kdtree_build_export(const float* boxes, uint32_t n, uint32_t* n_nodes, uint32_t* n_refs, int32_t* depth_limit,
+                            uint8_t* leaf, int32_t* axis, float* split, int32_t* left, int32_t* right, int32_t* first,
+                            int32_t* count, uint32_t* refs) {
+    return guarded([&] {
+        require(n == 0 || boxes, "boxes is null");
+        KdTree t;
+        kd_build(boxes, n, t);
+        if (n_nodes) *n_nodes = (uint32_t)t.nodes.size();
+        if (n_refs) *n_refs = (uint32_t)t.refs.size();
+        if (depth_limit) *depth_limit = t.depth_limit;
+        if (leaf) {
+            require(axis && split && left && right && first && count && refs, "output arrays are null");
+            for (size_t i = 0; i < t.nodes.size(); ++i) {
+                const KdNode& nd = t.nodes[i];
+                leaf[i] = nd.leaf; axis[i] = nd.axis; split[i] = nd.split; left[i] = nd.left;
+                right[i] = nd.right; first[i] = nd.first; count[i] = nd.count;
+            }
+            if (!t.refs.empty()) std::memcpy(refs, t.refs.data(), t.refs.size() * 4);
+        }
+    });
+}
+
+int lrt_fresnel_diffuse_reflectance(float eta, float* out) {
+    return guarded([&] {
+        require(out != nullptr, "out is null");
+        *out = fresnel_diffuse_reflectance(eta);
+    });
+}
+
+int lrt_rough_transmittance_slice(const void* table_file, size_t table_bytes, float eta, float alpha, float* out101) {
+    return guarded([&] {
+        require(table_file && out101, "null pointer");
+        rough_transmittance_slice(table_file, table_bytes, eta, alpha, out101);
+    });
+}
+
+int lrt_scene_create(int hip_device, const lrt_scene_desc* desc, lrt_scene** out) {
+    return guarded([&] {
+        require(desc && out, "null pointer");
+        *out = nullptr;
+        int ndev = 0;
+        require(hipGetDeviceCount(&ndev) == hipSuccess && hip_device >= 0 && hip_device < ndev,
+                "no HIP device " + std::to_string(hip_device), LRT_ERR_HIP);
+        HIPCHECK(hipSetDevice(hip_device));
+        auto sc = std::make_unique<lrt_scene>();
+        sc->device = hip_device;
+        sc->cus = cu_count(hip_device);
+        HIPCHECK(hipStreamCreateWithFlags(&sc->stream, hipStreamNonBlocking));
+        build_scene(*sc, *desc);
+        *out = sc.release();
+    });
+}
+
+int lrt_scene_release(lrt_scene* scene) {
+    return guarded([&] {
+        if (!scene) return;
+        (void)hipSetDevice(scene->device);
+        (void)hipStreamSynchronize(scene->stream);
+        delete scene;
+    });
+}
+
+int lrt_scene_info_get(const lrt_scene* scene, lrt_scene_info* info) {
+    return guarded([&] {
+        require(scene && info, "null pointer");
+        *info = scene->info;
+    });
+}
+
+int lrt_scene_sync(lrt_scene* scene) {
+    return guarded([&] {
+        require(scene, "null scene");
+        HIPCHECK(hipSetDevice(scene->device));
+        HIPCHECK(hipStreamSynchronize(scene->stream));
+        for (auto& t : scene->timers) t.collect();
+    });
+}
+
+int lrt_scene_kernel_timing(lrt_scene* scene, int kernel, int reset, double* total_ms, uint64_t* launches, uint64_t* items) {
+    return guarded([&] {
+        require(scene && kernel >= 0 && kernel < LRT_KERNEL_COUNT_, "bad arguments");
+        HIPCHECK(hipSetDevice(scene->device));
+        HIPCHECK(hipStreamSynchronize(scene->stream));
+        Timer& t = scene->timers[kernel];
+        t.collect();
+        if (total_ms) *total_ms = t.total_ms;
+        if (launches) *launches = t.launches;
+        if (items) *items = t.items;
+        if (reset) { t.total_ms = 0.0; t.launches = 0; t.items = 0; }
+    });
+}
+
+// ---- ray queries -------------------------------------------------------------
+int lrt_trace_closest(lrt_scene* sc, const float* rays, uint32_t n, int32_t* hit_obj_prim, float* hit_t, float* hit_uv,
+                      int device_ptrs) {
+    return guarded([&] {
+        require(sc && rays && hit_obj_prim && hit_t, "null pointer");
+        if (n == 0) return;
+        HIPCHECK(hipSetDevice(sc->device));
+        const float* d_rays;
+        stage_rays(*sc, rays, n, device_ptrs, d_rays);
+        int32_t* d_op = hit_obj_prim;
+        float* d_t = hit_t;
+        float* d_uv = hit_uv;
+        if (!device_ptrs) {
+            sc->q_out0.alloc((size_t)n * 8);
+            sc->q_out1.alloc((size_t)n * 4);
+            d_op = sc->q_out0.as<int32_t>();
+            d_t = sc->q_out1.as<float>();
+            if (hit_uv) { sc->q_out2.alloc((size_t)n * 8); d_uv = sc->q_out2.as<float>(); }
+        }
+        Timer& tm = sc->timers[LRT_KERNEL_CLOSEST];
+        auto ev = tm.next();
+        HIPCHECK(hipEventRecord(ev.first, sc->stream));
+        hipLaunchKernelGGL(k_trace_closest, dim3(grid_for(*sc, n, 20)), dim3(kBlock), 0, sc->stream, sc->S, d_rays, n, d_op, d_t, d_uv);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(ev.second, sc->stream));
+        tm.items += n;
+        if (!device_ptrs) {
+            HIPCHECK(hipMemcpyAsync(hit_obj_prim, d_op, (size_t)n * 8, hipMemcpyDeviceToHost, sc->stream));
+            HIPCHECK(hipMemcpyAsync(hit_t, d_t, (size_t)n * 4, hipMemcpyDeviceToHost, sc->stream));
+            if (hit_uv) HIPCHECK(hipMemcpyAsync(hit_uv, d_uv, (size_t)n * 8, hipMemcpyDeviceToHost, sc->stream));
+            HIPCHECK(hipStreamSynchronize(sc->stream));
+        }
+    });
+}
+
+int lrt_trace_occluded(lrt_scene* sc, const float* rays, uint32_t n, const int32_t* ignore_emitter, uint8_t* occluded,
+                       int device_ptrs) {
+    return guarded([&] {
+        require(sc && rays && occluded, "null pointer");
+        if (n == 0) return;
+        HIPCHECK(hipSetDevice(sc->device));
+        const float* d_rays;
+        stage_rays(*sc, rays, n, device_ptrs, d_rays);
+        const int32_t* d_ign = ignore_emitter;
+        uint8_t* d_out = occluded;
+        if (!device_ptrs) {
+            if (ignore_emitter) { upload(sc->q_aux, ignore_emitter, (size_t)n * 4, sc->stream); d_ign = sc->q_aux.as<int32_t>(); }
+            sc->q_out0.alloc(n);
+            d_out = sc->q_out0.as<uint8_t>();
+        }
+        Timer& tm = sc->timers[LRT_KERNEL_OCCLUDED];
+        auto ev = tm.next();
+        HIPCHECK(hipEventRecord(ev.first, sc->stream));
+        hipLaunchKernelGGL(k_trace_occluded, dim3(grid_for(*sc, n, 20)), dim3(kBlock), 0, sc->stream, sc->S, d_rays, n, d_ign, d_out);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipEventRecord(ev.second, sc->stream));
+        tm.items += n;
+        if (!device_ptrs) {
+            HIPCHECK(hipMemcpyAsync(occluded, d_out, n, hipMemcpyDeviceToHost, sc->stream));
+            HIPCHECK(hipStreamSynchronize(sc->stream));
+        }
+    });
+}
+
+int lrt_trace_count(lrt_scene* sc, const float* rays, uint32_t n, uint32_t* nodes, uint32_t* refs, uint32_t* tris,
+                    int device_ptrs) {
+    return guarded([&] {
+        require(sc && rays && nodes && refs && tris, "null pointer");
+        if (n == 0) return;
+        HIPCHECK(hipSetDevice(sc->device));
+        const float* d_rays;
+        stage_rays(*sc, rays, n, device_ptrs, d_rays);
+        uint32_t *a = nodes, *b = refs, *c = tris;
+        if (!device_ptrs) {
+            sc->q_out0.alloc((size_t)n * 4);
+            sc->q_out1.alloc((size_t)n * 4);
+            sc->q_out2.alloc((size_t)n * 4);
+            a = sc->q_out0.as<uint32_t>(); b = sc->q_out1.as<uint32_t>(); c = sc->q_out2.as<uint32_t>();
+        }
+        hipLaunchKernelGGL(k_trace_count, dim3(grid_for(*sc, n, 20)), dim3(kBlock), 0, sc->stream, sc->S, d_rays, n, a, b, c);
+        HIPCHECK(hipGetLastError());
+        if (!device_ptrs) {
+            HIPCHECK(hipMemcpyAsync(nodes, a, (size_t)n * 4, hipMemcpyDeviceToHost, sc->stream));
+            HIPCHECK(hipMemcpyAsync(refs, b, (size_t)n * 4, hipMemcpyDeviceToHost, sc->stream));
+            HIPCHECK(hipMemcpyAsync(tris, c, (size_t)n * 4, hipMemcpyDeviceToHost, sc->stream));
+            HIPCHECK(hipStreamSynchronize(sc->stream));
+        }
+    });
+}
+
+int lrt_bsdf_eval_pdf(int hip_device, const lrt_material* materials, uint32_t n_materials, const float* tables,
+                      uint32_t n_tables, const int32_t* mat, const float* wi, const float* wo, const uint8_t* front_face,
+                      uint32_t n, int measure, int mode, int component, uint32_t type_mask, float* f_cos, float* pdf) {
+    return guarded([&] {
+        require(materials && mat && wi && wo && front_face && f_cos && pdf, "null pointer");
+        require(n_tables == 0 || tables, "tables is null");
+        check_materials(materials, n_materials, n_tables);
+        for (uint32_t i = 0; i < n; ++i) require(mat[i] >= 0 && (uint32_t)mat[i] < n_materials, "material index out of range");
+        if (n == 0) return;
+        BatchDev B(hip_device);
+        Mats M{B.put(materials, n_materials), B.put(tables, (size_t)n_tables * 101)};
+        const int32_t* dm = B.put(mat, n);
+        const float* dwi = B.put(wi, 3 * (size_t)n);
+        const float* dwo = B.put(wo, 3 * (size_t)n);
+        const uint8_t* df = B.put(front_face, n);
+        float* dfc = B.put<float>(nullptr, 3 * (size_t)n);
+        float* dp = B.put<float>(nullptr, n);
+        hipLaunchKernelGGL(k_bsdf_eval, dim3((n + 255) / 256), dim3(256), 0, B.s, M, dm, dwi, dwo, df, n, measure, mode,
+                           component, type_mask, dfc, dp);
+        HIPCHECK(hipGetLastError());
+        B.get(f_cos, dfc, 3 * (size_t)n);
+        B.get(pdf, dp, n);
+        HIPCHECK(hipStreamSynchronize(B.s));
+    });
+}
+
+int lrt_bsdf_sample(int hip_device, const lrt_material* materials, uint32_t n_materials, const float* tables,
+                    uint32_t n_tables, const int32_t* mat, const float* wi, const uint8_t* front_face, const float* u,
+                    const float* lobe_u, uint32_t n, int mode, int component, uint32_t type_mask, float* weight, float* wo,
+                    float* pdf, float* eta, uint32_t* sampled_type, int32_t* sampled_component) {
+    return guarded([&] {
+        require(materials && mat && wi && front_face && u && lobe_u && weight && wo && pdf && eta && sampled_type &&
+                    sampled_component, "null pointer");
+        require(n_tables == 0 || tables, "tables is null");
+        check_materials(materials, n_materials, n_tables);
+        for (uint32_t i = 0; i < n; ++i) require(mat[i] >= 0 && (uint32_t)mat[i] < n_materials, "material index out of range");
+        if (n == 0) return;
+        BatchDev B(hip_device);
+        Mats M{B.put(materials, n_materials), B.put(tables, (size_t)n_tables * 101)};
+        const int32_t* dm = B.put(mat, n);
+        const float* dwi = B.put(wi, 3 * (size_t)n);
+        const uint8_t* df = B.put(front_face, n);
+        const float* du = B.put(u, 2 * (size_t)n);
+        const float* dl = B.put(lobe_u, n);
+        float* dw = B.put<float>(nullptr, 3 * (size_t)n);
+        float* dwo = B.put<float>(nullptr, 3 * (size_t)n);
+        float* dp = B.put<float>(nullptr, n);
+        float* de = B.put<float>(nullptr, n);
+        uint32_t* dt = B.put<uint32_t>(nullptr, n);
+        int32_t* dc = B.put<int32_t>(nullptr, n);
+        hipLaunchKernelGGL(k_bsdf_sample, dim3((n + 255) / 256), dim3(256), 0, B.s, M, dm, dwi, df, du, dl, n, mode,
+                           component, type_mask, dw, dwo, dp, de, dt, dc);
+        HIPCHECK(hipGetLastError());
+        B.get(weight, dw, 3 * (size_t)n);
+        B.get(wo, dwo, 3 * (size_t)n);
+        B.get(pdf, dp, n);
+        B.get(eta, de, n);
+        B.get(sampled_type, dt, n);
+        B.get(sampled_component, dc, n);
+        HIPCHECK(hipStreamSynchronize(B.s));
+    });
+}
+
+int lrt_bsdf_flags(const lrt_material* materials, uint32_t n_materials, int32_t index, uint32_t* flags, int32_t* component_count) {
+    return guarded([&] {
+        require(materials && flags && component_count && index >= 0 && (uint32_t)index < n_materials, "bad arguments");
+        check_materials(materials, n_materials, 0xffffffffu);
+        auto base = [](const lrt_material& M) -> uint32_t {
+            switch (M.type) {
+                case LRT_BSDF_DIFFUSE: return LRT_FLAG_DIFFUSE | LRT_FLAG_REFLECTIVE;
+                case LRT_BSDF_CONDUCTOR: return M.alpha <= 0.0f ? (LRT_FLAG_SPECULAR | LRT_FLAG_REFLECTIVE | LRT_FLAG_METALLIC)
+                                                                : (LRT_FLAG_GLOSSY | LRT_FLAG_REFLECTIVE | LRT_FLAG_METALLIC);
+                case LRT_BSDF_DIELECTRIC: return LRT_FLAG_SPECULAR | LRT_FLAG_REFLECTIVE | LRT_FLAG_REFRACTIVE | LRT_FLAG_DIELECTRIC;
+                case LRT_BSDF_ROUGH_DIELECTRIC: return LRT_FLAG_GLOSSY | LRT_FLAG_REFLECTIVE | LRT_FLAG_REFRACTIVE | LRT_FLAG_DIELECTRIC;
+                case LRT_BSDF_PLASTIC: return LRT_FLAG_SPECULAR | LRT_FLAG_REFLECTIVE | LRT_FLAG_DIELECTRIC | LRT_FLAG_DIFFUSE;
+                case LRT_BSDF_ROUGH_PLASTIC: return LRT_FLAG_GLOSSY | LRT_FLAG_REFLECTIVE | LRT_FLAG_DIELECTRIC | LRT_FLAG_DIFFUSE;
+                default: return 0u;
+            }
+        };
+        auto count = [](const lrt_material& M) {
+            return (M.type == LRT_BSDF_DIELECTRIC || M.type == LRT_BSDF_PLASTIC || M.type == LRT_BSDF_ROUGH_PLASTIC) ? 2 : 1;
+        };
+        const lrt_material& M = materials[index];
+        if (M.type == LRT_BSDF_COATING || M.type == LRT_BSDF_ROUGH_COATING) {
+            const lrt_material& N = materials[M.nested];
+            *flags = base(N) | (M.type == LRT_BSDF_COATING ? LRT_FLAG_SPECULAR : LRT_FLAG_GLOSSY) | LRT_FLAG_REFLECTIVE |
+                     LRT_FLAG_DIELECTRIC;
+            *component_count = count(N) + 1;
+        } else {
+            *flags = base(M);
+            *component_count = count(M);
+        }
+    });
+}
+
+int lrt_sky_eval(int hip_device, const lrt_light_desc* sky, const float* dirs, uint32_t n, float* rgb) {
+    return guarded([&] {
+        require(sky && dirs && rgb, "null pointer");
+        require(sky->kind == LRT_LIGHT_SKY, "not a sky light");
+        if (n == 0) return;
+        const DevLight L = make_light(*sky, {});
+        BatchDev B(hip_device);
+        const float* dd = B.put(dirs, 3 * (size_t)n);
+        float* dc = B.put<float>(nullptr, 3 * (size_t)n);
+        hipLaunchKernelGGL(k_sky_eval, dim3((n + 255) / 256), dim3(256), 0, B.s, L, dd, n, dc);
+        HIPCHECK(hipGetLastError());
+        B.get(rgb, dc, 3 * (size_t)n);
+        HIPCHECK(hipStreamSynchronize(B.s));
+    });
+}
+
+// ---- path tracer ----------------------------------------------------------------
+int lrt_render(lrt_scene* sc, const lrt_render_params* prm, double* sum_rgb, double* sumsq_rgb, int device_ptrs,
+               lrt_render_stats* stats) {
+    return guarded([&] {
+        require(sc && prm && sum_rgb, "null pointer");
+        const uint32_t W = prm->width, H = prm->height;
+        require(W >= 2 && H >= 2, "image must be at least 2x2");
+        require(prm->max_depth >= 1, "max_depth must be >= 1");
+        const uint32_t rb = prm->row_end > prm->row_begin ? prm->row_begin : 0;
+        const uint32_t re = prm->row_end > prm->row_begin ? prm->row_end : H;
+        require(re <= H, "row range outside the image");
+        const uint64_t n64 = (uint64_t)W * (re - rb);
+        require(n64 < (1ull << 31), "too many paths per pass");
+        const uint32_t n = (uint32_t)n64;
+        HIPCHECK(hipSetDevice(sc->device));
+        const auto t0 = std::chrono::steady_clock::now();
+        // work buffers: path state (14 floats), hits (5), shadow queue (12), two queues
+        if (sc->cap < n) {
+            sc->pstate.alloc((size_t)n * 14 * 4 + 14 * 256);
+            sc->hits.alloc((size_t)n * 5 * 4 + 5 * 256);
+            sc->shadow.alloc((size_t)n * 12 * 4 + 12 * 256);
+            sc->queues.alloc((size_t)n * 2 * 4 + 2 * 256);
+            sc->cap = n;
+        }
+        sc->counters.alloc(256);
+        sc->stats.alloc(256);
+        PathState P;
+        char* c = sc->pstate.as<char>();
+        float** pf[14] = {&P.ox, &P.oy, &P.oz, &P.dx, &P.dy, &P.dz, &P.tr, &P.tg, &P.tb, &P.lr, &P.lg, &P.lb, &P.eta, &P.pdf};
+        for (float** f : pf) *f = alloc_carve<float>(c, n);
+        P.pixel0 = rb * W;
+        HitBuf Hb;
+        c = sc->hits.as<char>();
+        Hb.obj = alloc_carve<int32_t>(c, n);
+        Hb.prim = alloc_carve<int32_t>(c, n);
+        Hb.t = alloc_carve<float>(c, n);
+        Hb.u = alloc_carve<float>(c, n);
+        Hb.v = alloc_carve<float>(c, n);
+        ShadowBuf Sb;
+        c = sc->shadow.as<char>();
+        float** sf[10] = {&Sb.ox, &Sb.oy, &Sb.oz, &Sb.dx, &Sb.dy, &Sb.dz, &Sb.tmax, &Sb.cr, &Sb.cg, &Sb.cb};
+        for (float** f : sf) *f = alloc_carve<float>(c, n);
+        Sb.ignore = alloc_carve<int32_t>(c, n);
+        Sb.path = alloc_carve<uint32_t>(c, n);
+        c = sc->queues.as<char>();
+        uint32_t* qa = alloc_carve<uint32_t>(c, n);
+        uint32_t* qb = alloc_carve<uint32_t>(c, n);
+        uint32_t* counters = sc->counters.as<uint32_t>();
+        unsigned long long* dstats = sc->stats.as<unsigned long long>();
+        // accumulation target
+        double* dsum = sum_rgb;
+        double* dsq = sumsq_rgb;
+        const size_t img = (size_t)W * H * 3;
+        if (!device_ptrs) {
+            sc->accum.alloc(img * 8);
+            HIPCHECK(hipMemcpyAsync(sc->accum.p, sum_rgb, img * 8, hipMemcpyHostToDevice, sc->stream));
+            dsum = sc->accum.as<double>();
+            if (sumsq_rgb) {
+                sc->accum2.alloc(img * 8);
+                HIPCHECK(hipMemcpyAsync(sc->accum2.p, sumsq_rgb, img * 8, hipMemcpyHostToDevice, sc->stream));
+                dsq = sc->accum2.as<double>();
+            }
+        }
+        const unsigned long long zero[4] = {0, 0, 0, 0};
+        HIPCHECK(hipMemcpyAsync(dstats, zero, sizeof(zero), hipMemcpyHostToDevice, sc->stream));
+        const bool counting = (prm->flags & LRT_RENDER_COUNT) != 0;
+        sc->totals.alloc(256);
+        HIPCHECK(hipMemcpyAsync(sc->totals.p, zero, sizeof(zero), hipMemcpyHostToDevice, sc->stream));
+        const uint2 key = make_uint2((uint32_t)prm->seed, (uint32_t)(prm->seed >> 32));
+        const uint32_t g_all = (n + kBlock - 1) / kBlock;
+        const uint32_t g_trace = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc->cus * 20);
+        const uint32_t g_shade = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc->cus * 8);
+        Timer& tc = sc->timers[LRT_KERNEL_CLOSEST];
+        Timer& to = sc->timers[LRT_KERNEL_OCCLUDED];
+        Timer& ts = sc->timers[LRT_KERNEL_SHADE];
+        for (uint32_t s = 0; s < prm->spp; ++s) {
+            const uint32_t smp = prm->spp_offset + s;
+            hipLaunchKernelGGL(k_raygen, dim3(g_all), dim3(kBlock), 0, sc->stream, P, sc->cam, W, H, rb, n, smp, key, qa, counters);
+            uint32_t* q = qa;
+            uint32_t* qn = qb;
+            for (uint32_t depth = 0; depth < prm->max_depth; ++depth) {
+                if (counting)
+                    hipLaunchKernelGGL(k_extend_count, dim3(g_trace), dim3(kBlock), 0, sc->stream, sc->S, P, q, counters,
+                                       sc->totals.as<unsigned long long>());
+                auto e1 = tc.next();
+                HIPCHECK(hipEventRecord(e1.first, sc->stream));
+                hipLaunchKernelGGL(k_extend, dim3(g_trace), dim3(kBlock), 0, sc->stream, sc->S, P, q, counters, Hb);
+                HIPCHECK(hipEventRecord(e1.second, sc->stream));
+                ShadeParams sp{depth, prm->max_depth, prm->rr_depth, smp, key};
+                auto e2 = ts.next();
+                HIPCHECK(hipEventRecord(e2.first, sc->stream));
+                hipLaunchKernelGGL(k_shade, dim3(g_shade), dim3(kBlock), 0, sc->stream, sc->S, sc->Ls, sc->M, P, q, qn,
+                                   counters, Hb, Sb, sp);
+                HIPCHECK(hipEventRecord(e2.second, sc->stream));
+                auto e3 = to.next();
+                HIPCHECK(hipEventRecord(e3.first, sc->stream));
+                hipLaunchKernelGGL(k_shadow, dim3(g_trace), dim3(kBlock), 0, sc->stream, sc->S, P, counters, Sb);
+                HIPCHECK(hipEventRecord(e3.second, sc->stream));
+                hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, sc->stream, counters, dstats);
+                std::swap(q, qn);
+            }
+            hipLaunchKernelGGL(k_accumulate, dim3(g_all), dim3(kBlock), 0, sc->stream, P, n, dsum, dsq);
+            HIPCHECK(hipGetLastError());
+        }
+        unsigned long long hst[4] = {0, 0, 0, 0}, work[4] = {0, 0, 0, 0};
+        HIPCHECK(hipMemcpyAsync(hst, dstats, sizeof(hst), hipMemcpyDeviceToHost, sc->stream));
+        HIPCHECK(hipMemcpyAsync(work, sc->totals.p, sizeof(work), hipMemcpyDeviceToHost, sc->stream));
+        if (!device_ptrs) {
+            HIPCHECK(hipMemcpyAsync(sum_rgb, dsum, img * 8, hipMemcpyDeviceToHost, sc->stream));
+            if (sumsq_rgb) HIPCHECK(hipMemcpyAsync(sumsq_rgb, dsq, img * 8, hipMemcpyDeviceToHost, sc->stream));
+        }
+        HIPCHECK(hipStreamSynchronize(sc->stream));
+        const auto t1 = std::chrono::steady_clock::now();
+        const double before_c = tc.total_ms, before_o = to.total_ms, before_s = ts.total_ms;
+        const uint64_t lc0 = tc.launches, lo0 = to.launches;
+        tc.collect();
+        to.collect();
+        ts.collect();
+        tc.items += hst[0];
+        to.items += hst[1];
+        if (stats) {
+            stats->samples = (uint64_t)n * prm->spp;
+            stats->extension_rays = hst[0];
+            stats->shadow_rays = hst[1];
+            stats->segments = hst[0];
+            stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+            stats->trace_ms = tc.total_ms - before_c;
+            stats->occluded_ms = to.total_ms - before_o;
+            stats->shade_ms = ts.total_ms - before_s;
+            stats->trace_launches = tc.launches - lc0;
+            stats->occluded_launches = to.launches - lo0;
+            stats->nodes_visited = work[0];
+            stats->refs_read = work[1];
+            stats->tris_tested = work[2];
+        }
+    });
+}
+
+// ---- generators ------------------------------------------------------------------
+int lrt_generate_scene(const lrt_gen_params* params, const float* tables, uint32_t n_tables, const lrt_light_desc* sky,
+                       lrt_generated** out) {
+    return guarded([&] {
+        require(params && out, "null pointer");
+        *out = (lrt_generated*)generate(*params, tables, n_tables, sky);
+    });
+}
+const lrt_scene_desc* lrt_generated_desc(const lrt_generated* g) { return g ? generated_desc((const Generated*)g) : nullptr; }
+int lrt_generated_free(lrt_generated* g) {
+    free_generated((Generated*)g);
+    return LRT_OK;
+}
+
+}  // extern "C"
+
+namespace lrt {
+void set_error(const std::string& msg) { g_error = msg; }
+}  // namespace lrt

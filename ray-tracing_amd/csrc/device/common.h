@@ -1,0 +1,69 @@
+// Device math for the render kernels (gfx950).  The whole library is
+// compiled with -ffp-contract=off and IEEE-correct division and square root,
+// and every expression keeps the reference's operation order, so products
+// like the Moller-Trumbore t round exactly as on the reference CPU build.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define LRT_D __device__ __forceinline__
+#define LRT_HD __host__ __device__ __forceinline__
+
+namespace lrt {
+
+constexpr float kPiF = 3.14159265358979323846f;
+constexpr float kInfF = __builtin_huge_valf();
+
+struct f3 {
+    float x, y, z;
+};
+LRT_HD f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+LRT_HD f3 splat(float s) { return f3{s, s, s}; }
+LRT_HD f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+LRT_HD f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+LRT_HD f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+LRT_HD f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+LRT_HD f3 operator*(float s, f3 a) { return f3{a.x * s, a.y * s, a.z * s}; }
+LRT_HD f3 operator*(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+LRT_HD f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+LRT_HD float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+LRT_HD f3 cross(f3 a, f3 b) { return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+LRT_HD float len2(f3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+LRT_D float len(f3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+LRT_D f3 normalize(f3 a) { return a / len(a); }
+LRT_HD bool is_zero(f3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
+LRT_HD float comp(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+LRT_HD f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
+
+// orthonormal basis around a normal (onb.h:8-23)
+struct Frame {
+    f3 s, t, n;
+};
+LRT_D Frame frame_from(f3 nrm) {
+    Frame f;
+    f.n = normalize(nrm);
+    const f3 a = (fabsf(f.n.x) > 0.9f) ? f3{0.0f, 1.0f, 0.0f} : f3{1.0f, 0.0f, 0.0f};
+    f.t = normalize(cross(f.n, a));
+    f.s = cross(f.n, f.t);
+    return f;
+}
+LRT_D f3 to_world(const Frame& f, f3 a) { return a.x * f.s + a.y * f.t + a.z * f.n; }
+LRT_D f3 to_local(const Frame& f, f3 a) { return f3{dot(a, f.s), dot(a, f.t), dot(a, f.n)}; }
+
+// Philox-4x32-10 counter-based generator (Salmon et al., SC'11)
+LRT_D uint4 philox(uint4 c, uint2 k) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
+        k.x += 0x9E3779B9u;
+        k.y += 0xBB67AE85u;
+    }
+    return c;
+}
+LRT_D float u24(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+LRT_D uint32_t pick(uint4 v, uint32_t lane) { return lane == 0 ? v.x : (lane == 1 ? v.y : (lane == 2 ? v.z : v.w)); }
+
+}  // namespace lrt

@@ -1,0 +1,241 @@
+// kd-tree traversal and primitive tests on the device.
+//
+// Traversal restates sah_kdtree::intersect (sah_kdtree.h:61-121) over the
+// 8-byte flattened nodes: front-to-back with a (node, mint) stack, the far
+// child pushed only when the split plane lies inside the current [mint, maxt].
+// A stack entry's maxt is not stored: it always equals the mint of the entry
+// below it (or the initial maxt for the bottom entry), because an entry is
+// pushed with the current maxt and the current maxt is always the mint of the
+// top entry.  Entries live in LDS (kLdsStack per lane) and spill to a private
+// array beyond that.
+#pragma once
+
+#include "common.h"
+
+namespace lrt {
+
+enum : int { kShapeMesh = 0, kShapeRect = 1, kShapeSphere = 2 };
+
+struct DevObject {
+    int32_t shape, material, emitter, per_tri_material;
+    uint32_t node_base, tri_base, n_tris, pad;
+    float corner[3], eu[3], ev[3], rn[3];  // rectangle
+    float center[3], radius;               // sphere
+    float area;
+};
+
+constexpr int kMaxObjects = 16;
+
+struct DevScene {
+    const uint2* nodes;        // flattened kd nodes of every mesh object
+    const uint32_t* refs;      // global triangle index per leaf reference
+    const float* tris;         // 9 floats per triangle: p0, p1-p0, p2-p0 (degenerate: edges zeroed)
+    const float* cnormals;     // 9 floats per triangle: corner normals
+    const uint16_t* tri_mat;   // per triangle material (or null)
+    int32_t n_objects;
+    int32_t stack_depth;       // deepest root-to-leaf path over all trees
+    DevObject obj[kMaxObjects];
+};
+
+constexpr int kLdsStack = 16;
+constexpr int kMaxStack = 48;
+
+struct Ray {
+    f3 o, d;
+};
+
+struct HitInfo {
+    float t;
+    int32_t obj, prim;
+    float u, v;
+};
+
+// Moller-Trumbore restated from triangle_mesh.h:43-79 with the voxel window
+LRT_D bool tri_test(const float* tri, const Ray& r, float t0, float t1, float& t, float& u, float& v) {
+    const f3 p0 = ld3(tri), e1 = ld3(tri + 3), e2 = ld3(tri + 6);
+    const f3 pv = cross(r.d, e2);
+    const float det = dot(e1, pv);
+    if (fabsf(det) < 1e-8f) return false;
+    const float inv = 1.0f / det;
+    const f3 tv = r.o - p0;
+    u = dot(tv, pv) * inv;
+    if (u < 0.0f || u > 1.0f) return false;
+    const f3 qv = cross(tv, e1);
+    v = dot(r.d, qv) * inv;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    t = dot(e2, qv) * inv;
+    return !(t < t0 || t > t1);
+}
+
+// rectangle.h:18-48
+LRT_D bool rect_test(const DevObject& ob, const Ray& r, float tmin, float tmax, float& t, float& a, float& b) {
+    const f3 rn = ld3(ob.rn), c = ld3(ob.corner), eu = ld3(ob.eu), ev = ld3(ob.ev);
+    const float denom = dot(rn, r.d);
+    if (fabsf(denom) < 1e-8f) return false;
+    t = dot(c - r.o, rn) / denom;
+    if (t < tmin || t > tmax) return false;
+    const f3 p = r.o + t * r.d;
+    const f3 d = p - c;
+    const float uu = dot(eu, eu), vv = dot(ev, ev), uv = dot(eu, ev);
+    const float du = dot(d, eu), dv = dot(d, ev);
+    const float det = uu * vv - uv * uv;
+    if (fabsf(det) < 1e-12f) return false;
+    a = (vv * du - uv * dv) / det;
+    b = (uu * dv - uv * du) / det;
+    return !(a < 0.0f || a > 1.0f || b < 0.0f || b > 1.0f);
+}
+
+// sphere.h:21-44
+LRT_D bool sphere_test(const DevObject& ob, const Ray& r, float tmin, float tmax, float& t) {
+    const f3 oc = r.o - ld3(ob.center);
+    const float a = len2(r.d);
+    const float hb = dot(oc, r.d);
+    const float c = len2(oc) - ob.radius * ob.radius;
+    const float disc = hb * hb - a * c;
+    if (disc < 0) return false;
+    const float sq = sqrtf(disc);
+    float root = (-hb - sq) / a;
+    if (root < tmin || root > tmax) {
+        root = (-hb + sq) / a;
+        if (root < tmin || root > tmax) return false;
+    }
+    t = root;
+    return true;
+}
+
+struct Counters {
+    uint32_t nodes, refs, tris;
+};
+
+// One kd-tree, closest hit.  `closest` enters as the object's t_max and
+// leaves as the nearest accepted t.  ANY: stop at the first accepted hit.
+template <bool ANY, bool COUNT>
+LRT_D bool kd_traverse(const DevScene& S, const DevObject& ob, const Ray& r, float inv[3], float tmin,
+                       float& closest, int32_t& prim, float& hu, float& hv, uint2* lds, Counters& cnt) {
+    const float maxt0 = closest;
+    uint2 spill[kMaxStack - kLdsStack];
+    int top = 0;
+    float mint = tmin, maxt = closest;
+    uint32_t node = ob.node_base;
+    const uint32_t stride = blockDim.x;
+    bool found = false;
+    for (;;) {
+        if (closest < mint) break;
+        const uint2 nd = S.nodes[node];
+        if (COUNT) cnt.nodes++;
+        const uint32_t tag = nd.y & 3u;
+        if (tag == 3u) {
+            const uint32_t count = nd.y >> 2;
+            const uint32_t first = nd.x;
+            float limit = fminf(maxt, closest);
+            for (uint32_t k = 0; k < count; ++k) {
+                const uint32_t tri = S.refs[first + k];
+                if (COUNT) { cnt.refs++; cnt.tris++; }
+                float t, u, v;
+                if (tri_test(S.tris + 9 * (size_t)tri, r, mint, limit, t, u, v)) {
+                    closest = t;
+                    prim = (int32_t)(tri - ob.tri_base);
+                    hu = u;
+                    hv = v;
+                    found = true;
+                    if (ANY) return true;
+                }
+                limit = fminf(maxt, closest);
+            }
+            if (top == 0) break;
+            --top;
+            const uint2 e = top < kLdsStack ? lds[top * stride] : spill[top - kLdsStack];
+            node = e.x;
+            mint = __uint_as_float(e.y);
+            if (top == 0) maxt = maxt0;
+            else {
+                const uint2 below = (top - 1) < kLdsStack ? lds[(top - 1) * stride] : spill[top - 1 - kLdsStack];
+                maxt = __uint_as_float(below.y);
+            }
+            continue;
+        }
+        const int axis = (int)tag;
+        const float split = __uint_as_float(nd.x);
+        const float oa = comp(r.o, axis), da = comp(r.d, axis);
+        const float ia = axis == 0 ? inv[0] : (axis == 1 ? inv[1] : inv[2]);
+        const float t_plane = (split - oa) * ia;
+        const bool left_first = oa < split || (oa == split && da <= 0.0f);
+        const uint32_t left = node + 1, right = ob.node_base + (nd.y >> 2);
+        const uint32_t first = left_first ? left : right;
+        const uint32_t second = left_first ? right : left;
+        if (t_plane > maxt || t_plane <= 0.0f) {
+            node = first;
+        } else if (t_plane < mint) {
+            node = second;
+        } else {
+            const uint2 e = make_uint2(second, __float_as_uint(t_plane));
+            if (top < kLdsStack) lds[top * stride] = e;
+            else spill[top - kLdsStack] = e;
+            ++top;
+            node = first;
+            maxt = t_plane;
+        }
+    }
+    return found;
+}
+
+// hittable_list::hit over the object list (hittable_list.h:19-34); a later
+// object wins an equal t.  `lds` points at this lane's first stack slot.
+template <bool COUNT>
+LRT_D bool trace_closest(const DevScene& S, const Ray& r, float tmin, float tmax, HitInfo& h, uint2* lds,
+                         Counters& cnt) {
+    float inv[3] = {1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
+    float closest = tmax;
+    bool any = false;
+    h.obj = -1;
+    h.prim = -1;
+    for (int o = 0; o < S.n_objects; ++o) {
+        const DevObject& ob = S.obj[o];
+        if (ob.shape == kShapeMesh) {
+            float c = closest, u = 0.0f, v = 0.0f;
+            int32_t p = -1;
+            if (kd_traverse<false, COUNT>(S, ob, r, inv, tmin, c, p, u, v, lds, cnt)) {
+                closest = c; h.obj = o; h.prim = p; h.u = u; h.v = v; any = true;
+            }
+        } else if (ob.shape == kShapeRect) {
+            float t, a, b;
+            if (rect_test(ob, r, tmin, closest, t, a, b)) {
+                closest = t; h.obj = o; h.prim = 0; h.u = a; h.v = b; any = true;
+            }
+        } else {
+            float t;
+            if (sphere_test(ob, r, tmin, closest, t)) {
+                closest = t; h.obj = o; h.prim = 0; h.u = 0.0f; h.v = 0.0f; any = true;
+            }
+        }
+    }
+    h.t = closest;
+    return any;
+}
+
+// Shadow test of path_integrator.h:113-116: blocked iff a surface that does
+// not belong to emitter `ignore` is hit in [tmin, tmax].  Exact when the
+// ignored emitter is convex (rectangle, sphere): its own surface cannot be
+// nearer than the sampled point on it.
+LRT_D bool trace_occluded(const DevScene& S, const Ray& r, float tmin, float tmax, int ignore, uint2* lds) {
+    float inv[3] = {1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
+    Counters cnt{0, 0, 0};
+    for (int o = 0; o < S.n_objects; ++o) {
+        const DevObject& ob = S.obj[o];
+        if (ob.emitter >= 0 && ob.emitter == ignore) continue;
+        if (ob.shape == kShapeMesh) {
+            float c = tmax, u, v;
+            int32_t p;
+            if (kd_traverse<true, false>(S, ob, r, inv, tmin, c, p, u, v, lds, cnt)) return true;
+        } else if (ob.shape == kShapeRect) {
+            float t, a, b;
+            if (rect_test(ob, r, tmin, tmax, t, a, b)) return true;
+        } else {
+            float t;
+            if (sphere_test(ob, r, tmin, tmax, t)) return true;
+        }
+    }
+    return false;
+}
+
+}  // namespace lrt

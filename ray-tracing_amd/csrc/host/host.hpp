@@ -1,0 +1,61 @@
+// Host-side structures of liblinrt_hip.so: kd-tree build, flattening, scene
+// generators and the constructor-time precompute of the reference materials.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../../include/linrt.h"
+
+namespace lrt {
+
+// Reference node numbering (sah_kdtree.h:124-132): the left child of an
+// interior node is always the next node; retracted subtrees stay as orphans.
+struct KdNode {
+    uint8_t leaf = 0;
+    uint8_t axis = 0;
+    float split = 0.0f;
+    int32_t left = -1, right = -1, first = 0, count = 0;
+};
+
+struct KdTree {
+    std::vector<KdNode> nodes;
+    std::vector<uint32_t> refs;
+    int depth_limit = 1;
+};
+
+// SAH kd-tree over boxes (6 floats: min xyz, max xyz).  Same splits, same node
+// numbering and same reference order as sah_kdtree::build (sah_kdtree.h:35-57,
+// 175-347).  threads <= 0: hardware concurrency.
+void kd_build(const float* boxes, uint32_t n, KdTree& out, int threads = 0);
+
+// 8-byte device node (PBRT-style kd layout):
+//   interior: w0 = split (float bits), w1 = axis | (right_child << 2); left child = index + 1
+//   leaf:     w0 = first reference,    w1 = 3 | (count << 2)
+struct FlatTree {
+    std::vector<uint32_t> words;  // 2 per node
+    std::vector<uint32_t> refs;
+    int max_depth = 0;            // deepest root-to-leaf path (stack bound)
+};
+// Drops orphan nodes (pre-order renumbering keeps left = parent + 1).
+void kd_flatten(const KdTree& t, FlatTree& out);
+
+// fresnel.h:128-133 + gauss_lobatto.h:23-121 (FP32 adaptive Gauss-Lobatto)
+float fresnel_diffuse_reflectance(float eta);
+// rough_transmittance.h:17-24,67-185 from a ggx.dat image in memory; throws on
+// a bad table or out-of-range eta / alpha like the reference.
+void rough_transmittance_slice(const void* data, size_t bytes, float eta, float alpha, float out101[101]);
+
+// camera.h:20-44 (aperture 0, focus distance 1)
+struct CameraBasis {
+    float origin[3], llc[3], horizontal[3], vertical[3];
+};
+CameraBasis camera_basis(const lrt_camera_desc& c);
+
+// Philox-4x32-10 on the host
+void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+void set_error(const std::string& msg);
+
+}  // namespace lrt

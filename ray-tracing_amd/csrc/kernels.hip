@@ -1,0 +1,410 @@
+// Kernels of liblinrt_hip.so (gfx950, wave64).
+//
+// Ray queries: one lane per ray, grid-stride over a device-resident count so
+// the whole path-tracing frame is enqueued without host round trips.
+//
+// Path tracer (path_integrator.h:18-170) as a wavefront over SoA path state,
+// one path per pixel per sample pass:
+//   k_raygen -> [k_trace (extension) -> k_shade -> k_shadow]* -> k_accumulate
+// k_shade appends surviving paths and next-event shadow rays to queues with a
+// wave ballot + one atomic per wave.  Radiance is added to a path's L in the
+// reference's order (emission, then NEE, then the next bounce), so queue order
+// never changes results.
+#include "kernels.h"
+
+namespace lrt {
+
+// ---------------------------------------------------------------------------
+// wave-aggregated queue append: one atomic per wave, lanes get consecutive slots
+LRT_D uint32_t wave_append(uint32_t* counter, bool want) {
+    const uint64_t mask = __ballot(want);
+    if (!want) return 0xffffffffu;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+    const uint32_t rank = (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, (int)leader);
+    return base + rank;
+}
+
+// ---------------------------------------------------------------------------
+// ray queries
+__global__ __launch_bounds__(kBlock) void k_trace_closest(const DevScene S, const float* __restrict__ rays, uint32_t n,
+                                                          int32_t* __restrict__ obj_prim, float* __restrict__ t_out,
+                                                          float* __restrict__ uv_out) {
+    __shared__ uint2 stack[kLdsStack * kBlock];
+    uint2* lds = stack + threadIdx.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const Ray r{f3{rays[i], rays[n + i], rays[2 * (size_t)n + i]},
+                    f3{rays[3 * (size_t)n + i], rays[4 * (size_t)n + i], rays[5 * (size_t)n + i]}};
+        HitInfo h;
+        Counters c{0, 0, 0};
+        const bool any = trace_closest<false>(S, r, rays[6 * (size_t)n + i], rays[7 * (size_t)n + i], h, lds, c);
+        obj_prim[2 * (size_t)i] = any ? h.obj : -1;
+        obj_prim[2 * (size_t)i + 1] = any ? h.prim : -1;
+        t_out[i] = any ? h.t : 0.0f;
+        if (uv_out) {
+            uv_out[2 * (size_t)i] = any ? h.u : 0.0f;
+            uv_out[2 * (size_t)i + 1] = any ? h.v : 0.0f;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_trace_count(const DevScene S, const float* __restrict__ rays, uint32_t n,
+                                                        uint32_t* __restrict__ nodes, uint32_t* __restrict__ refs,
+                                                        uint32_t* __restrict__ tris) {
+    __shared__ uint2 stack[kLdsStack * kBlock];
+    uint2* lds = stack + threadIdx.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const Ray r{f3{rays[i], rays[n + i], rays[2 * (size_t)n + i]},
+                    f3{rays[3 * (size_t)n + i], rays[4 * (size_t)n + i], rays[5 * (size_t)n + i]}};
+        HitInfo h;
+        Counters c{0, 0, 0};
+        trace_closest<true>(S, r, rays[6 * (size_t)n + i], rays[7 * (size_t)n + i], h, lds, c);
+        nodes[i] = c.nodes;
+        refs[i] = c.refs;
+        tris[i] = c.tris;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_trace_occluded(const DevScene S, const float* __restrict__ rays, uint32_t n,
+                                                           const int32_t* __restrict__ ignore, uint8_t* __restrict__ out) {
+    __shared__ uint2 stack[kLdsStack * kBlock];
+    uint2* lds = stack + threadIdx.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const Ray r{f3{rays[i], rays[n + i], rays[2 * (size_t)n + i]},
+                    f3{rays[3 * (size_t)n + i], rays[4 * (size_t)n + i], rays[5 * (size_t)n + i]}};
+        out[i] = trace_occluded(S, r, rays[6 * (size_t)n + i], rays[7 * (size_t)n + i], ignore ? ignore[i] : -1, lds) ? 1 : 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BSDF / sky batches
+__global__ void k_bsdf_eval(const Mats M, const int32_t* mat, const float* wi, const float* wo, const uint8_t* front,
+                            uint32_t n, int meas, int mode, int comp, uint32_t mask, float* f_cos, float* pdf) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        BsdfCtx c;
+        c.mode = mode; c.component = comp; c.mask = mask;
+        SRec r;
+        r.wi = ld3(wi + 3 * (size_t)i);
+        r.wo = ld3(wo + 3 * (size_t)i);
+        r.front = front[i] != 0;
+        const EP e = bsdf_ep(M, mat[i], c, r, meas);
+        f_cos[3 * (size_t)i] = e.f.x;
+        f_cos[3 * (size_t)i + 1] = e.f.y;
+        f_cos[3 * (size_t)i + 2] = e.f.z;
+        pdf[i] = e.pdf;
+    }
+}
+
+__global__ void k_bsdf_sample(const Mats M, const int32_t* mat, const float* wi, const uint8_t* front, const float* u,
+                              const float* lobe_u, uint32_t n, int mode, int comp, uint32_t mask, float* weight,
+                              float* wo, float* pdf, float* eta, uint32_t* stype, int32_t* scomp) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        BsdfCtx c;
+        c.mode = mode; c.component = comp; c.mask = mask; c.lobe = lobe_u[i];
+        SRec r;
+        r.wi = ld3(wi + 3 * (size_t)i);
+        r.wo = splat(0.0f);
+        r.front = front[i] != 0;
+        const f3 w = bsdf_sample(M, mat[i], c, r, u[2 * (size_t)i], u[2 * (size_t)i + 1]);
+        weight[3 * (size_t)i] = w.x; weight[3 * (size_t)i + 1] = w.y; weight[3 * (size_t)i + 2] = w.z;
+        wo[3 * (size_t)i] = r.wo.x; wo[3 * (size_t)i + 1] = r.wo.y; wo[3 * (size_t)i + 2] = r.wo.z;
+        pdf[i] = r.pdf; eta[i] = r.eta; stype[i] = r.type; scomp[i] = r.comp;
+    }
+}
+
+__global__ void k_sky_eval(const DevLight L, const float* dirs, uint32_t n, float* rgb) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const f3 c = sky_eval(L, ld3(dirs + 3 * (size_t)i));
+        rgb[3 * (size_t)i] = c.x; rgb[3 * (size_t)i + 1] = c.y; rgb[3 * (size_t)i + 2] = c.z;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// path tracer stages
+
+// Philox dimension d of sample (pixel, sample): block d/4, lane d%4
+LRT_D float dim_u(uint32_t pix, uint32_t smp, uint32_t d, uint2 key) {
+    const uint4 v = philox(make_uint4(pix, smp, d >> 2, 0u), key);
+    return u24(pick(v, d & 3u));
+}
+
+__global__ __launch_bounds__(kBlock) void k_raygen(PathState P, const CameraDev cam, uint32_t W, uint32_t H, uint32_t row0,
+                                                   uint32_t n, uint32_t smp, uint2 key, uint32_t* queue,
+                                                   uint32_t* counters) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        counters[0] = n;  // active queue length
+        counters[1] = 0;  // next queue
+        counters[2] = 0;  // shadow queue
+    }
+    if (i >= n) return;
+    const uint32_t x = i % W, y = row0 + i / W;
+    const uint32_t pix = y * W + x;
+    const uint4 v = philox(make_uint4(pix, smp, 0u, 0u), key);
+    const float jx = u24(v.x), jy = u24(v.y);
+    const float u = ((float)x + jx) / (float)(W - 1);
+    const float vv = ((float)y + jy) / (float)(H - 1);
+    const f3 d = normalize(ld3(cam.llc) + u * ld3(cam.horizontal) + vv * ld3(cam.vertical) - ld3(cam.origin));
+    P.ox[i] = cam.origin[0]; P.oy[i] = cam.origin[1]; P.oz[i] = cam.origin[2];
+    P.dx[i] = d.x; P.dy[i] = d.y; P.dz[i] = d.z;
+    P.tr[i] = 1.0f; P.tg[i] = 1.0f; P.tb[i] = 1.0f;
+    P.lr[i] = 0.0f; P.lg[i] = 0.0f; P.lb[i] = 0.0f;
+    P.eta[i] = 1.0f;
+    P.pdf[i] = -1.0f;  // no solid-angle pdf yet (camera ray)
+    queue[i] = i;
+}
+
+// extension rays of the active queue -> hit records by queue slot
+__global__ __launch_bounds__(kBlock) void k_extend(const DevScene S, const PathState P, const uint32_t* __restrict__ queue,
+                                                   const uint32_t* __restrict__ counters, HitBuf Hb) {
+    __shared__ uint2 stack[kLdsStack * kBlock];
+    uint2* lds = stack + threadIdx.x;
+    const uint32_t n = counters[0];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t p = queue[i];
+        const Ray r{f3{P.ox[p], P.oy[p], P.oz[p]}, f3{P.dx[p], P.dy[p], P.dz[p]}};
+        HitInfo h;
+        Counters c{0, 0, 0};
+        const bool any = trace_closest<false>(S, r, 0.001f, kInfF, h, lds, c);
+        Hb.obj[i] = any ? h.obj : -1;
+        Hb.prim[i] = h.prim;
+        Hb.t[i] = h.t;
+        Hb.u[i] = h.u;
+        Hb.v[i] = h.v;
+    }
+}
+
+// same rays with algorithmic-work counters (node visits, references, tests)
+__global__ __launch_bounds__(kBlock) void k_extend_count(const DevScene S, const PathState P, const uint32_t* __restrict__ queue,
+                                                         const uint32_t* __restrict__ counters, unsigned long long* totals) {
+    __shared__ uint2 stack[kLdsStack * kBlock];
+    uint2* lds = stack + threadIdx.x;
+    const uint32_t n = counters[0];
+    uint64_t a = 0, b = 0, c3 = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t p = queue[i];
+        const Ray r{f3{P.ox[p], P.oy[p], P.oz[p]}, f3{P.dx[p], P.dy[p], P.dz[p]}};
+        HitInfo h;
+        Counters c{0, 0, 0};
+        trace_closest<true>(S, r, 0.001f, kInfF, h, lds, c);
+        a += c.nodes; b += c.refs; c3 += c.tris;
+    }
+    atomicAdd(&totals[0], (unsigned long long)a);
+    atomicAdd(&totals[1], (unsigned long long)b);
+    atomicAdd(&totals[2], (unsigned long long)c3);
+}
+
+struct SurfaceHit {
+    f3 p, n;  // shading normal, face-forwarded (triangle_mesh.h:81-96)
+    bool front;
+    int material, emitter;
+};
+
+LRT_D SurfaceHit surface(const DevScene& S, const Ray& r, int obj, int prim, float t, float u, float v) {
+    SurfaceHit s;
+    const DevObject& ob = S.obj[obj];
+    s.p = r.o + t * r.d;
+    s.emitter = ob.emitter;
+    if (ob.shape == kShapeMesh) {
+        const uint32_t g = ob.tri_base + (uint32_t)prim;
+        const float* tri = S.tris + 9 * (size_t)g;
+        const f3 gn = cross(ld3(tri + 3), ld3(tri + 6));
+        const f3 outward = gn / len(gn);
+        s.front = dot(r.d, outward) < 0;
+        s.n = s.front ? outward : -outward;
+        const float* cn = S.cnormals + 9 * (size_t)g;
+        const float b0 = 1.0f - u - v;
+        f3 sh = b0 * ld3(cn) + u * ld3(cn + 3) + v * ld3(cn + 6);
+        if (len2(sh) > 0.0f) {
+            sh = sh / len(sh);
+            if (dot(r.d, sh) > 0.0f) sh = -sh;
+            s.n = sh;
+        }
+        s.material = ob.per_tri_material ? (int)S.tri_mat[g] : ob.material;
+    } else if (ob.shape == kShapeRect) {
+        const f3 rn = ld3(ob.rn);
+        s.front = dot(r.d, rn) < 0;
+        s.n = s.front ? rn : -rn;
+        s.material = -1;
+    } else {
+        const f3 outward = (s.p - ld3(ob.center)) / ob.radius;
+        s.front = dot(r.d, outward) < 0;
+        s.n = s.front ? outward : -outward;
+        s.material = ob.material;
+    }
+    return s;
+}
+
+// one bounce of path_integrator::radiance for every queued path
+__global__ __launch_bounds__(kBlock) void k_shade(const DevScene S, const DevLights Ls, const Mats M, PathState P,
+                                                  const uint32_t* __restrict__ queue, uint32_t* __restrict__ next_queue,
+                                                  uint32_t* counters, const HitBuf Hb, ShadowBuf Sb, ShadeParams prm) {
+    const uint32_t n = counters[0];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // grid-stride with a wave-uniform trip count so the queue appends see full waves
+    const uint32_t n_round = (n + stride - 1) / stride * stride;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
+        bool cont = false, shadow = false;
+        uint32_t p = 0;
+        f3 so, sd, contrib;
+        float tmax = 0.0f;
+        int ignore = -1;
+        if (i < n) {
+            p = queue[i];
+            const Ray r{f3{P.ox[p], P.oy[p], P.oz[p]}, f3{P.dx[p], P.dy[p], P.dz[p]}};
+            f3 T{P.tr[p], P.tg[p], P.tb[p]};
+            f3 L{P.lr[p], P.lg[p], P.lb[p]};
+            float eta = P.eta[p];
+            const float sa_pdf = P.pdf[p];
+            const bool have = sa_pdf >= 0.0f;
+            const int obj = Hb.obj[i];
+            if (obj < 0) {  // escape: environment with MIS (path_integrator.h:32-45)
+                const f3 Le = env_eval(Ls, r.d);
+                if (!have) {
+                    L = L + T * Le;
+                } else {
+                    const float lp = pdf_emitter_direction(S, Ls, splat(0.0f), r.d);
+                    L = L + T * power_heuristic(sa_pdf, lp) * Le;
+                }
+            } else {
+                const SurfaceHit sh = surface(S, r, obj, Hb.prim[i], Hb.t[i], Hb.u[i], Hb.v[i]);
+                if (sh.emitter >= 0 && sh.front) {  // path_integrator.h:47-59
+                    const f3 out = -normalize(r.d);
+                    const f3 Le = dot(sh.n, out) <= 0.0f ? splat(0.0f) : ld3(Ls.l[sh.emitter].radiance);
+                    if (!is_zero(Le)) {
+                        if (!have) {
+                            L = L + T * Le;
+                        } else {
+                            const float lp = pdf_emitter_direction(S, Ls, r.o, r.d);
+                            L = L + T * power_heuristic(sa_pdf, lp) * Le;
+                        }
+                    }
+                }
+                if (sh.material >= 0) {
+                    const Frame fr = frame_from(sh.n);
+                    SRec sr;
+                    sr.wi = to_local(fr, -normalize(r.d));
+                    sr.front = sh.front;
+                    BsdfCtx ctx;
+                    // six sampler dimensions per bounce (path_integrator.h:80-83)
+                    const uint32_t pix = P.pixel0 + p, d0 = 2u + 6u * prm.depth;
+                    const uint4 A = philox(make_uint4(pix, prm.sample, d0 >> 2, 0u), prm.key);
+                    const uint4 B = philox(make_uint4(pix, prm.sample, (d0 >> 2) + 1u, 0u), prm.key);
+                    float u6[6];
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) {
+                        const uint32_t d = d0 + (uint32_t)k;
+                        u6[k] = u24((d >> 2) == (d0 >> 2) ? pick(A, d & 3u) : pick(B, d & 3u));
+                    }
+                    ctx.lobe = u6[2];
+                    if (bsdf_flags(M, sh.material) & (F_DIFFUSE | F_GLOSSY)) {  // NEE (path_integrator.h:86-128)
+                        DirectSample ds;
+                        if (sample_emitter_direction(S, Ls, sh.p, u6[0], u6[1], ds)) {
+                            so = sh.p + sh.n * 1e-4f;
+                            sd = ds.dir;
+                            tmax = kInfF;
+                            bool target = true;
+                            if (!isinf(ds.dist)) {
+                                const f3 to = ds.point - so;
+                                const float dist = len(to);
+                                if (dist < 1e-6f) target = false;
+                                else { sd = to / dist; tmax = dist * (1.0f - 1e-4f); }
+                            }
+                            if (target) {
+                                sr.wo = to_local(fr, sd);
+                                const EP e = bsdf_ep(M, sh.material, ctx, sr, M_SOLID);
+                                if (!is_zero(e.f)) {
+                                    const float lp = pdf_emitter_direction(S, Ls, sh.p, sd);
+                                    if (lp > 0.0f) {
+                                        contrib = T * e.f * ds.radiance * (power_heuristic(lp, e.pdf) / lp);
+                                        ignore = ds.emitter;
+                                        shadow = true;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                    if (prm.depth + 1 < prm.max_depth) {  // last bounce does NEE only
+                        const f3 bw = bsdf_sample(M, sh.material, ctx, sr, u6[3], u6[4]);
+                        if (!is_zero(bw)) {
+                            const f3 nd = to_world(fr, sr.wo);
+                            T = T * bw;
+                            eta *= sr.eta;
+                            const bool hp = ((sr.type & F_SPECULAR) == 0) && sr.pdf > 0.0f;
+                            cont = true;
+                            if (prm.depth >= prm.rr_depth) {  // Russian roulette (path_integrator.h:155-166)
+                                const float q = fminf(fmaxf(fmaxf(T.x, T.y), T.z) * eta * eta, 0.95f);
+                                if (u6[5] >= q) cont = false;
+                                else T = f3{T.x / q, T.y / q, T.z / q};
+                            }
+                            if (cont) {
+                                P.ox[p] = sh.p.x; P.oy[p] = sh.p.y; P.oz[p] = sh.p.z;
+                                P.dx[p] = nd.x; P.dy[p] = nd.y; P.dz[p] = nd.z;
+                                P.tr[p] = T.x; P.tg[p] = T.y; P.tb[p] = T.z;
+                                P.eta[p] = eta;
+                                P.pdf[p] = hp ? sr.pdf : -1.0f;
+                            }
+                        }
+                    }
+                }
+            }
+            P.lr[p] = L.x; P.lg[p] = L.y; P.lb[p] = L.z;
+        }
+        const uint32_t slot = wave_append(&counters[1], cont);
+        if (cont) next_queue[slot] = p;
+        const uint32_t ss = wave_append(&counters[2], shadow);
+        if (shadow) {
+            Sb.ox[ss] = so.x; Sb.oy[ss] = so.y; Sb.oz[ss] = so.z;
+            Sb.dx[ss] = sd.x; Sb.dy[ss] = sd.y; Sb.dz[ss] = sd.z;
+            Sb.tmax[ss] = tmax;
+            Sb.cr[ss] = contrib.x; Sb.cg[ss] = contrib.y; Sb.cb[ss] = contrib.z;
+            Sb.ignore[ss] = ignore;
+            Sb.path[ss] = p;
+        }
+    }
+}
+
+// shadow rays: add the pending NEE contribution when unoccluded
+__global__ __launch_bounds__(kBlock) void k_shadow(const DevScene S, PathState P, const uint32_t* __restrict__ counters,
+                                                   const ShadowBuf Sb) {
+    __shared__ uint2 stack[kLdsStack * kBlock];
+    uint2* lds = stack + threadIdx.x;
+    const uint32_t n = counters[2];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const Ray r{f3{Sb.ox[i], Sb.oy[i], Sb.oz[i]}, f3{Sb.dx[i], Sb.dy[i], Sb.dz[i]}};
+        if (!trace_occluded(S, r, 0.001f, Sb.tmax[i], Sb.ignore[i], lds)) {
+            const uint32_t p = Sb.path[i];
+            const f3 L = f3{P.lr[p], P.lg[p], P.lb[p]} + f3{Sb.cr[i], Sb.cg[i], Sb.cb[i]};
+            P.lr[p] = L.x; P.lg[p] = L.y; P.lb[p] = L.z;
+        }
+    }
+}
+
+// swap: next queue becomes active, counters reset (no memset nodes, graph safe)
+__global__ void k_advance(uint32_t* counters, unsigned long long* stats) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        stats[0] += counters[0];  // extension rays traced at this depth
+        stats[1] += counters[2];  // shadow rays traced at this depth
+        counters[0] = counters[1];
+        counters[1] = 0;
+        counters[2] = 0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_accumulate(const PathState P, uint32_t n, double* __restrict__ sum,
+                                                       double* __restrict__ sumsq) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const size_t pix = (size_t)P.pixel0 + i;
+    const float l[3] = {P.lr[i], P.lg[i], P.lb[i]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        sum[3 * pix + c] += (double)l[c];
+        if (sumsq) sumsq[3 * pix + c] += (double)l[c] * (double)l[c];
+    }
+}
+
+}  // namespace lrt

@@ -1,0 +1,420 @@
+"""Python host layer of liblinrt_hip.so (the MI355X render hot path of LinRT).
+
+A thin ctypes mirror of ``include/linrt.h``: the structs keep the C layout and
+every call goes through the C ABI, so what runs is the hand-written HIP code.
+The roles follow the reference's interfaces:
+
+* accelerator  -- ``kdtree_build``            (sah_kdtree.h:35-57)
+* scene / hit  -- ``Scene.trace_closest``      (scene.h:36-38, hittable_list.h:19-34)
+* materials    -- ``bsdf_eval_pdf`` / ``bsdf_sample`` (bsdf.h:78-88)
+* emitters     -- ``sky_eval``                 (env_emitter.h:71-88)
+* integrator   -- ``Scene.render``             (path_integrator.h:18-170 driven
+                  like the frame loop of src/main.cpp:103-147)
+
+The library must be built (``__graft_entry__.build()``); importing this
+module without it raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+PKG_ROOT = _HERE.parent
+LIB_PATH = PKG_ROOT / "liblinrt_hip.so"
+DATA_DIR = PKG_ROOT / "data"
+
+OK = 0
+ERR_ARG, ERR_HIP, ERR_ALLOC, ERR_UNSUPPORTED, ERR_INTERNAL = -1, -2, -3, -4, -5
+
+# BSDF flags (bsdf.h:18-28)
+DIFFUSE, SPECULAR, REFLECTIVE, TRANSPARENT, GLOSSY, METALLIC, DIELECTRIC, REFRACTIVE = (1 << i for i in range(8))
+MEASURE_SOLID_ANGLE, MEASURE_DISCRETE = 1, 5
+MODE_RADIANCE, MODE_IMPORTANCE = 0, 1
+(BSDF_NONE, BSDF_DIFFUSE, BSDF_CONDUCTOR, BSDF_DIELECTRIC, BSDF_ROUGH_DIELECTRIC, BSDF_PLASTIC,
+ BSDF_ROUGH_PLASTIC, BSDF_COATING, BSDF_ROUGH_COATING) = range(9)
+SHAPE_MESH, SHAPE_RECTANGLE, SHAPE_SPHERE = 0, 1, 2
+LIGHT_AREA, LIGHT_SKY = 0, 1
+GEN_BOX_ICOSPHERE, GEN_DISPLACED_SPHERE = 0, 1
+KERNEL_CLOSEST, KERNEL_OCCLUDED, KERNEL_SHADE = 0, 1, 2
+
+F3 = C.c_float * 3
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_int32), ("nested", C.c_int32), ("int_ior", C.c_float), ("ext_ior", C.c_float),
+                ("alpha", C.c_float), ("nonlinear", C.c_int32), ("diffuse", F3), ("specular_reflectance", F3),
+                ("specular_transmittance", F3), ("cond_eta", F3), ("cond_k", F3), ("sigma_a", F3),
+                ("thickness", C.c_float), ("fdr_int", C.c_float), ("table_ext", C.c_int32), ("table_int", C.c_int32)]
+
+
+class ObjectDesc(C.Structure):
+    _fields_ = [("shape", C.c_int32), ("material", C.c_int32), ("emitter", C.c_int32),
+                ("n_vertices", C.c_uint32), ("n_triangles", C.c_uint32),
+                ("positions", C.POINTER(C.c_float)), ("indices", C.POINTER(C.c_uint32)),
+                ("corner_normals", C.POINTER(C.c_float)), ("triangle_material", C.POINTER(C.c_uint16)),
+                ("corner", F3), ("edge_u", F3), ("edge_v", F3), ("center", F3), ("radius", C.c_float)]
+
+
+class LightDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("object", C.c_int32), ("radiance", F3), ("sun_dir", F3),
+                ("turbidity", C.c_float), ("ground_albedo", C.c_float), ("scale", C.c_float),
+                ("config", (C.c_double * 9) * 3), ("channel_radiance", C.c_double * 3)]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("origin", F3), ("look_at", F3), ("up", F3), ("fov_deg", C.c_float), ("aspect", C.c_float)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("n_objects", C.c_uint32), ("objects", C.POINTER(ObjectDesc)),
+                ("n_materials", C.c_uint32), ("materials", C.POINTER(Material)),
+                ("n_lights", C.c_uint32), ("lights", C.POINTER(LightDesc)),
+                ("n_tables", C.c_uint32), ("tables", C.POINTER(C.c_float)), ("camera", CameraDesc)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("n_triangles", C.c_uint64), ("n_nodes", C.c_uint64), ("n_refs", C.c_uint64),
+                ("device_bytes", C.c_uint64), ("max_tree_depth", C.c_int32),
+                ("build_seconds", C.c_double), ("upload_seconds", C.c_double)]
+
+
+class RenderParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_uint32), ("spp_offset", C.c_uint32),
+                ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32), ("seed", C.c_uint64),
+                ("row_begin", C.c_uint32), ("row_end", C.c_uint32), ("flags", C.c_uint32)]
+
+RENDER_COUNT = 1
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("samples", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("segments", C.c_uint64), ("seconds", C.c_double), ("trace_ms", C.c_double),
+                ("occluded_ms", C.c_double), ("shade_ms", C.c_double),
+                ("trace_launches", C.c_uint64), ("occluded_launches", C.c_uint64),
+                ("nodes_visited", C.c_uint64), ("refs_read", C.c_uint64), ("tris_tested", C.c_uint64)]
+
+
+class GenParams(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("seed", C.c_uint64), ("segments", C.c_uint32), ("rings", C.c_uint32),
+                ("subdivisions", C.c_uint32), ("aspect", C.c_float)]
+
+
+class LinrtError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(f"linrt error {code}: {message}")
+        self.code = code
+
+
+def _load():
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    lib = C.CDLL(str(LIB_PATH))
+    lib.lrt_last_error.restype = C.c_char_p
+    lib.lrt_generated_desc.restype = C.POINTER(SceneDesc)
+    lib.lrt_generated_desc.argtypes = [C.c_void_p]
+    return lib
+
+
+LIB = _load()
+EXPORTED = [
+    "lrt_last_error", "lrt_version", "lrt_device_count", "lrt_kdtree_build_export", "lrt_scene_create",
+    "lrt_scene_release", "lrt_scene_info_get", "lrt_trace_closest", "lrt_trace_occluded", "lrt_trace_count",
+    "lrt_scene_sync", "lrt_scene_kernel_timing", "lrt_bsdf_eval_pdf", "lrt_bsdf_sample", "lrt_bsdf_flags",
+    "lrt_fresnel_diffuse_reflectance", "lrt_rough_transmittance_slice", "lrt_sky_eval", "lrt_render",
+    "lrt_generate_scene", "lrt_generated_desc", "lrt_generated_free", "lrt_philox4x32_10",
+]
+
+
+def _check(rc):
+    if rc != OK:
+        raise LinrtError(rc, LIB.lrt_last_error().decode())
+
+
+def _p(a, ctype=C.c_float):
+    return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
+
+
+def device_count():
+    return int(LIB.lrt_device_count())
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    _check(LIB.lrt_philox4x32_10(c, k, o))
+    return list(o)
+
+
+def kdtree_build(boxes):
+    """SAH kd-tree in the reference's node numbering (sah_kdtree.h:35-57)."""
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = boxes.shape[0]
+    nn, nr, dl = C.c_uint32(), C.c_uint32(), C.c_int32()
+    z = None
+    _check(LIB.lrt_kdtree_build_export(_p(boxes), C.c_uint32(n), C.byref(nn), C.byref(nr), C.byref(dl),
+                                       z, z, z, z, z, z, z, z))
+    out = dict(leaf=np.zeros(nn.value, np.uint8), axis=np.zeros(nn.value, np.int32),
+               split=np.zeros(nn.value, np.float32), left=np.zeros(nn.value, np.int32),
+               right=np.zeros(nn.value, np.int32), first=np.zeros(nn.value, np.int32),
+               count=np.zeros(nn.value, np.int32), refs=np.zeros(max(nr.value, 1), np.uint32))
+    _check(LIB.lrt_kdtree_build_export(_p(boxes), C.c_uint32(n), C.byref(nn), C.byref(nr), C.byref(dl),
+                                       _p(out["leaf"], C.c_uint8), _p(out["axis"], C.c_int32),
+                                       _p(out["split"]), _p(out["left"], C.c_int32), _p(out["right"], C.c_int32),
+                                       _p(out["first"], C.c_int32), _p(out["count"], C.c_int32),
+                                       _p(out["refs"], C.c_uint32)))
+    out["refs"] = out["refs"][: nr.value]
+    out["depth_limit"] = dl.value
+    return out
+
+
+def fresnel_diffuse_reflectance(eta):
+    o = C.c_float()
+    _check(LIB.lrt_fresnel_diffuse_reflectance(C.c_float(eta), C.byref(o)))
+    return o.value
+
+
+def rough_transmittance_slice(table_bytes, eta, alpha):
+    out = np.zeros(101, np.float32)
+    buf = C.create_string_buffer(table_bytes, len(table_bytes))
+    _check(LIB.lrt_rough_transmittance_slice(buf, C.c_size_t(len(table_bytes)), C.c_float(eta),
+                                             C.c_float(alpha), _p(out)))
+    return out
+
+
+# ---- committed data: rough-transmittance slices and cooked sky --------------
+def load_tables(name="tables_displaced_sphere.json"):
+    d = json.loads((DATA_DIR / name).read_text())
+    return np.asarray(d["tables"], np.float32).reshape(-1, 101), d
+
+
+def load_sky(name="sky_t3_a0.3_e30.json"):
+    d = json.loads((DATA_DIR / name).read_text())
+    L = LightDesc()
+    L.kind = LIGHT_SKY
+    L.object = -1
+    for i in range(3):
+        L.sun_dir[i] = d["sun_dir"][i]
+    L.turbidity = d["turbidity"]
+    L.ground_albedo = d["ground_albedo"]
+    L.scale = d["scale"]
+    for c in range(3):
+        for k in range(9):
+            L.config[c][k] = d["config"][c][k]
+        L.channel_radiance[c] = d["channel_radiance"][c]
+    return L
+
+
+class Generated:
+    """A seeded scene owned by the library; ``desc`` stays valid while alive."""
+
+    def __init__(self, kind, seed=1, segments=1024, rings=512, subdivisions=5, aspect=0.0, tables=None, sky=None):
+        p = GenParams(kind, seed, segments, rings, subdivisions, aspect)
+        self._tables = None if tables is None else np.ascontiguousarray(tables, np.float32)
+        self._sky = sky
+        h = C.c_void_p()
+        nt = 0 if self._tables is None else self._tables.size // 101
+        _check(LIB.lrt_generate_scene(C.byref(p), _p(self._tables), C.c_uint32(nt),
+                                      C.byref(sky) if sky is not None else None, C.byref(h)))
+        self._h = h
+        self.desc_ptr = LIB.lrt_generated_desc(h)
+        self.desc = self.desc_ptr.contents
+
+    def close(self):
+        if self._h:
+            LIB.lrt_generated_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def mesh(self, obj):
+        o = self.desc.objects[obj]
+        pos = np.ctypeslib.as_array(o.positions, (o.n_vertices * 3,)).reshape(-1, 3).copy()
+        idx = np.ctypeslib.as_array(o.indices, (o.n_triangles * 3,)).reshape(-1, 3).copy()
+        return pos, idx
+
+
+def box_icosphere(seed=1, subdivisions=5, aspect=1.0):
+    """Closed box room + icosphere + ceiling light (the 256x256 16-spp workload)."""
+    return Generated(GEN_BOX_ICOSPHERE, seed=seed, subdivisions=subdivisions, aspect=aspect)
+
+
+def displaced_sphere(seed=1, segments=1024, rings=512, aspect=16.0 / 9.0):
+    """Displaced sphere with 7 material regions, ground and Hosek sky (1080p workload)."""
+    tables, _ = load_tables()
+    return Generated(GEN_DISPLACED_SPHERE, seed=seed, segments=segments, rings=rings, aspect=aspect,
+                     tables=tables, sky=load_sky())
+
+
+class Scene:
+    """Scene uploaded to one GPU (lrt_scene_create)."""
+
+    def __init__(self, desc_ptr, device=0):
+        h = C.c_void_p()
+        _check(LIB.lrt_scene_create(C.c_int(device), desc_ptr, C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            LIB.lrt_scene_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def info(self):
+        i = SceneInfo()
+        _check(LIB.lrt_scene_info_get(self._h, C.byref(i)))
+        return i
+
+    def trace_closest(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(8, -1)
+        n = rays.shape[1]
+        op = np.zeros((n, 2), np.int32)
+        t = np.zeros(n, np.float32)
+        uv = np.zeros((n, 2), np.float32)
+        _check(LIB.lrt_trace_closest(self._h, _p(rays), C.c_uint32(n), _p(op, C.c_int32), _p(t), _p(uv), 0))
+        return op, t, uv
+
+    def trace_occluded(self, rays, ignore=None):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(8, -1)
+        n = rays.shape[1]
+        out = np.zeros(n, np.uint8)
+        ig = None if ignore is None else np.ascontiguousarray(ignore, np.int32)
+        _check(LIB.lrt_trace_occluded(self._h, _p(rays), C.c_uint32(n), _p(ig, C.c_int32), _p(out, C.c_uint8), 0))
+        return out.astype(bool)
+
+    def trace_count(self, rays):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(8, -1)
+        n = rays.shape[1]
+        a, b, c = (np.zeros(n, np.uint32) for _ in range(3))
+        _check(LIB.lrt_trace_count(self._h, _p(rays), C.c_uint32(n), _p(a, C.c_uint32), _p(b, C.c_uint32),
+                                   _p(c, C.c_uint32), 0))
+        return a, b, c
+
+    def trace_closest_device(self, rays_ptr, n, obj_prim_ptr, t_ptr, uv_ptr=None):
+        _check(LIB.lrt_trace_closest(self._h, C.c_void_p(rays_ptr), C.c_uint32(n), C.c_void_p(obj_prim_ptr),
+                                     C.c_void_p(t_ptr), C.c_void_p(uv_ptr) if uv_ptr else None, 1))
+
+    def sync(self):
+        _check(LIB.lrt_scene_sync(self._h))
+
+    def kernel_timing(self, kernel=KERNEL_CLOSEST, reset=False):
+        ms, nl, it = C.c_double(), C.c_uint64(), C.c_uint64()
+        _check(LIB.lrt_scene_kernel_timing(self._h, C.c_int(kernel), C.c_int(int(reset)), C.byref(ms),
+                                           C.byref(nl), C.byref(it)))
+        return ms.value, nl.value, it.value
+
+    def render(self, width, height, spp, max_depth, rr_depth, seed=1, spp_offset=0, rows=None, sumsq=True,
+               sum_rgb=None, sumsq_rgb=None):
+        """Adds spp samples per pixel into (sum, sumsq) float64 buffers [H, W, 3]
+        (row 0 = bottom, like src/main.cpp:122) and returns them with stats."""
+        if sum_rgb is None:
+            sum_rgb = np.zeros((height, width, 3), np.float64)
+        if sumsq and sumsq_rgb is None:
+            sumsq_rgb = np.zeros((height, width, 3), np.float64)
+        rb, re = rows if rows is not None else (0, 0)
+        prm = RenderParams(width, height, spp, spp_offset, max_depth, rr_depth, seed, rb, re, 0)
+        st = RenderStats()
+        _check(LIB.lrt_render(self._h, C.byref(prm), _p(sum_rgb, C.c_double),
+                              _p(sumsq_rgb, C.c_double) if sumsq else None, 0, C.byref(st)))
+        return sum_rgb, sumsq_rgb, st
+
+    def render_device(self, width, height, spp, max_depth, rr_depth, sum_ptr, sumsq_ptr=0, seed=1, spp_offset=0,
+                      count=False):
+        prm = RenderParams(width, height, spp, spp_offset, max_depth, rr_depth, seed, 0, 0,
+                           RENDER_COUNT if count else 0)
+        st = RenderStats()
+        _check(LIB.lrt_render(self._h, C.byref(prm), C.c_void_p(sum_ptr), C.c_void_p(sumsq_ptr) if sumsq_ptr else None,
+                              1, C.byref(st)))
+        return st
+
+
+def _mats_arg(materials):
+    arr = (Material * len(materials))(*materials)
+    return arr, len(materials)
+
+
+def bsdf_eval_pdf(materials, tables, mat, wi, wo, front, measure=MEASURE_SOLID_ANGLE, mode=MODE_RADIANCE,
+                  component=-1, type_mask=0xFFFFFFFF, device=0):
+    marr, nm = _mats_arg(materials)
+    tb = np.ascontiguousarray(tables if tables is not None else np.zeros((0, 101)), np.float32)
+    mat = np.ascontiguousarray(mat, np.int32)
+    wi = np.ascontiguousarray(wi, np.float32)
+    wo = np.ascontiguousarray(wo, np.float32)
+    front = np.ascontiguousarray(front, np.uint8)
+    n = mat.size
+    f = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    _check(LIB.lrt_bsdf_eval_pdf(C.c_int(device), marr, C.c_uint32(nm), _p(tb), C.c_uint32(tb.size // 101),
+                                 _p(mat, C.c_int32), _p(wi), _p(wo), _p(front, C.c_uint8), C.c_uint32(n),
+                                 C.c_int(measure), C.c_int(mode), C.c_int(component), C.c_uint32(type_mask),
+                                 _p(f), _p(pdf)))
+    return f, pdf
+
+
+def bsdf_sample(materials, tables, mat, wi, front, u, lobe_u, mode=MODE_RADIANCE, component=-1,
+                type_mask=0xFFFFFFFF, device=0):
+    marr, nm = _mats_arg(materials)
+    tb = np.ascontiguousarray(tables if tables is not None else np.zeros((0, 101)), np.float32)
+    mat = np.ascontiguousarray(mat, np.int32)
+    wi = np.ascontiguousarray(wi, np.float32)
+    front = np.ascontiguousarray(front, np.uint8)
+    u = np.ascontiguousarray(u, np.float32)
+    lobe_u = np.ascontiguousarray(lobe_u, np.float32)
+    n = mat.size
+    w = np.zeros((n, 3), np.float32)
+    wo = np.zeros((n, 3), np.float32)
+    pdf = np.zeros(n, np.float32)
+    eta = np.zeros(n, np.float32)
+    st = np.zeros(n, np.uint32)
+    sc = np.zeros(n, np.int32)
+    _check(LIB.lrt_bsdf_sample(C.c_int(device), marr, C.c_uint32(nm), _p(tb), C.c_uint32(tb.size // 101),
+                               _p(mat, C.c_int32), _p(wi), _p(front, C.c_uint8), _p(u), _p(lobe_u), C.c_uint32(n),
+                               C.c_int(mode), C.c_int(component), C.c_uint32(type_mask), _p(w), _p(wo), _p(pdf),
+                               _p(eta), _p(st, C.c_uint32), _p(sc, C.c_int32)))
+    return dict(weight=w, wo=wo, pdf=pdf, eta=eta, sampled_type=st, sampled_component=sc)
+
+
+def bsdf_flags(materials, index):
+    marr, nm = _mats_arg(materials)
+    f, c = C.c_uint32(), C.c_int32()
+    _check(LIB.lrt_bsdf_flags(marr, C.c_uint32(nm), C.c_int32(index), C.byref(f), C.byref(c)))
+    return f.value, c.value
+
+
+def sky_eval(sky, dirs, device=0):
+    dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    n = dirs.shape[0]
+    rgb = np.zeros((n, 3), np.float32)
+    _check(LIB.lrt_sky_eval(C.c_int(device), C.byref(sky), _p(dirs), C.c_uint32(n), _p(rgb)))
+    return rgb
+
+
+def material(type, **kw):
+    """lrt_material with the reference constructors' defaults."""
+    m = Material()
+    m.type = type
+    m.nested = -1
+    m.table_ext = m.table_int = -1
+    m.int_ior, m.ext_ior = 1.5, 1.0
+    m.thickness = 1.0
+    for i in range(3):
+        m.specular_reflectance[i] = 1.0
+        m.specular_transmittance[i] = 1.0
+        m.diffuse[i] = 0.5
+    for k, v in kw.items():
+        if isinstance(v, (list, tuple, np.ndarray)):
+            arr = getattr(m, k)
+            for i in range(3):
+                arr[i] = float(v[i])
+        else:
+            setattr(m, k, v)
+    return m

@@ -1,0 +1,121 @@
+"""CPU tests: the C-ABI library loads and exports every declared symbol; the
+oracle is pinned (Philox known-answer vectors, the reference's own kd-tree and
+per-sample path radiance through oracle/_ref); host-side product code (kd
+build, precompute) matches the oracle/reference exactly.  No GPU call."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import linrt
+import oracle_lib as OL
+
+REPO = Path(__file__).resolve().parent.parent
+needs_ref = pytest.mark.skipif(OL.reference is None, reason="oracle/_ref not built (reference tree absent)")
+
+
+def test_library_exports_every_declared_symbol():
+    decl = set(re.findall(r"\b(lrt_[a-z0-9_]+)\s*\(", (REPO / "include" / "linrt.h").read_text()))
+    decl = {d for d in decl if not d.endswith("_t")}
+    missing = [d for d in sorted(decl) if not hasattr(linrt.LIB, d)]
+    assert not missing, missing
+    assert set(linrt.EXPORTED) <= decl
+
+
+def test_philox_known_answers():
+    """Random123 philox4x32-10 known-answer vectors (kat_vectors)."""
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, want in kat:
+        assert OL.philox(ctr, key) == want
+        assert linrt.philox(ctr, key) == want
+
+
+def _boxes(pos, idx):
+    tri = pos[idx]
+    return np.concatenate([tri.min(1), tri.max(1)], 1).astype(np.float32)
+
+
+def _same_tree(a, b):
+    for k in ("leaf", "axis", "split", "left", "right", "first", "count", "refs"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["depth_limit"] == b["depth_limit"]
+
+
+@pytest.mark.parametrize("kind", ["icosphere", "room", "sphere_small", "random"])
+def test_kdtree_product_equals_oracle(kind):
+    if kind == "random":
+        rng = np.random.default_rng(5)
+        lo = rng.random((3000, 3)).astype(np.float32)
+        hi = lo + rng.random((3000, 3)).astype(np.float32) * 0.05
+        hi[::7, 1] = lo[::7, 1]  # planar boxes
+        boxes = np.concatenate([lo, hi], 1)
+    elif kind == "sphere_small":
+        g = linrt.displaced_sphere(seed=2, segments=128, rings=64)
+        boxes = _boxes(*g.mesh(0))
+    else:
+        g = linrt.box_icosphere(seed=1, subdivisions=4)
+        boxes = _boxes(*g.mesh(1 if kind == "icosphere" else 0))
+    _same_tree(linrt.kdtree_build(boxes), OL.oracle.kd_build(boxes))
+
+
+@needs_ref
+@pytest.mark.parametrize("kind", ["icosphere", "sphere_small", "random"])
+def test_kdtree_oracle_equals_reference(kind):
+    if kind == "random":
+        rng = np.random.default_rng(6)
+        lo = rng.random((2000, 3)).astype(np.float32)
+        boxes = np.concatenate([lo, lo + 0.03 * rng.random((2000, 3)).astype(np.float32)], 1)
+    elif kind == "sphere_small":
+        boxes = _boxes(*linrt.displaced_sphere(seed=2, segments=128, rings=64).mesh(0))
+    else:
+        boxes = _boxes(*linrt.box_icosphere(seed=1, subdivisions=4).mesh(1))
+    _same_tree(OL.oracle.kd_build(boxes), OL.reference.kd_build(boxes))
+
+
+def test_kdtree_empty_and_tiny():
+    t = linrt.kdtree_build(np.zeros((0, 6), np.float32))
+    assert len(t["leaf"]) == 0
+    one = np.array([[0, 0, 0, 1, 1, 1]], np.float32)
+    t = linrt.kdtree_build(one)
+    assert list(t["leaf"]) == [1] and list(t["refs"]) == [0]
+
+
+@needs_ref
+def test_oracle_render_is_the_reference_per_sample():
+    """Reference-sampler mode of the oracle reproduces the reference's own
+    path_integrator (depth 50, RR from 5) bit for bit on every sample."""
+    g = linrt.box_icosphere(seed=1, subdivisions=3)
+    a = OL.oracle.scene(g.desc_ptr).render(20, 20, 4, 50, 5, mode=1, per_sample=True, fsum=True)
+    b = OL.reference.scene(g.desc_ptr).render(20, 20, 4, 50, 5, per_sample=True, fsum=True)
+    assert np.array_equal(a["per_sample"], b["per_sample"])
+    assert np.array_equal(a["fsum"], b["fsum"])
+
+
+@needs_ref
+def test_oracle_hits_are_the_reference():
+    from common import random_rays, surface_rays
+    g = linrt.box_icosphere(seed=1, subdivisions=4)
+    pos, idx = g.mesh(1)
+    rays = np.concatenate([surface_rays(pos, idx, 20000, 1), random_rays(np.full(3, -0.9), np.full(3, 0.9), 20000, 2)], 1)
+    oa, ta = OL.oracle.scene(g.desc_ptr).trace(rays)
+    ob, tb = OL.reference.scene(g.desc_ptr).trace(rays)
+    assert np.array_equal(oa, ob) and np.array_equal(ta, tb)
+
+
+def test_oracle_kd_traversal_matches_brute_force():
+    from common import hit_parity, surface_rays
+    g = linrt.displaced_sphere(seed=4, segments=96, rings=48)
+    sc = OL.oracle.scene(g.desc_ptr)
+    pos, idx = g.mesh(0)
+    rays = surface_rays(pos, idx, 5000, 3)
+    oa, ta = sc.trace(rays)
+    ob, tb = sc.trace_brute(rays)
+    frac, bad = hit_parity(oa, ta, ob, tb)
+    assert frac >= 0.999, bad[:20]

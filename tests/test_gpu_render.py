@@ -1,0 +1,89 @@
+"""GPU path tracer (lrt_render) against the oracle's CPU render of the same
+scene with the same Philox sample streams.
+
+Per-pixel means must agree within 3 sigma of the estimated standard error,
+applied distributionally: the fraction of pixel-channels with |z| > 3 stays
+within a binomial bound of 0.27%, mean(z) ~ 0 and var(z) ~ 1.  Because both
+sides consume identical random numbers, most pixels agree far closer than
+that; the test also reports the fraction equal to 1e-4 relative.
+"""
+import numpy as np
+import pytest
+
+import linrt
+from oracle_lib import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def z_scores(sum_a, sq_a, n_a, sum_b, sq_b, n_b):
+    ma, mb = sum_a / n_a, sum_b / n_b
+    va = np.maximum(sq_a / n_a - ma * ma, 0.0) * n_a / max(n_a - 1, 1)
+    vb = np.maximum(sq_b / n_b - mb * mb, 0.0) * n_b / max(n_b - 1, 1)
+    se = np.sqrt(va / n_a + vb / n_b)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        z = np.where(se > 0, (ma - mb) / se, np.where(np.abs(ma - mb) > 1e-6 * np.abs(mb) + 1e-9, np.inf, 0.0))
+    return z
+
+
+def check_z(z, label):
+    z = z.ravel()
+    n = z.size
+    frac3 = np.mean(np.abs(z) > 3)
+    p = 0.0027
+    bound = p + 5 * np.sqrt(p * (1 - p) / n)
+    finite = z[np.isfinite(z)]
+    print(f"{label}: {n} pixel-channels, |z|>3 fraction {frac3:.5f} (bound {bound:.5f}), "
+          f"mean z {finite.mean():+.4f}, var z {finite.var():.4f}, worst {np.sort(np.abs(finite))[-5:]}")
+    assert frac3 <= bound
+    assert abs(finite.mean()) < 0.1
+
+
+@pytest.mark.parametrize("spp", [16])
+def test_box_icosphere_image(spp):
+    W = H = 64
+    g = linrt.box_icosphere(seed=1, subdivisions=5)
+    gs = linrt.Scene(g.desc_ptr)
+    s_g, q_g, st = gs.render(W, H, spp, max_depth=8, rr_depth=3, seed=7)
+    o = oracle.scene(g.desc_ptr).render(W, H, spp, 8, 3, seed=7, mode=0)
+    z = z_scores(s_g, q_g, spp, o["sum"], o["sumsq"], spp)
+    close = np.isclose(s_g, o["sum"], rtol=1e-4, atol=1e-6).mean()
+    print(f"box_icosphere {W}x{H} {spp}spp: samples {st.samples}, ext rays {st.extension_rays}, "
+          f"shadow {st.shadow_rays} (oracle {o['ext']}, {o['shadow']}); equal to 1e-4: {close:.4f}")
+    check_z(z, "box_icosphere")
+    assert close > 0.95
+    assert st.samples == W * H * spp
+
+
+def test_displaced_sphere_crop():
+    W, H, spp = 96, 54, 8
+    g = linrt.displaced_sphere(seed=1, segments=256, rings=128)
+    gs = linrt.Scene(g.desc_ptr)
+    s_g, q_g, st = gs.render(W, H, spp, max_depth=16, rr_depth=5, seed=3)
+    o = oracle.scene(g.desc_ptr).render(W, H, spp, 16, 5, seed=3, mode=0)
+    z = z_scores(s_g, q_g, spp, o["sum"], o["sumsq"], spp)
+    close = np.isclose(s_g, o["sum"], rtol=1e-4, atol=1e-6).mean()
+    print(f"displaced_sphere {W}x{H} {spp}spp: equal to 1e-4: {close:.4f}; ext {st.extension_rays} vs {o['ext']}")
+    check_z(z, "displaced_sphere")
+    assert close > 0.9
+
+
+def test_progressive_offset_matches_single_pass():
+    """Two passes of 4 spp (offset 0 and 4) add up to one pass of 8 spp."""
+    W = H = 32
+    g = linrt.box_icosphere(seed=2, subdivisions=3)
+    gs = linrt.Scene(g.desc_ptr)
+    a, qa, _ = gs.render(W, H, 8, 8, 3, seed=5)
+    b, qb, _ = gs.render(W, H, 4, 8, 3, seed=5)
+    b, qb, _ = gs.render(W, H, 4, 8, 3, seed=5, spp_offset=4, sum_rgb=b, sumsq_rgb=qb)
+    assert np.allclose(a, b, rtol=1e-12, atol=0)
+
+
+def test_rows_subset_matches_full():
+    W = H = 32
+    g = linrt.box_icosphere(seed=2, subdivisions=3)
+    gs = linrt.Scene(g.desc_ptr)
+    full, _, _ = gs.render(W, H, 2, 8, 3, seed=9)
+    part, _, _ = gs.render(W, H, 2, 8, 3, seed=9, rows=(5, 17))
+    assert np.array_equal(part[5:17], full[5:17])
+    assert np.all(part[:5] == 0) and np.all(part[17:] == 0)
