@@ -40,7 +40,9 @@ DATA = REPO / "ray-tracing_amd" / "data"
 # (eta, alpha) slices used by the displaced-sphere materials: rough plastic
 # external and internal tables (rough_plastic_bsdf.h:31-32), rough coating
 # (rough_coating_bsdf.h:234)
-SLICES = [(1.49 / 1.000277, 0.3), (1.000277 / 1.49, 0.3), (1.7 / 1.000277, 0.25)]
+_F = np.float32
+_ETA_PLASTIC = _F(1.49) / _F(1.000277)  # float division, as the constructors compute it
+SLICES = [(_ETA_PLASTIC, _F(0.3)), (_F(1.0) / _ETA_PLASTIC, _F(0.3)), (_F(1.7) / _F(1.000277), _F(0.25))]
 SUN = (math.cos(math.radians(30)) * math.cos(math.radians(45)), math.sin(math.radians(30)),
        math.cos(math.radians(30)) * math.sin(math.radians(45)))
 
@@ -79,12 +81,73 @@ def make_sky():
     (DATA / "sky_t3_a0.3_e30.json").write_text(json.dumps(doc, indent=1))
 
 
+def make_bsdf_fixture(tables):
+    """Reference eval/pdf (both measures) and sample outputs on seeded tuples."""
+    from common import bsdf_tuples, material_zoo
+    mats = material_zoo(tables)
+    n = 1024 * len(mats)
+    mat, wi, wo, front, u, lobe = bsdf_tuples(n, seed=2024, n_mats=len(mats))
+    out = dict(mat=mat, wi=wi, wo=wo, front=front, u=u, lobe=lobe)
+    for meas, tag in ((linrt.MEASURE_SOLID_ANGLE, "sa"), (linrt.MEASURE_DISCRETE, "disc")):
+        f, p = OL.reference.bsdf_eval_pdf(mats, tables, mat, wi, wo, front, measure=meas)
+        out[f"f_{tag}"], out[f"pdf_{tag}"] = f, p
+    f, p = OL.reference.bsdf_eval_pdf(mats, tables, mat, wi, wo, front, mode=linrt.MODE_IMPORTANCE)
+    out["f_imp"], out["pdf_imp"] = f, p
+    s = OL.reference.bsdf_sample(mats, tables, mat, wi, front, u, lobe)
+    for k, v in s.items():
+        out["s_" + k] = v
+    flags = [OL.reference.bsdf_flags(mats, i) for i in range(len(mats))]
+    out["flags"] = np.array(flags, np.int64)
+    np.savez_compressed(OUT / "bsdf_reference.npz", **out)
+
+
+def make_hits_fixture():
+    from common import random_rays, surface_rays
+    g = linrt.box_icosphere(seed=1, subdivisions=4)
+    pos, idx = g.mesh(1)
+    rays = np.concatenate([surface_rays(pos, idx, 3000, 7), random_rays(np.full(3, -0.95), np.full(3, 0.95), 3000, 8)], 1)
+    op, t = OL.reference.scene(g.desc_ptr).trace(rays)
+    np.savez_compressed(OUT / "hits_box_icosphere_l4.npz", rays=rays, obj_prim=op, t=t)
+
+
+def make_misc_fixture(tables):
+    rng = np.random.default_rng(99)
+    dirs = rng.normal(size=(2048, 3)).astype(np.float32)
+    dirs[:1024, 1] = np.abs(dirs[:1024, 1])
+    rgb = np.zeros_like(dirs)
+    sun = (C.c_float * 3)(*SUN)
+    OL.REF.ref_sky_eval(sun, C.c_float(3.0), C.c_float(0.3), C.c_float(0.05), OL._p(dirs), C.c_uint32(len(dirs)),
+                        OL._p(rgb))
+    etas = np.array([1.0 / np.float32(1.49 / 1.000277), 1 / 1.5, 1 / 1.33, 1 / 1.7, 1.5, 0.5], np.float32)
+    fdr = np.zeros_like(etas)
+    for i, e in enumerate(etas):
+        o = C.c_float()
+        OL.REF.ref_fdr(C.c_float(e), C.byref(o))
+        fdr[i] = o.value
+    seq = np.zeros(64, np.float32)
+    OL.REF.ref_sampler_seq(C.c_uint32(1234), C.c_uint32(17), C.c_uint32(32), OL._p(seq))
+    np.savez_compressed(OUT / "misc_reference.npz", sky_dirs=dirs, sky_rgb=rgb, fdr_eta=etas, fdr=fdr,
+                        slices=tables, sampler_seq=seq)
+
+
+def make_render_fixture():
+    """Per-sample radiance of the reference's own integrator (depth 50, RR from
+    5, reference Sobol sampler) on a small box-icosphere scene."""
+    g = linrt.box_icosphere(seed=3, subdivisions=3)
+    r = OL.reference.scene(g.desc_ptr).render(16, 16, 4, 50, 5, per_sample=True, fsum=True)
+    np.savez_compressed(OUT / "render_box_icosphere_16x16x4.npz", per_sample=r["per_sample"], fsum=r["fsum"])
+
+
 def main():
     DATA.mkdir(parents=True, exist_ok=True)
     assert OL.REF is not None, "oracle/_ref/libref.so is not built (make -C oracle ref)"
-    make_tables()
+    tables = make_tables()
     make_sky()
-    print("data written")
+    make_bsdf_fixture(tables)
+    make_hits_fixture()
+    make_misc_fixture(tables)
+    make_render_fixture()
+    print("data and fixtures written")
 
 
 if __name__ == "__main__":

@@ -1,0 +1,96 @@
+"""The oracle against the committed vectors of the reference (tests/golden/,
+written by make_golden.py from the reference's own sources): BSDF eval/pdf/
+sample, ray hits, Hosek sky radiance, fdr, rough-transmittance slices, the
+reference sampler sequence and per-sample path radiance.  These run anywhere
+(no reference tree, no GPU)."""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import linrt
+import oracle_lib as OL
+from common import material_zoo
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module")
+def tables():
+    return np.load(GOLD / "misc_reference.npz")["slices"]
+
+
+def test_slices_and_product_data(tables):
+    """The committed displaced-sphere tables are the reference's slices."""
+    t, _ = linrt.load_tables()
+    assert np.array_equal(t, tables)
+
+
+def test_fdr_matches_reference():
+    d = np.load(GOLD / "misc_reference.npz")
+    mine = np.array([linrt.fresnel_diffuse_reflectance(e) for e in d["fdr_eta"]], np.float32)
+    assert np.array_equal(mine, d["fdr"])
+
+
+def test_fdr_against_double_quadrature():
+    """fdr is the cosine-weighted hemispherical average of Fresnel: compare
+    the FP32 adaptive Gauss-Lobatto with a double-precision Gauss-Legendre
+    integral (independent of the reference code)."""
+    def fresnel(ci, eta):
+        s = np.where(ci > 0, 1 / eta, eta)
+        ct2 = 1 - (1 - ci * ci) * s * s
+        out = np.ones_like(ci)
+        ok = ct2 > 0
+        ai, at = np.abs(ci[ok]), np.sqrt(ct2[ok])
+        rs = (ai - eta * at) / (ai + eta * at)
+        rp = (eta * ai - at) / (eta * ai + at)
+        out[ok] = 0.5 * (rs * rs + rp * rp)
+        return out
+    x, w = np.polynomial.legendre.leggauss(4000)
+    for eta in (1 / 1.49, 1 / 1.5, 1.5, 1 / 1.7):
+        xi = 0.5 * (x + 1)
+        ref = 0.5 * np.sum(w * fresnel(np.sqrt(xi), eta))
+        assert abs(linrt.fresnel_diffuse_reflectance(eta) - ref) < 2e-5 * max(ref, 1e-3)
+
+
+def test_sky_matches_reference():
+    d = np.load(GOLD / "misc_reference.npz")
+    sky = linrt.load_sky()
+    mine = OL.sky_eval(sky, d["sky_dirs"])
+    assert np.array_equal(mine, d["sky_rgb"])
+
+
+def test_bsdf_oracle_is_the_reference(tables):
+    d = np.load(GOLD / "bsdf_reference.npz")
+    mats = material_zoo(tables)
+    o = OL.oracle
+    for meas, tag in ((linrt.MEASURE_SOLID_ANGLE, "sa"), (linrt.MEASURE_DISCRETE, "disc")):
+        f, p = o.bsdf_eval_pdf(mats, tables, d["mat"], d["wi"], d["wo"], d["front"], measure=meas)
+        assert np.array_equal(f, d[f"f_{tag}"]), tag
+        assert np.array_equal(p, d[f"pdf_{tag}"]), tag
+    f, p = o.bsdf_eval_pdf(mats, tables, d["mat"], d["wi"], d["wo"], d["front"], mode=linrt.MODE_IMPORTANCE)
+    assert np.array_equal(f, d["f_imp"]) and np.array_equal(p, d["pdf_imp"])
+    s = o.bsdf_sample(mats, tables, d["mat"], d["wi"], d["front"], d["u"], d["lobe"])
+    for k in ("weight", "wo", "pdf", "eta", "sampled_type", "sampled_component"):
+        assert np.array_equal(s[k], d["s_" + k]), k
+    flags = [o.bsdf_flags(mats, i) for i in range(len(mats))]
+    assert np.array_equal(np.array(flags), d["flags"])
+    # the tuples exercise every lobe: nonzero values for every material
+    nz = [(d["f_sa"][d["mat"] == m].any() or d["f_disc"][d["mat"] == m].any()) for m in range(len(mats))]
+    assert all(nz)
+
+
+def test_hits_oracle_is_the_reference():
+    d = np.load(GOLD / "hits_box_icosphere_l4.npz")
+    g = linrt.box_icosphere(seed=1, subdivisions=4)
+    op, t = OL.oracle.scene(g.desc_ptr).trace(d["rays"])
+    assert np.array_equal(op, d["obj_prim"]) and np.array_equal(t, d["t"])
+
+
+def test_render_oracle_is_the_reference():
+    d = np.load(GOLD / "render_box_icosphere_16x16x4.npz")
+    g = linrt.box_icosphere(seed=3, subdivisions=3)
+    r = OL.oracle.scene(g.desc_ptr).render(16, 16, 4, 50, 5, mode=1, per_sample=True, fsum=True)
+    assert np.array_equal(r["per_sample"], d["per_sample"])
+    assert np.array_equal(r["fsum"], d["fsum"])
