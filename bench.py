@@ -20,6 +20,8 @@ import json
 import os
 import sys
 import time
+
+import numpy as np
 from pathlib import Path
 
 REPO = Path(__file__).resolve().parent
@@ -61,6 +63,52 @@ def cpu_baseline(gen, budget_s=15.0):
             "extension_rays": int(r["ext"]), "shadow_rays": int(r["shadow"])}
 
 
+def ray_bench(args, torch, linrt):
+    """Incoherent-ray traversal throughput (closest hit) on the same 1M-tri
+    mesh: area-uniform origins on the displaced sphere offset 1e-4 along the
+    normal, cosine-distributed directions (secondary rays)."""
+    sys.path.insert(0, str(REPO / "tests"))
+    from common import surface_rays
+    gen = linrt.displaced_sphere(seed=SEED)
+    scene = linrt.Scene(gen.desc_ptr, device=0)
+    pos, idx = gen.mesh(0)
+    n = args.rays
+    rays = surface_rays(pos, idx, n, seed=77)
+    d_rays = torch.from_numpy(rays).to("cuda:0")
+    d_op = torch.empty((n, 2), dtype=torch.int32, device="cuda:0")
+    d_t = torch.empty(n, dtype=torch.float32, device="cuda:0")
+    torch.cuda.synchronize()
+    a, b, c = scene.trace_count(rays)
+    bytes_per_ray = (32 + 16) + (8 * a.astype(np.int64) + 4 * b + 36 * c.astype(np.int64)).mean()
+    for _ in range(args.warmup):
+        scene.trace_closest_device(d_rays.data_ptr(), n, d_op.data_ptr(), d_t.data_ptr())
+    scene.sync()
+    scene.kernel_timing(linrt.KERNEL_CLOSEST, reset=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        scene.trace_closest_device(d_rays.data_ptr(), n, d_op.data_ptr(), d_t.data_ptr())
+    scene.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ms, nl, _ = scene.kernel_timing(linrt.KERNEL_CLOSEST)
+    avg = ms / max(nl, 1)
+    achieved = bytes_per_ray * n / (avg * 1e-3) / 1e9
+    hit = (d_op[:, 0] >= 0).float().mean().item()
+    out = {"metric": "M incoherent rays/s (closest hit, 1M tris)", "value": round(n * args.steps / el / 1e6, 2),
+           "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(el / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded secondary rays)",
+           "config": {"workload": f"{n} secondary rays on the displaced sphere (1,048,576 tris)"},
+           "hit_fraction": round(hit, 4),
+           "per_ray": {"nodes": float(a.mean()), "refs": float(b.mean()), "tris": float(c.mean()),
+                       "bytes": float(bytes_per_ray)},
+           "roofline": {"bound": "hbm", "kernel": "k_trace_closest", "achieved": round(achieved, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "avg_launch_ms": round(avg, 4)}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,6 +116,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--spp", type=int, default=SPP)
+    ap.add_argument("--rays", type=int, default=0, help="ray-traversal benchmark with this many rays instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by pixel/sample ranges "
@@ -76,6 +125,9 @@ def main():
 
     import torch
     import linrt
+
+    if args.rays:
+        return ray_bench(args, torch, linrt)
 
     if linrt.device_count() < 1 or not torch.cuda.is_available():
         log("no GPU visible")

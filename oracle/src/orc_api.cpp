@@ -124,7 +124,10 @@ int orc_scene_occluded(void* h, const float* rays, uint32_t n, const int32_t* ig
         const ray r = ray_at(rays, n, i, tmin, tmax);
         hit_rec rec;
         const bool any = S.hit(r, tmin, tmax, rec);
-        out[i] = (uint8_t)(any && rec.emitter != (ignore ? ignore[i] : -1));
+        // the reference compares against a real emitter (never null); -1 here
+        // means no emitter is exempt, so any hit blocks
+        const int ig = ignore ? ignore[i] : -1;
+        out[i] = (uint8_t)(any && (ig < 0 || rec.emitter != ig));
     });
     return 0;
 }

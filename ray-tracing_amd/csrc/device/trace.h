@@ -37,7 +37,16 @@ struct DevScene {
     DevObject obj[kMaxObjects];
 };
 
-constexpr int kLdsStack = 16;
+#ifndef LRT_LDS_STACK
+#define LRT_LDS_STACK 8
+#endif
+#ifndef LRT_LEAF_BATCH
+#define LRT_LEAF_BATCH 2
+#endif
+// stack entries kept in LDS per lane (the rest spill to private memory), and
+// triangles whose reference and vertex loads are issued together in a leaf
+constexpr int kLdsStack = LRT_LDS_STACK;
+constexpr int kLeafBatch = LRT_LEAF_BATCH;
 constexpr int kMaxStack = 48;
 
 struct Ray {
@@ -50,9 +59,14 @@ struct HitInfo {
     float u, v;
 };
 
+struct TriRec {
+    f3 p0, e1, e2;
+};
+LRT_D TriRec load_tri(const float* __restrict__ tri) { return TriRec{ld3(tri), ld3(tri + 3), ld3(tri + 6)}; }
+
 // Moller-Trumbore restated from triangle_mesh.h:43-79 with the voxel window
-LRT_D bool tri_test(const float* tri, const Ray& r, float t0, float t1, float& t, float& u, float& v) {
-    const f3 p0 = ld3(tri), e1 = ld3(tri + 3), e2 = ld3(tri + 6);
+LRT_D bool tri_test(const TriRec& T, const Ray& r, float t0, float t1, float& t, float& u, float& v) {
+    const f3 p0 = T.p0, e1 = T.e1, e2 = T.e2;
     const f3 pv = cross(r.d, e2);
     const float det = dot(e1, pv);
     if (fabsf(det) < 1e-8f) return false;
@@ -128,19 +142,33 @@ LRT_D bool kd_traverse(const DevScene& S, const DevObject& ob, const Ray& r, flo
             const uint32_t count = nd.y >> 2;
             const uint32_t first = nd.x;
             float limit = fminf(maxt, closest);
-            for (uint32_t k = 0; k < count; ++k) {
-                const uint32_t tri = S.refs[first + k];
-                if (COUNT) { cnt.refs++; cnt.tris++; }
-                float t, u, v;
-                if (tri_test(S.tris + 9 * (size_t)tri, r, mint, limit, t, u, v)) {
-                    closest = t;
-                    prim = (int32_t)(tri - ob.tri_base);
-                    hu = u;
-                    hv = v;
-                    found = true;
-                    if (ANY) return true;
+            for (uint32_t k = 0; k < count; k += kLeafBatch) {
+                // issue the reference and vertex loads of a batch together,
+                // then test in reference order (ties go to the later triangle)
+                uint32_t tri[kLeafBatch];
+                TriRec rec[kLeafBatch];
+#pragma unroll
+                for (int j = 0; j < kLeafBatch; ++j)
+                    if (k + j < count) tri[j] = S.refs[first + k + j];
+#pragma unroll
+                for (int j = 0; j < kLeafBatch; ++j)
+                    if (k + j < count) rec[j] = load_tri(S.tris + 9 * (size_t)tri[j]);
+#pragma unroll
+                for (int j = 0; j < kLeafBatch; ++j) {
+                    if (k + j < count) {
+                        if (COUNT) { cnt.refs++; cnt.tris++; }
+                        float t, u, v;
+                        if (tri_test(rec[j], r, mint, limit, t, u, v)) {
+                            closest = t;
+                            prim = (int32_t)(tri[j] - ob.tri_base);
+                            hu = u;
+                            hv = v;
+                            found = true;
+                            if (ANY) return true;
+                        }
+                        limit = fminf(maxt, closest);
+                    }
                 }
-                limit = fminf(maxt, closest);
             }
             if (top == 0) break;
             --top;

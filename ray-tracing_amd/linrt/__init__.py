@@ -25,7 +25,8 @@ import numpy as np
 
 _HERE = Path(__file__).resolve().parent
 PKG_ROOT = _HERE.parent
-LIB_PATH = PKG_ROOT / "liblinrt_hip.so"
+# LINRT_LIB selects another build of the same C ABI (e.g. a tuning variant)
+LIB_PATH = Path(os.environ.get("LINRT_LIB", str(PKG_ROOT / "liblinrt_hip.so")))
 DATA_DIR = PKG_ROOT / "data"
 
 OK = 0
