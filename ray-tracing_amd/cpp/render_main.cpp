@@ -1,0 +1,53 @@
+// linrt_render: C++ driver in the shape of the reference's src/main.cpp
+// (scene -> integrator -> timed frame loop -> image), rendering the seeded
+// box-icosphere scene on the GPU through linrt::gpu_path_integrator and
+// writing a little-endian PFM (FP32 RGB, bottom row first).
+//
+//   linrt_render OUT.pfm [width height spp max_depth rr_depth seed]
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iostream>
+
+#include "linrt.hpp"
+
+int main(int argc, char** argv) {
+    if (argc < 2) {
+        std::cerr << "usage: linrt_render OUT.pfm [width height spp max_depth rr_depth seed]\n";
+        return 2;
+    }
+    const uint32_t W = argc > 2 ? (uint32_t)std::atoi(argv[2]) : 256;
+    const uint32_t H = argc > 3 ? (uint32_t)std::atoi(argv[3]) : 256;
+    const uint32_t spp = argc > 4 ? (uint32_t)std::atoi(argv[4]) : 16;
+    const uint32_t max_depth = argc > 5 ? (uint32_t)std::atoi(argv[5]) : 8;
+    const uint32_t rr_depth = argc > 6 ? (uint32_t)std::atoi(argv[6]) : 3;
+    const uint64_t seed = argc > 7 ? (uint64_t)std::atoll(argv[7]) : 1;
+    try {
+        lrt_gen_params gp;
+        std::memset(&gp, 0, sizeof(gp));
+        gp.kind = LRT_GEN_BOX_ICOSPHERE;
+        gp.seed = 1;
+        gp.subdivisions = 5;
+        gp.aspect = (float)W / (float)H;
+        linrt::generated gen(gp);
+        linrt::scene scene(gen.desc());
+        const linrt::gpu_path_integrator integrator(max_depth, rr_depth, seed);
+        std::vector<linrt::color> pixels;
+        const auto t0 = std::chrono::steady_clock::now();
+        const lrt_render_stats st = integrator.render(scene, W, H, spp, pixels);
+        const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        std::printf("%u x %u x %u spp: %.3f s, %.2f Msamples/s, %llu extension + %llu shadow rays\n", W, H, spp, s,
+                    (double)st.samples / s / 1e6, (unsigned long long)st.extension_rays,
+                    (unsigned long long)st.shadow_rays);
+        FILE* f = std::fopen(argv[1], "wb");
+        if (!f) { std::perror(argv[1]); return 1; }
+        std::fprintf(f, "PF\n%u %u\n-1.0\n", W, H);
+        std::fwrite(pixels.data(), sizeof(linrt::color), pixels.size(), f);
+        std::fclose(f);
+    } catch (const linrt::error& e) {
+        std::cerr << "linrt error " << e.code << ": " << e.what() << "\n";
+        return 1;
+    }
+    return 0;
+}
