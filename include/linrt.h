@@ -164,6 +164,8 @@ typedef struct lrt_scene_info {
     uint64_t n_triangles, n_nodes, n_refs, device_bytes;
     int32_t max_tree_depth;
     double build_seconds, upload_seconds;
+    float bounding_radius;   /* scene::bounding_sphere_radius (scene.h:41-45) */
+    int32_t has_delta_bsdf;  /* scene::has_delta_bsdf (scene.h:48-57) */
 } lrt_scene_info;
 int lrt_scene_info_get(const lrt_scene* scene, lrt_scene_info* info);
 
@@ -263,13 +265,55 @@ int lrt_render(lrt_scene* scene, const lrt_render_params* params,
                double* sum_rgb, double* sumsq_rgb, int device_ptrs,
                lrt_render_stats* stats);
 
+/* ---- photon mapping (photon_integrator.h, photon_gather.h, photon_tracer.h,
+ *      photon_map.h) ---------------------------------------------------------
+ * A scene holds two photon maps, as photon_integrator does: kind 0 the global
+ * (surface) map and kind 1 the caustic map.  lrt_photon_build traces
+ * particles 0, 1, 2, ... in order until `requested` photons are returned
+ * (photon_gather.h:15-101 with one particle per work unit), keeps the first
+ * `requested`, and builds the reference's median kd tree over them
+ * (photon_map.h:83-114).  A gather that fails (photon_gather.h:99-101) leaves
+ * the map empty, like photon_integrator::preprocess (:25-36). */
+typedef struct lrt_photon_params {
+    int32_t kind;          /* 0 global (surface) map, 1 caustic map */
+    uint32_t requested;    /* photons to store (photon_integrator.h:62-63: 250000) */
+    uint32_t max_depth;    /* photon path depth (photon_integrator.h:27 passes max_depth - 1 = 127) */
+    uint32_t rr_depth;     /* photon_integrator.h:59: 5 */
+    uint64_t seed;
+    float radius;          /* query radius (photon_integrator.h:17-18: 0.05 / 0.0125 x bounding radius) */
+    uint32_t k;            /* photons per lookup, <= 128 (photon_integrator.h:64: 120) */
+} lrt_photon_params;
+
+typedef struct lrt_photon_stats {
+    uint64_t photons, shot, particles_traced, returned;
+    int32_t full;          /* photons == requested (photon_gather.h:59) */
+    double trace_ms, sort_ms, seconds;
+} lrt_photon_stats;
+
+int lrt_photon_build(lrt_scene* scene, const lrt_photon_params* params, lrt_photon_stats* stats);
+/* the map's photons in particle order: 12 floats (position, direction,
+ * normal, power) + (depth, particle) per photon; shot = particles traced */
+int lrt_photon_export(lrt_scene* scene, int32_t kind, float* photons12, uint32_t* depth_particle, uint64_t capacity,
+                      uint64_t* n, uint64_t* shot);
+/* builds map `kind` from given photons (scale = 1 / shot; n = 0 empties it) */
+int lrt_photon_map_load(lrt_scene* scene, int32_t kind, const float* photons12, const uint32_t* depth_particle,
+                        uint64_t n, float scale, float radius, uint32_t k);
+/* photon_map::estimate_irradiance of map `kind` at n (point, normal) queries (6 floats) */
+int lrt_photon_estimate(lrt_scene* scene, int32_t kind, const float* queries6, uint32_t n, int max_depth, float* out3);
+/* photon_integrator::radiance (photon_integrator.h:39-123) for every (pixel,
+ * sample) with the scene's maps (an empty map is skipped); max_depth is the
+ * integrator's depth limit (photon_integrator.h:57: 128), rr_depth unused. */
+int lrt_render_photon(lrt_scene* scene, const lrt_render_params* params, double* sum_rgb, double* sumsq_rgb,
+                      int device_ptrs, lrt_render_stats* stats);
+
 /* ---- seeded scene generators for the benchmark configurations --------- */
 /* Generated geometry lives in a library-owned lrt_generated; its desc()
  * pointer stays valid until lrt_generated_free. */
 typedef struct lrt_generated lrt_generated;
 enum {
     LRT_GEN_BOX_ICOSPHERE = 0,   /* closed box room + icosphere L5 + ceiling light */
-    LRT_GEN_DISPLACED_SPHERE = 1 /* displaced sphere, 7 material regions, ground, sky */
+    LRT_GEN_DISPLACED_SPHERE = 1, /* displaced sphere, 7 material regions, ground, sky */
+    LRT_GEN_PHOTON_BOX = 2        /* box room + displaced sphere (7 regions, radius 0.5) + 0.5x0.5 light */
 };
 typedef struct lrt_gen_params {
     int32_t kind;

@@ -6,7 +6,8 @@
 //
 // Private members of sah_kdtree (nodes, ordered), triangle_mesh (kdtree) and
 // rough_transmittance (trans, diff_trans) are read through the access macro
-// below; nothing is modified.
+// below; the photon integrator's maps and radii are set through it from given
+// photon lists (its own preprocess gathers in a thread-dependent order).
 #include "../include/linrt.h"
 
 #define private public
@@ -19,6 +20,9 @@
 #include "geometry/sphere.h"
 #include "geometry/triangle_mesh.h"
 #include "integrator/path/path_integrator.h"
+#include "integrator/photonmapper/photon_integrator.h"
+#include "integrator/photonmapper/photon_map.h"
+#include "integrator/photonmapper/photon_tracer.h"
 #include "material/coating_bsdf.h"
 #include "material/dielectric_bsdf.h"
 #include "material/diffuse_bsdf.h"
@@ -354,6 +358,59 @@ int ref_sampler_seq(uint32_t pixel, uint32_t sample, uint32_t n, float* out) {
     return 0;
 }
 
+// photon_tracer::trace_particle for particles [first, first+count), each with
+// its sampler started at (0, index) as photon_gather does (photon_gather.h:45),
+// one after another (the reference interleaves chunks over threads)
+int64_t ref_trace_particles(void* h, int kind, uint32_t first, uint32_t count, int max_depth, int rr_depth,
+                            float* out12, uint32_t* depth_particle, int64_t capacity) {
+    ref_scene& S = *(ref_scene*)h;
+    std::vector<photon> ph;
+    std::vector<uint32_t> owner;
+    for (uint32_t i = 0; i < count; ++i) {
+        LinRT::sampler smp;
+        smp.start(0, first + i);
+        const size_t before = ph.size();
+        photon_tracer::trace_particle(*S.world, kind == 1 ? photon_tracer::gather::caustic : photon_tracer::gather::surface,
+                                      smp, max_depth, rr_depth, ph);
+        owner.resize(ph.size(), first + i);
+        (void)before;
+    }
+    if (out12) {
+        const int64_t n = std::min<int64_t>((int64_t)ph.size(), capacity);
+        for (int64_t i = 0; i < n; ++i) {
+            const photon& p = ph[(size_t)i];
+            float* o = out12 + 12 * i;
+            o[0] = p.position.x; o[1] = p.position.y; o[2] = p.position.z;
+            o[3] = p.direction.x; o[4] = p.direction.y; o[5] = p.direction.z;
+            o[6] = p.normal.x; o[7] = p.normal.y; o[8] = p.normal.z;
+            o[9] = p.power.x; o[10] = p.power.y; o[11] = p.power.z;
+            depth_particle[2 * i] = p.depth;
+            depth_particle[2 * i + 1] = owner[(size_t)i];
+        }
+    }
+    return (int64_t)ph.size();
+}
+
+// photon_map::build + estimate_irradiance (photon_map.h:26-70)
+int ref_photon_estimate(const float* photons12, const uint32_t* depth_particle, int64_t n_photons, float scale,
+                        const float* queries6, uint32_t n_queries, float radius, int max_depth, int k, float* out3) {
+    std::vector<photon> ph((size_t)n_photons);
+    for (int64_t i = 0; i < n_photons; ++i) {
+        const float* p = photons12 + 12 * i;
+        photon& o = ph[(size_t)i];
+        o.position = V(p); o.direction = V(p + 3); o.normal = V(p + 6); o.power = V(p + 9);
+        o.depth = (std::uint16_t)depth_particle[2 * i];
+    }
+    photon_map m;
+    m.build(std::move(ph), scale);
+    for (uint32_t q = 0; q < n_queries; ++q) {
+        const color e = m.estimate_irradiance(V(queries6 + 6 * (size_t)q), V(queries6 + 6 * (size_t)q + 3), radius,
+                                              max_depth, (size_t)k);
+        out3[3 * (size_t)q] = e.x; out3[3 * (size_t)q + 1] = e.y; out3[3 * (size_t)q + 2] = e.z;
+    }
+    return 0;
+}
+
 // src/main.cpp:118-136 loop with the reference path_integrator (max depth 50,
 // RR from depth 5, path_integrator.h:173,19) and the reference sampler.
 // per_sample: 3 floats per (pixel, sample); fsum: FP32 per-pixel sum.
@@ -387,6 +444,56 @@ int ref_render(void* h, int W, int H, int spp, int row_begin, int row_end, float
             }
         }
     });
+    return 0;
+}
+
+// photon_integrator::radiance (photon_integrator.h:39-43) per (pixel, sample)
+// in the src/main.cpp:118-136 loop, with the global and caustic maps built
+// from the given photons (n = 0: empty map) and the given radii.
+int ref_render_photon(void* h, int W, int H, int spp, const float* g_ph, const uint32_t* g_dp, int64_t g_n,
+                      float g_scale, float g_radius, const float* c_ph, const uint32_t* c_dp, int64_t c_n,
+                      float c_scale, float c_radius, int row_begin, int row_end, float* per_sample, int threads) {
+    ref_scene& S = *(ref_scene*)h;
+    photon_integrator integ;
+    auto load = [](const float* ph, const uint32_t* dp, int64_t n) {
+        std::vector<photon> v((size_t)std::max<int64_t>(n, 0));
+        for (int64_t i = 0; i < n; ++i) {
+            const float* p = ph + 12 * i;
+            photon& o = v[(size_t)i];
+            o.position = V(p); o.direction = V(p + 3); o.normal = V(p + 6); o.power = V(p + 9);
+            o.depth = (std::uint16_t)dp[2 * i];
+        }
+        return v;
+    };
+    if (g_n > 0) integ.global_map.build(load(g_ph, g_dp, g_n), g_scale);
+    if (c_n > 0) integ.caustic_map.build(load(c_ph, c_dp, c_n), c_scale);
+    integ.global_radius = g_radius;
+    integ.caustic_radius = c_radius;
+    const int rb = row_end > row_begin ? row_begin : 0, re = row_end > row_begin ? row_end : H;
+    parallel_for((uint32_t)(re - rb), threads, [&](uint32_t ry) {
+        const int y = rb + (int)ry;
+        for (int x = 0; x < W; ++x)
+            for (int s = 0; s < spp; ++s) {
+                const auto pix = static_cast<uint32_t>(y * W + x);
+                LinRT::sampler smp;
+                smp.start(pix, static_cast<uint32_t>(s));
+                const auto off = smp.next_2d();
+                const auto u = (static_cast<float>(x) + off.x) / static_cast<float>(W - 1);
+                const auto v = (static_cast<float>(y) + off.y) / static_cast<float>(H - 1);
+                auto r = S.world->sensor().get_ray(u, v);
+                const color L = integ.radiance(r, *S.world, smp);
+                float* o = per_sample + ((size_t)pix * spp + s) * 3;
+                o[0] = L.x; o[1] = L.y; o[2] = L.z;
+            }
+    });
+    return 0;
+}
+
+// scene::bounding_sphere_radius and has_delta_bsdf (scene.h:41-57)
+int ref_scene_bounds(void* h, float* radius, int* has_delta) {
+    ref_scene& S = *(ref_scene*)h;
+    *radius = S.world->bounding_sphere_radius();
+    *has_delta = S.world->has_delta_bsdf() ? 1 : 0;
     return 0;
 }
 

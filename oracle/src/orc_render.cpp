@@ -69,6 +69,8 @@ uint32_t sobol_bits(uint32_t index, uint32_t dim) {
 }
 }  // namespace
 
+uint32_t orc_sobol_bits(uint32_t index, uint32_t dim) { return sobol_bits(index, dim); }
+
 struct sampler {
     int mode = 0;  // 0 Philox, 1 reference Sobol
     uint32_t pixel = 0, sample = 0, dim = 0, pixel_seed = 0;

@@ -11,6 +11,7 @@
 
 #include "host/host.hpp"
 #include "kernels.hip"
+#include "eye.hip"
 
 namespace lrt {
 
@@ -111,6 +112,46 @@ void check_materials(const lrt_material* m, uint32_t nm, uint32_t n_tables) {
     }
 }
 
+// bsdf::flags of a material on the host (the device's bsdf_flags)
+uint32_t host_bsdf_flags(const std::vector<lrt_material>& mats, int m) {
+    const lrt_material& M = mats[(size_t)m];
+    const uint32_t S = LRT_FLAG_SPECULAR, G = LRT_FLAG_GLOSSY, D = LRT_FLAG_DIFFUSE, R = LRT_FLAG_REFLECTIVE,
+                   T = LRT_FLAG_REFRACTIVE, DI = LRT_FLAG_DIELECTRIC, ME = LRT_FLAG_METALLIC;
+    switch (M.type) {
+        case LRT_BSDF_DIFFUSE: return D | R;
+        case LRT_BSDF_CONDUCTOR: return (M.alpha <= 0.0f ? S : G) | R | ME;
+        case LRT_BSDF_DIELECTRIC: return S | R | T | DI;
+        case LRT_BSDF_ROUGH_DIELECTRIC: return G | R | T | DI;
+        case LRT_BSDF_PLASTIC: return S | R | DI | D;
+        case LRT_BSDF_ROUGH_PLASTIC: return G | R | DI | D;
+        case LRT_BSDF_COATING:
+        case LRT_BSDF_ROUGH_COATING:
+            if (M.nested < 0 || (size_t)M.nested >= mats.size() || M.nested == m) return 0u;
+            return host_bsdf_flags(mats, M.nested) | (M.type == LRT_BSDF_COATING ? S : G) | R | DI;
+        default: return 0u;
+    }
+}
+
+// the largest number of specular components of any material
+int host_max_specular_components(const std::vector<lrt_material>& mats) {
+    int best = 0;
+    for (size_t m = 0; m < mats.size(); ++m) {
+        const lrt_material& M = mats[m];
+        auto base = [](const lrt_material& B) {
+            if (B.type == LRT_BSDF_DIELECTRIC) return 2;
+            if (B.type == LRT_BSDF_PLASTIC) return 1;
+            if (B.type == LRT_BSDF_CONDUCTOR && B.alpha <= 0.0f) return 1;
+            return 0;
+        };
+        int c = base(M);
+        if ((M.type == LRT_BSDF_COATING || M.type == LRT_BSDF_ROUGH_COATING) && M.nested >= 0 &&
+            (size_t)M.nested < mats.size())
+            c = base(mats[(size_t)M.nested]) + (M.type == LRT_BSDF_COATING ? 1 : 0);
+        best = std::max(best, c);
+    }
+    return best;
+}
+
 DevLight make_light(const lrt_light_desc& d, const std::vector<lrt_object_desc>& objs) {
     DevLight L;
     std::memset(&L, 0, sizeof(L));
@@ -183,6 +224,14 @@ struct Timer {
 
 using namespace lrt;
 
+struct PhotonState {
+    PhotonBuf buf;
+    DevBuf raw, per, sort, hist, perm, xyz, nodes, order, pos, pay;
+    PhotonMapDev map;
+    uint64_t n = 0, shot = 0;
+    bool valid = false;
+};
+
 struct lrt_scene {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -199,6 +248,14 @@ struct lrt_scene {
     DevBuf pstate, hits, shadow, queues, counters, stats, accum, accum2, totals;
     // ray-query staging
     DevBuf q_rays, q_out0, q_out1, q_out2, q_aux;
+    // photon mapping
+    PhotonState pm[2];   // global, caustic
+    DevBuf eye_items[2], eye_hits, eye_shadow, eye_query, eye_acc;
+    size_t eye_cap[2] = {0, 0}, eye_hit_cap = 0, eye_shadow_cap = 0, eye_query_cap = 0;
+    EmitCdf emit;
+    float bounding_radius = 0.0f;
+    bool has_delta = false;
+    int max_spec_components = 0;
     Timer timers[LRT_KERNEL_COUNT_];
     ~lrt_scene() {
         if (stream) (void)hipStreamDestroy(stream);
@@ -342,6 +399,72 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
             const lrt_light_desc& L = d.lights[d.objects[o].emitter];
             require(L.kind == LRT_LIGHT_AREA && L.object == (int)o, "object " + std::to_string(o) + ": emitter does not name this object");
         }
+    // emission CDF by luminance(power) over area lights (scene.h:21-34)
+    std::memset(&sc.emit, 0, sizeof(sc.emit));
+    float esum = 0.0f;
+    for (uint32_t i = 0; i < d.n_lights; ++i) {
+        const lrt_light_desc& L = d.lights[i];
+        if (L.kind != LRT_LIGHT_AREA) continue;
+        const float area = sc.S.obj[L.object].area;
+        const float pa = area <= 0.0f ? 0.0f : kPiF * area;
+        const float w = 0.212671f * (L.radiance[0] * pa) + 0.715160f * (L.radiance[1] * pa) + 0.072169f * (L.radiance[2] * pa);
+        if (w <= 0.0f) continue;
+        esum += w;
+        sc.emit.index[sc.emit.n] = (int32_t)i;
+        sc.emit.cdf[sc.emit.n] = esum;
+        sc.emit.n++;
+    }
+    sc.emit.total = esum;
+    // scene::bounding_sphere_radius and has_delta_bsdf (scene.h:41-57)
+    {
+        float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+        auto grow = [&](const float* q) {
+            for (int a = 0; a < 3; ++a) { lo[a] = std::min(lo[a], q[a]); hi[a] = std::max(hi[a], q[a]); }
+        };
+        std::vector<char> delta_mat(mats.size(), 0);
+        for (size_t m = 0; m < mats.size() && d.n_materials; ++m)
+            delta_mat[m] = (host_bsdf_flags(mats, (int)m) & LRT_FLAG_SPECULAR) != 0;
+        bool delta = false;
+        for (uint32_t o = 0; o < d.n_objects; ++o) {
+            const lrt_object_desc& od = d.objects[o];
+            float b0[3] = {INFINITY, INFINITY, INFINITY}, b1[3] = {-INFINITY, -INFINITY, -INFINITY};
+            auto bgrow = [&](const float* q) {
+                for (int a = 0; a < 3; ++a) { b0[a] = std::min(b0[a], q[a]); b1[a] = std::max(b1[a], q[a]); }
+            };
+            if (od.shape == LRT_SHAPE_MESH) {
+                for (uint32_t v = 0; v < od.n_vertices; ++v) bgrow(od.positions + 3 * (size_t)v);
+                if (od.triangle_material) {
+                    for (uint32_t t = 0; t < od.n_triangles; ++t)
+                        if (od.triangle_material[t] < delta_mat.size() && delta_mat[od.triangle_material[t]]) delta = true;
+                } else if (od.material >= 0 && (size_t)od.material < delta_mat.size() && delta_mat[(size_t)od.material]) {
+                    delta = true;
+                }
+            } else if (od.shape == LRT_SHAPE_RECTANGLE) {
+                float q[3];
+                bgrow(od.corner);
+                for (int a = 0; a < 3; ++a) q[a] = od.corner[a] + od.edge_u[a];
+                bgrow(q);
+                for (int a = 0; a < 3; ++a) q[a] = od.corner[a] + od.edge_v[a];
+                bgrow(q);
+                for (int a = 0; a < 3; ++a) q[a] = (od.corner[a] + od.edge_u[a]) + od.edge_v[a];
+                bgrow(q);
+            } else {
+                float q[3];
+                for (int a = 0; a < 3; ++a) q[a] = od.center[a] - od.radius;
+                bgrow(q);
+                for (int a = 0; a < 3; ++a) q[a] = od.center[a] + od.radius;
+                bgrow(q);
+                if (od.material >= 0 && (size_t)od.material < delta_mat.size() && delta_mat[(size_t)od.material]) delta = true;
+            }
+            grow(b0);
+            grow(b1);
+        }
+        float h[3];
+        for (int a = 0; a < 3; ++a) h[a] = (hi[a] - lo[a]) * 0.5f;
+        sc.bounding_radius = d.n_objects ? std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]) : 0.0f;
+        sc.has_delta = delta;
+        sc.max_spec_components = d.n_materials ? host_max_specular_components(mats) : 0;
+    }
     const CameraBasis cb = camera_basis(d.camera);
     std::memcpy(&sc.cam, &cb, sizeof(sc.cam));
     const auto t2 = std::chrono::steady_clock::now();
@@ -353,6 +476,8 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
     sc.info.max_tree_depth = max_depth;
     sc.info.build_seconds = std::chrono::duration<double>(t1 - t0).count();
     sc.info.upload_seconds = std::chrono::duration<double>(t2 - t1).count();
+    sc.info.bounding_radius = sc.bounding_radius;
+    sc.info.has_delta_bsdf = sc.has_delta ? 1 : 0;
 }
 
 uint32_t grid_for(const lrt_scene& sc, uint64_t n, int per_cu) {
@@ -715,9 +840,10 @@ int lrt_sky_eval(int hip_device, const lrt_light_desc* sky, const float* dirs, u
 }
 
 // ---- path tracer ----------------------------------------------------------------
-int lrt_render(lrt_scene* sc, const lrt_render_params* prm, double* sum_rgb, double* sumsq_rgb, int device_ptrs,
-               lrt_render_stats* stats) {
-    return guarded([&] {
+// pm: photon-map eye pass (k_shade<true> + k_photon_eye with the scene's map)
+static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum_rgb, double* sumsq_rgb,
+                        int device_ptrs, lrt_render_stats* stats) {
+    {
         require(sc && prm && sum_rgb, "null pointer");
         const uint32_t W = prm->width, H = prm->height;
         require(W >= 2 && H >= 2, "image must be at least 2x2");
@@ -849,7 +975,479 @@ int lrt_render(lrt_scene* sc, const lrt_render_params* prm, double* sum_rgb, dou
             stats->refs_read = work[1];
             stats->tris_tested = work[2];
         }
+    }
+}
+
+int lrt_render(lrt_scene* sc, const lrt_render_params* prm, double* sum_rgb, double* sumsq_rgb, int device_ptrs,
+               lrt_render_stats* stats) {
+    return guarded([&] { render_impl(sc, prm, sum_rgb, sumsq_rgb, device_ptrs, stats); });
+}
+
+// ---- photon mapping -----------------------------------------------------------------
+static void radix_sort(lrt_scene& sc, PhotonState& S, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t n,
+                       int bits, uint32_t*& out_k, uint32_t*& out_v) {
+    const uint32_t nb = std::max<uint32_t>(1, (n + kRsTile - 1) / kRsTile);
+    S.hist.alloc((size_t)nb * 16 * 4);
+    uint32_t* hist = S.hist.as<uint32_t>();
+    for (int shift = 0; shift < bits; shift += 4) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(kRsBlock), 0, sc.stream, k0, n, shift, hist, nb);
+        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, sc.stream, hist, nb * 16);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(kRsBlock), 0, sc.stream, k0, v0, k1, v1, n, shift, hist, nb);
+        std::swap(k0, k1);
+        std::swap(v0, v1);
+    }
+    HIPCHECK(hipGetLastError());
+    out_k = k0;
+    out_v = v0;
+}
+
+// raw photons (S.buf, count n_raw) -> kept in particle order -> kd tree
+// (host build from the positions) -> photons packed in tree order
+static void build_photon_map(lrt_scene& sc, PhotonState& S, uint32_t n_raw, uint32_t n_keep, float scale, float radius,
+                             uint32_t k) {
+    const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>((n_raw + 255) / 256, (uint32_t)sc.cus * 16));
+    S.sort.alloc((size_t)std::max<uint32_t>(n_raw, 1) * 4 * 4 + 4 * 256);
+    char* c = S.sort.as<char>();
+    uint32_t* ka = alloc_carve<uint32_t>(c, std::max<uint32_t>(n_raw, 1));
+    uint32_t* va = alloc_carve<uint32_t>(c, std::max<uint32_t>(n_raw, 1));
+    uint32_t* kb = alloc_carve<uint32_t>(c, std::max<uint32_t>(n_raw, 1));
+    uint32_t* vb = alloc_carve<uint32_t>(c, std::max<uint32_t>(n_raw, 1));
+    S.valid = false;
+    // 1. particle order: sort by particle << 8 | depth, keep the first n_keep
+    uint32_t *sk = ka, *sv = va;
+    if (n_raw) {
+        HIPCHECK(hipMemcpyAsync(ka, S.buf.key, (size_t)n_raw * 4, hipMemcpyDeviceToDevice, sc.stream));
+        hipLaunchKernelGGL(k_iota, dim3(g), dim3(256), 0, sc.stream, va, n_raw);
+        radix_sort(sc, S, ka, va, kb, vb, n_raw, 32, sk, sv);
+    }
+    S.perm.alloc((size_t)std::max<uint32_t>(n_keep, 1) * 4);
+    if (n_keep) HIPCHECK(hipMemcpyAsync(S.perm.p, sv, (size_t)n_keep * 4, hipMemcpyDeviceToDevice, sc.stream));
+    // 2. the kd tree over the kept positions, in particle order
+    S.xyz.alloc((size_t)std::max<uint32_t>(n_keep, 1) * 12);
+    if (n_keep)
+        hipLaunchKernelGGL(k_photon_positions, dim3(g), dim3(256), 0, sc.stream, S.buf, S.perm.as<uint32_t>(), n_keep,
+                           S.xyz.as<float>());
+    HIPCHECK(hipGetLastError());
+    std::vector<float> xyz((size_t)n_keep * 3);
+    if (n_keep) HIPCHECK(hipMemcpyAsync(xyz.data(), S.xyz.p, (size_t)n_keep * 12, hipMemcpyDeviceToHost, sc.stream));
+    HIPCHECK(hipStreamSynchronize(sc.stream));
+    PhotonKd kd;
+    photon_kd_build(xyz.data(), n_keep, kd);
+    require(kd.depth < kPkStack, "photon kd tree deeper than the search stack");
+    // 3. photons in tree order
+    S.nodes.alloc(std::max<size_t>(kd.nodes.size(), 4) * 4);
+    S.order.alloc((size_t)std::max<uint32_t>(n_keep, 1) * 4);
+    S.pos.alloc((size_t)std::max<uint32_t>(n_keep, 1) * 16);
+    S.pay.alloc((size_t)std::max<uint32_t>(n_keep, 1) * 36);
+    if (n_keep) {
+        HIPCHECK(hipMemcpyAsync(S.nodes.p, kd.nodes.data(), kd.nodes.size() * 4, hipMemcpyHostToDevice, sc.stream));
+        HIPCHECK(hipMemcpyAsync(S.order.p, kd.order.data(), (size_t)n_keep * 4, hipMemcpyHostToDevice, sc.stream));
+        hipLaunchKernelGGL(k_photon_pack, dim3(g), dim3(256), 0, sc.stream, S.buf, S.perm.as<uint32_t>(),
+                           S.order.as<uint32_t>(), n_keep, S.pos.as<float4>(), S.pay.as<float>());
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(sc.stream));
+    S.map.pos = S.pos.as<float4>();
+    S.map.pay = S.pay.as<float>();
+    S.map.nodes = S.nodes.as<uint4>();
+    S.map.n = n_keep;
+    S.map.scale = scale;
+    S.map.radius = radius;
+    S.map.k = (int32_t)k;
+    S.n = n_keep;
+    S.valid = true;
+}
+
+static void alloc_photons(PhotonState& S, uint32_t capacity) {
+    S.raw.alloc((size_t)capacity * 13 * 4 + 13 * 256 + 256);
+    char* c = S.raw.as<char>();
+    float** f[12] = {&S.buf.px, &S.buf.py, &S.buf.pz, &S.buf.dx, &S.buf.dy, &S.buf.dz,
+                     &S.buf.nx, &S.buf.ny, &S.buf.nz, &S.buf.wr, &S.buf.wg, &S.buf.wb};
+    for (float** p : f) *p = alloc_carve<float>(c, capacity);
+    S.buf.key = alloc_carve<uint32_t>(c, capacity);
+    S.buf.count = alloc_carve<uint32_t>(c, 64);
+    S.buf.capacity = capacity;
+}
+
+static PhotonState& photon_state(lrt_scene* sc, int32_t kind) {
+    require(kind == 0 || kind == 1, "kind must be 0 (global map) or 1 (caustic map)");
+    return sc->pm[kind];
+}
+
+int lrt_photon_build(lrt_scene* sc, const lrt_photon_params* prm, lrt_photon_stats* stats) {
+    return guarded([&] {
+        require(sc && prm, "null pointer");
+        require(prm->k >= 1 && prm->k <= (uint32_t)kMaxK, "k must be in [1, 128]");
+        require(prm->radius > 0.0f && prm->max_depth >= 1, "bad photon parameters");
+        require(prm->max_depth < 256, "photon depth is stored in 8 bits: max_depth must be < 256");
+        PhotonState& S = photon_state(sc, prm->kind);
+        HIPCHECK(hipSetDevice(sc->device));
+        const auto t0 = std::chrono::steady_clock::now();
+        S.valid = false;
+        S.n = 0;
+        S.shot = 0;
+        if (stats) std::memset(stats, 0, sizeof(*stats));
+        if (prm->requested == 0) return;
+        uint32_t capacity = prm->requested + prm->requested / 2 + 65536;
+        alloc_photons(S, capacity);
+        std::vector<uint32_t> per;  // photons per particle, particle order
+        uint64_t returned = 0;
+        uint32_t traced = 0;
+        double trace_ms = 0.0;
+        bool failed = false;
+        const uint2 key = make_uint2((uint32_t)prm->seed, (uint32_t)(prm->seed >> 32));
+        HIPCHECK(hipMemsetAsync(S.buf.count, 0, 4, sc->stream));
+        uint32_t batch = std::max<uint32_t>(65536, prm->requested / 4);
+        for (int iter = 0; iter < 256 && returned < prm->requested; ++iter) {
+            // photon_gather.h:99-101: stop when far too few photons are stored
+            if (traced > 100000 && (returned == 0 || returned < traced / 1024)) { failed = true; break; }
+            require((uint64_t)traced + batch < (1ull << 24), "photon tracing needs more than 2^24 particles");
+            S.per.alloc((size_t)batch * 4);
+            hipEvent_t a, b;
+            HIPCHECK(hipEventCreate(&a));
+            HIPCHECK(hipEventCreate(&b));
+            HIPCHECK(hipEventRecord(a, sc->stream));
+            PhotonTraceParams tp{traced, batch, prm->max_depth, prm->rr_depth, (uint32_t)prm->kind, key};
+            const uint32_t g = std::min<uint32_t>((batch + 255) / 256, (uint32_t)sc->cus * 8);
+            hipLaunchKernelGGL(k_photon_trace, dim3(g), dim3(256), 0, sc->stream, sc->S, sc->Ls, sc->M, sc->emit, S.buf,
+                               S.per.as<uint32_t>(), tp);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipEventRecord(b, sc->stream));
+            const size_t off = per.size();
+            per.resize(off + batch);
+            HIPCHECK(hipMemcpyAsync(per.data() + off, S.per.p, (size_t)batch * 4, hipMemcpyDeviceToHost, sc->stream));
+            uint32_t total = 0;
+            HIPCHECK(hipMemcpyAsync(&total, S.buf.count, 4, hipMemcpyDeviceToHost, sc->stream));
+            HIPCHECK(hipStreamSynchronize(sc->stream));
+            float ms = 0.0f;
+            HIPCHECK(hipEventElapsedTime(&ms, a, b));
+            (void)hipEventDestroy(a);
+            (void)hipEventDestroy(b);
+            trace_ms += ms;
+            traced += batch;
+            returned = total;
+            if (total > capacity) {  // grow and trace everything again
+                capacity = total + total / 4 + 65536;
+                alloc_photons(S, capacity);
+                HIPCHECK(hipMemsetAsync(S.buf.count, 0, 4, sc->stream));
+                per.clear();
+                traced = 0;
+                returned = 0;
+                continue;
+            }
+            if (returned < prm->requested) {
+                const double rate = returned > 0 ? (double)returned / traced : 0.01;
+                batch = (uint32_t)std::min<double>(1u << 22, std::max(4096.0, (prm->requested - returned) / rate * 1.1 + 1024));
+            }
+        }
+        // shot: particles up to the one whose photons first reach `requested`
+        uint64_t cum = 0, shot = traced;
+        for (uint32_t i = 0; i < traced; ++i) {
+            cum += per[i];
+            if (cum >= prm->requested) { shot = (uint64_t)i + 1; break; }
+        }
+        const bool full = !failed && returned >= prm->requested;
+        const auto t1 = std::chrono::steady_clock::now();
+        if (full) {
+            build_photon_map(*sc, S, (uint32_t)returned, prm->requested, 1.0f / (float)shot, prm->radius, prm->k);
+            S.shot = shot;
+        }
+        const auto t2 = std::chrono::steady_clock::now();
+        if (stats) {
+            stats->photons = full ? prm->requested : std::min<uint64_t>(returned, prm->requested);
+            stats->shot = shot;
+            stats->particles_traced = traced;
+            stats->returned = returned;
+            stats->full = full ? 1 : 0;
+            stats->trace_ms = trace_ms;
+            stats->sort_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+            stats->seconds = std::chrono::duration<double>(t2 - t0).count();
+        }
     });
+}
+
+int lrt_photon_map_load(lrt_scene* sc, int32_t kind, const float* photons12, const uint32_t* depth_particle, uint64_t n,
+                        float scale, float radius, uint32_t k) {
+    return guarded([&] {
+        require(sc && (n == 0 || (photons12 && depth_particle)), "null pointer");
+        require(k >= 1 && k <= (uint32_t)kMaxK && radius > 0.0f, "bad map parameters");
+        require(n < (1ull << 24), "photon map load: more than 2^24 photons");
+        PhotonState& S = photon_state(sc, kind);
+        HIPCHECK(hipSetDevice(sc->device));
+        S.valid = false;
+        S.n = 0;
+        if (n == 0) return;
+        alloc_photons(S, (uint32_t)n);
+        std::vector<float> col((size_t)n);
+        float* dst[12] = {S.buf.px, S.buf.py, S.buf.pz, S.buf.dx, S.buf.dy, S.buf.dz,
+                          S.buf.nx, S.buf.ny, S.buf.nz, S.buf.wr, S.buf.wg, S.buf.wb};
+        for (int j = 0; j < 12; ++j) {
+            for (uint64_t i = 0; i < n; ++i) col[i] = photons12[12 * i + j];
+            HIPCHECK(hipMemcpy(dst[j], col.data(), n * 4, hipMemcpyHostToDevice));
+        }
+        // keys keep the given order: position << 8 | depth
+        std::vector<uint32_t> keys((size_t)n);
+        for (uint64_t i = 0; i < n; ++i) {
+            require(depth_particle[2 * i] < 256, "photon depth must be < 256");
+            keys[i] = ((uint32_t)i << 8) | depth_particle[2 * i];
+        }
+        HIPCHECK(hipMemcpy(S.buf.key, keys.data(), n * 4, hipMemcpyHostToDevice));
+        build_photon_map(*sc, S, (uint32_t)n, (uint32_t)n, scale, radius, k);
+        S.shot = (uint64_t)std::llround(1.0 / scale);
+    });
+}
+
+int lrt_photon_estimate(lrt_scene* sc, int32_t kind, const float* queries6, uint32_t n, int max_depth, float* out3) {
+    return guarded([&] {
+        require(sc && queries6 && out3, "null pointer");
+        PhotonState& S = photon_state(sc, kind);
+        if (n == 0) return;
+        if (!S.valid || S.n == 0) {  // photon_map::estimate_irradiance on an empty map
+            std::memset(out3, 0, (size_t)n * 12);
+            return;
+        }
+        HIPCHECK(hipSetDevice(sc->device));
+        upload(sc->q_rays, queries6, (size_t)n * 24, sc->stream);
+        sc->q_out0.alloc((size_t)n * 12);
+        const uint32_t g = std::min<uint32_t>((n + kGatherBlock - 1) / kGatherBlock, (uint32_t)sc->cus * 16);
+        hipLaunchKernelGGL(k_photon_estimate, dim3(g), dim3(kGatherBlock), 0, sc->stream, S.map, sc->q_rays.as<float>(), n,
+                           max_depth, sc->q_out0.as<float>());
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(out3, sc->q_out0.p, (size_t)n * 12, hipMemcpyDeviceToHost, sc->stream));
+        HIPCHECK(hipStreamSynchronize(sc->stream));
+    });
+}
+
+int lrt_photon_export(lrt_scene* sc, int32_t kind, float* photons12, uint32_t* depth_particle, uint64_t capacity,
+                      uint64_t* n, uint64_t* shot) {
+    return guarded([&] {
+        require(sc != nullptr, "null pointer");
+        PhotonState& S = photon_state(sc, kind);
+        const uint64_t count = S.valid ? S.n : 0;
+        if (n) *n = count;
+        if (shot) *shot = S.valid ? S.shot : 0;
+        if (!photons12 || count == 0) return;
+        require(depth_particle != nullptr, "null pointer");
+        HIPCHECK(hipSetDevice(sc->device));
+        const uint64_t m = std::min<uint64_t>(capacity, count);
+        std::vector<uint32_t> perm((size_t)m);
+        HIPCHECK(hipMemcpy(perm.data(), S.perm.p, m * 4, hipMemcpyDeviceToHost));
+        const uint32_t cap = S.buf.capacity;
+        std::vector<float> col(cap);
+        float* src[12] = {S.buf.px, S.buf.py, S.buf.pz, S.buf.dx, S.buf.dy, S.buf.dz,
+                          S.buf.nx, S.buf.ny, S.buf.nz, S.buf.wr, S.buf.wg, S.buf.wb};
+        for (int j = 0; j < 12; ++j) {
+            HIPCHECK(hipMemcpy(col.data(), src[j], (size_t)cap * 4, hipMemcpyDeviceToHost));
+            for (uint64_t i = 0; i < m; ++i) photons12[12 * i + j] = col[perm[i]];
+        }
+        std::vector<uint32_t> keys(cap);
+        HIPCHECK(hipMemcpy(keys.data(), S.buf.key, (size_t)cap * 4, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < m; ++i) {
+            depth_particle[2 * i] = keys[perm[i]] & 0xffu;
+            depth_particle[2 * i + 1] = keys[perm[i]] >> 8;
+        }
+    });
+}
+
+// ---- photon-map eye pass ----------------------------------------------------------------
+static void carve_items(DevBuf& b, size_t cap, EyeItems& It) {
+    b.alloc(cap * 16 * 4 + 16 * 256);
+    char* c = b.as<char>();
+    It.sid = alloc_carve<uint32_t>(c, cap);
+    It.path = alloc_carve<uint32_t>(c, cap);
+    It.meta = alloc_carve<uint32_t>(c, cap);
+    float** f[13] = {&It.ox, &It.oy, &It.oz, &It.dx, &It.dy, &It.dz, &It.tr, &It.tg, &It.tb, &It.wr, &It.wg, &It.wb, &It.pdf};
+    for (float** p : f) *p = alloc_carve<float>(c, cap);
+}
+
+static void carve_hits(DevBuf& b, size_t cap, HitBuf& Hb) {
+    b.alloc(cap * 5 * 4 + 5 * 256);
+    char* c = b.as<char>();
+    Hb.obj = alloc_carve<int32_t>(c, cap);
+    Hb.prim = alloc_carve<int32_t>(c, cap);
+    Hb.t = alloc_carve<float>(c, cap);
+    Hb.u = alloc_carve<float>(c, cap);
+    Hb.v = alloc_carve<float>(c, cap);
+}
+
+static void carve_shadow(DevBuf& b, size_t cap, EyeShadow& Sh) {
+    b.alloc(cap * 12 * 4 + 12 * 256);
+    char* c = b.as<char>();
+    Sh.sid = alloc_carve<uint32_t>(c, cap);
+    float** f[10] = {&Sh.ox, &Sh.oy, &Sh.oz, &Sh.dx, &Sh.dy, &Sh.dz, &Sh.tmax, &Sh.cr, &Sh.cg, &Sh.cb};
+    for (float** p : f) *p = alloc_carve<float>(c, cap);
+    Sh.ignore = alloc_carve<int32_t>(c, cap);
+}
+
+static void carve_query(DevBuf& b, size_t cap, EyeQuery& Q) {
+    b.alloc(cap * 14 * 4 + 14 * 256);
+    char* c = b.as<char>();
+    Q.sid = alloc_carve<uint32_t>(c, cap);
+    Q.meta = alloc_carve<uint32_t>(c, cap);
+    float** f[12] = {&Q.px, &Q.py, &Q.pz, &Q.nx, &Q.ny, &Q.nz, &Q.tr, &Q.tg, &Q.tb, &Q.wr, &Q.wg, &Q.wb};
+    for (float** p : f) *p = alloc_carve<float>(c, cap);
+}
+
+// grows a buffer by half again when it must be reallocated
+static size_t grow_cap(size_t have, size_t need) { return need <= have ? have : std::max(need, have + have / 2); }
+
+static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum_rgb, double* sumsq_rgb,
+                               int device_ptrs, lrt_render_stats* stats) {
+    require(sc && prm && sum_rgb, "null pointer");
+    require(prm->width >= 2 && prm->height >= 2 && prm->max_depth >= 1 && prm->max_depth < 65535, "bad render parameters");
+    const uint32_t W = prm->width, H = prm->height;
+    const uint32_t rb = prm->row_end > prm->row_begin ? prm->row_begin : 0;
+    const uint32_t re = prm->row_end > prm->row_begin ? prm->row_end : H;
+    require(re <= H, "row range outside the image");
+    require((uint64_t)W * H < (1ull << 31), "image too large");
+    HIPCHECK(hipSetDevice(sc->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t img = (size_t)W * H * 3;
+    double* dsum = sum_rgb;
+    double* dsq = sumsq_rgb;
+    if (!device_ptrs) {
+        sc->accum.alloc(img * 8);
+        HIPCHECK(hipMemcpyAsync(sc->accum.p, sum_rgb, img * 8, hipMemcpyHostToDevice, sc->stream));
+        dsum = sc->accum.as<double>();
+        if (sumsq_rgb) {
+            sc->accum2.alloc(img * 8);
+            HIPCHECK(hipMemcpyAsync(sc->accum2.p, sumsq_rgb, img * 8, hipMemcpyHostToDevice, sc->stream));
+            dsq = sc->accum2.as<double>();
+        }
+    }
+    const int max_spec = sc->max_spec_components;
+    const uint32_t maps = (sc->pm[0].valid && sc->pm[0].n ? 1u : 0u) | (sc->pm[1].valid && sc->pm[1].n ? 2u : 0u);
+    PhotonMapDev G0 = sc->pm[0].map, G1 = sc->pm[1].map;
+    if (!(maps & 1u)) std::memset(&G0, 0, sizeof(G0));
+    if (!(maps & 2u)) std::memset(&G1, 0, sizeof(G1));
+    const uint64_t pixels = (uint64_t)(re - rb) * W;
+    const uint32_t chunk = (uint32_t)std::min<uint64_t>(pixels, 1u << 18);
+    sc->counters.alloc(256);
+    uint32_t* counters = sc->counters.as<uint32_t>();
+    sc->eye_acc.alloc((size_t)chunk * 3 * 8);
+    unsigned long long* acc = sc->eye_acc.as<unsigned long long>();
+    const uint2 key = make_uint2((uint32_t)prm->seed, (uint32_t)(prm->seed >> 32));
+    uint64_t n_ext = 0, n_shadow = 0, n_query = 0;
+    Timer& tc = sc->timers[LRT_KERNEL_CLOSEST];
+    Timer& to = sc->timers[LRT_KERNEL_OCCLUDED];
+    Timer& ts = sc->timers[LRT_KERNEL_SHADE];
+    HIPCHECK(hipStreamSynchronize(sc->stream));
+    tc.collect();
+    to.collect();
+    ts.collect();
+    const double before_c = tc.total_ms, before_o = to.total_ms, before_s = ts.total_ms;
+    const uint64_t lc0 = tc.launches, lo0 = to.launches;
+    for (uint32_t s = 0; s < prm->spp; ++s) {
+        for (uint64_t c0 = 0; c0 < pixels; c0 += chunk) {
+            const uint32_t nc = (uint32_t)std::min<uint64_t>(chunk, pixels - c0);
+            EyeParams ep{prm->max_depth, prm->spp_offset + s, (uint32_t)((uint64_t)rb * W + c0), key, maps, 0, 0, 0};
+            int cur = 0;
+            sc->eye_cap[cur] = grow_cap(sc->eye_cap[cur], nc);
+            EyeItems A, B;
+            carve_items(sc->eye_items[cur], sc->eye_cap[cur], A);
+            hipLaunchKernelGGL(k_fill_u64, dim3(std::min<uint32_t>((3 * nc + 255) / 256, 4096)), dim3(256), 0, sc->stream,
+                               acc, (size_t)3 * nc);
+            hipLaunchKernelGGL(k_eye_raygen, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, sc->cam, W, H,
+                               nc, ep, A);
+            HIPCHECK(hipGetLastError());
+            uint32_t n = nc;
+            for (uint32_t depth = 1; n > 0; ++depth) {
+                // children bound: 32 BSDF samples at depth 1, else 1, plus the
+                // specular components while depth < 4
+                const size_t per = (depth == 1 ? 32 : 1) + (depth < 4 ? (size_t)max_spec : 0);
+                const size_t need_children = (size_t)n * per;
+                const size_t need_shadow = (size_t)n * (depth == 1 ? 16 : 1);
+                const int nxt = cur ^ 1;
+                // buffers only grow, and only after the kernels that read them are done
+                if (need_children > sc->eye_cap[nxt] || n > sc->eye_hit_cap || need_shadow > sc->eye_shadow_cap ||
+                    (size_t)n * 2 > sc->eye_query_cap)
+                    HIPCHECK(hipStreamSynchronize(sc->stream));
+                sc->eye_cap[nxt] = grow_cap(sc->eye_cap[nxt], need_children);
+                carve_items(sc->eye_items[nxt], sc->eye_cap[nxt], B);
+                carve_items(sc->eye_items[cur], sc->eye_cap[cur], A);
+                sc->eye_hit_cap = grow_cap(sc->eye_hit_cap, n);
+                HitBuf Hb;
+                carve_hits(sc->eye_hits, sc->eye_hit_cap, Hb);
+                sc->eye_shadow_cap = grow_cap(sc->eye_shadow_cap, need_shadow);
+                EyeShadow Sh;
+                carve_shadow(sc->eye_shadow, sc->eye_shadow_cap, Sh);
+                sc->eye_query_cap = grow_cap(sc->eye_query_cap, (size_t)n * 2);
+                EyeQuery Q;
+                carve_query(sc->eye_query, sc->eye_query_cap, Q);
+                ep.cap_items = (uint32_t)std::min<size_t>(sc->eye_cap[nxt], 0xffffffffu);
+                ep.cap_shadow = (uint32_t)std::min<size_t>(sc->eye_shadow_cap, 0xffffffffu);
+                ep.cap_query = (uint32_t)std::min<size_t>(sc->eye_query_cap, 0xffffffffu);
+                const uint32_t zero4[4] = {0, 0, 0, 0};
+                HIPCHECK(hipMemcpyAsync(counters, zero4, sizeof(zero4), hipMemcpyHostToDevice, sc->stream));
+                const uint32_t g = (uint32_t)std::min<uint64_t>((n + kBlock - 1) / kBlock, (uint64_t)sc->cus * 16);
+                auto e1 = tc.next();
+                HIPCHECK(hipEventRecord(e1.first, sc->stream));
+                hipLaunchKernelGGL(k_eye_extend, dim3(g), dim3(kBlock), 0, sc->stream, sc->S, A, n, Hb);
+                HIPCHECK(hipEventRecord(e1.second, sc->stream));
+                auto e2 = ts.next();
+                HIPCHECK(hipEventRecord(e2.first, sc->stream));
+                hipLaunchKernelGGL(k_eye_shade, dim3(g), dim3(kBlock), 0, sc->stream, sc->S, sc->Ls, sc->M, A, n, Hb, B, Sh,
+                                   Q, counters, ep, acc);
+                HIPCHECK(hipEventRecord(e2.second, sc->stream));
+                HIPCHECK(hipGetLastError());
+                uint32_t cnt[4] = {0, 0, 0, 0};
+                HIPCHECK(hipMemcpyAsync(cnt, counters, sizeof(cnt), hipMemcpyDeviceToHost, sc->stream));
+                HIPCHECK(hipStreamSynchronize(sc->stream));
+                require(cnt[0] <= ep.cap_items && cnt[1] <= ep.cap_shadow && cnt[2] <= ep.cap_query,
+                        "eye pass queue overflow");
+                n_ext += n;
+                n_shadow += cnt[1];
+                n_query += cnt[2];
+                if (cnt[1]) {
+                    auto e3 = to.next();
+                    HIPCHECK(hipEventRecord(e3.first, sc->stream));
+                    hipLaunchKernelGGL(k_eye_shadow,
+                                       dim3((uint32_t)std::min<uint64_t>((cnt[1] + kBlock - 1) / kBlock, (uint64_t)sc->cus * 16)),
+                                       dim3(kBlock), 0, sc->stream, sc->S, Sh, cnt[1], acc);
+                    HIPCHECK(hipEventRecord(e3.second, sc->stream));
+                }
+                if (cnt[2])
+                    hipLaunchKernelGGL(k_eye_gather,
+                                       dim3((uint32_t)std::min<uint64_t>((cnt[2] + kGatherBlock - 1) / kGatherBlock,
+                                                                         (uint64_t)sc->cus * 16)),
+                                       dim3(kGatherBlock), 0, sc->stream, G0, G1, Q, cnt[2], acc);
+                HIPCHECK(hipGetLastError());
+                cur = nxt;
+                std::swap(A, B);
+                n = cnt[0];
+            }
+            hipLaunchKernelGGL(k_eye_resolve, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, acc, nc,
+                               ep.pixel0, dsum, dsq);
+            HIPCHECK(hipGetLastError());
+        }
+    }
+    if (!device_ptrs) {
+        HIPCHECK(hipMemcpyAsync(sum_rgb, dsum, img * 8, hipMemcpyDeviceToHost, sc->stream));
+        if (sumsq_rgb) HIPCHECK(hipMemcpyAsync(sumsq_rgb, dsq, img * 8, hipMemcpyDeviceToHost, sc->stream));
+    }
+    HIPCHECK(hipStreamSynchronize(sc->stream));
+    const auto t1 = std::chrono::steady_clock::now();
+    tc.collect();
+    to.collect();
+    ts.collect();
+    if (stats) {
+        std::memset(stats, 0, sizeof(*stats));
+        stats->samples = pixels * prm->spp;
+        stats->extension_rays = n_ext;
+        stats->shadow_rays = n_shadow;
+        stats->segments = n_query;
+        stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+        stats->trace_ms = tc.total_ms - before_c;
+        stats->occluded_ms = to.total_ms - before_o;
+        stats->shade_ms = ts.total_ms - before_s;
+        stats->trace_launches = tc.launches - lc0;
+        stats->occluded_launches = to.launches - lo0;
+    }
+}
+
+int lrt_render_photon(lrt_scene* sc, const lrt_render_params* prm, double* sum_rgb, double* sumsq_rgb, int device_ptrs,
+                      lrt_render_stats* stats) {
+    return guarded([&] { render_photon_impl(sc, prm, sum_rgb, sumsq_rgb, device_ptrs, stats); });
 }
 
 // ---- generators ------------------------------------------------------------------

@@ -223,6 +223,34 @@ LRT_D uint32_t bsdf_flags(const Mats& S, int m) {
     if (M.type == LRT_BSDF_ROUGH_COATING) return base_flags(S.m[M.nested]) | (F_GLOSSY | F_REFLECTIVE | F_DIELECTRIC);
     return base_flags(M);
 }
+// component_count / component_flags (bsdf.h:60-62 and the overrides in
+// dielectric_bsdf.h:23-29, plastic_bsdf.h:33-39, rough_plastic_bsdf.h:39-43,
+// coating_bsdf.h:33-39, rough_coating_bsdf.h:39-45)
+LRT_D int bsdf_component_count(const Mats& S, int m) {
+    const lrt_material& M = S.m[m];
+    if (M.type == LRT_BSDF_COATING || M.type == LRT_BSDF_ROUGH_COATING) return base_count(S.m[M.nested]) + 1;
+    return base_count(M);
+}
+LRT_D uint32_t base_component_flags(const lrt_material& M, int c) {
+    switch (M.type) {
+        case LRT_BSDF_DIELECTRIC:
+            return c == 0 ? (F_SPECULAR | F_REFLECTIVE | F_DIELECTRIC) : (F_SPECULAR | F_REFRACTIVE | F_DIELECTRIC);
+        case LRT_BSDF_PLASTIC: return c == 0 ? (F_SPECULAR | F_REFLECTIVE | F_DIELECTRIC) : (F_DIFFUSE | F_REFLECTIVE);
+        case LRT_BSDF_ROUGH_PLASTIC: return c == 0 ? (F_GLOSSY | F_REFLECTIVE | F_DIELECTRIC) : (F_DIFFUSE | F_REFLECTIVE);
+        default: return base_flags(M);
+    }
+}
+LRT_D uint32_t bsdf_component_flags(const Mats& S, int m, int c) {
+    const lrt_material& M = S.m[m];
+    if (M.type == LRT_BSDF_COATING || M.type == LRT_BSDF_ROUGH_COATING) {
+        const lrt_material& N = S.m[M.nested];
+        if (c == base_count(N))
+            return M.type == LRT_BSDF_COATING ? (F_SPECULAR | F_REFLECTIVE | F_DIELECTRIC)
+                                              : (F_GLOSSY | F_REFLECTIVE | F_DIELECTRIC);
+        return base_component_flags(N, c);
+    }
+    return base_component_flags(M, c);
+}
 
 LRT_D void clear(SRec& r) {
     r.pdf = 0.0f;

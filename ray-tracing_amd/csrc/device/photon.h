@@ -1,0 +1,395 @@
+// Photon mapping on the device.
+//
+// Particle tracing restates photon_tracer::trace_particle (photon_tracer.h:
+// 23-96) with emitter rays chosen by luminance(power) (scene.h:113-142,
+// area_emitter.h:93-122); one lane per particle, photons appended with a wave
+// ballot and later ordered by (particle, depth) with a hand-written LSD radix
+// sort, so the stored set does not depend on scheduling.  The photon map is
+// the reference's median kd tree (photon_map.h:83-114, built on the host with
+// the same partitions), with photons stored in tree order; the k-nearest
+// search (photon_map.h:117-165) walks it with a per-lane stack and a max-heap
+// of (distance, tree index) pairs in LDS, and the irradiance sum follows
+// estimate_irradiance (photon_map.h:38-70), including its heap pop order.
+#pragma once
+
+#include "bsdf.h"
+#include "emit.h"
+
+namespace lrt {
+
+struct PhotonBuf {  // SoA, capacity entries
+    float *px, *py, *pz, *dx, *dy, *dz, *nx, *ny, *nz, *wr, *wg, *wb;
+    uint32_t* key;     // particle << 8 | depth
+    uint32_t* count;   // [0]: photons appended (may exceed capacity)
+    uint32_t capacity;
+};
+
+struct EmitCdf {
+    int32_t n;
+    int32_t index[kMaxLights];
+    float cdf[kMaxLights];
+    float total;
+};
+
+// scene::sample_emitter_ray + area_emitter::sample_ray (rectangles)
+LRT_D bool emitter_ray(const DevScene& S, const DevLights& Ls, const EmitCdf& E, float sx, float sy, float dx, float dy,
+                       Ray& r, f3& weight) {
+    if (E.total <= 0.0f || E.n <= 0) return false;
+    const float target = sx * E.total;
+    int bin = 0;
+    while (bin < E.n && !(target < E.cdf[bin])) ++bin;  // upper_bound
+    if (bin >= E.n) bin = E.n - 1;
+    const float prev = bin == 0 ? 0.0f : E.cdf[bin - 1];
+    const float span = E.cdf[bin] - prev;
+    if (span <= 0.0f) return false;
+    const float sel = span / E.total;
+    const float u = (target - prev) / span;
+    const DevLight& L = Ls.l[E.index[bin]];
+    const DevObject& ob = S.obj[L.object];
+    if (ob.area <= 0.0f) return false;
+    const f3 p = ld3(ob.corner) + u * ld3(ob.eu) + sy * ld3(ob.ev);
+    const f3 n = ld3(ob.rn);
+    const float ct = sqrtf(dx), st = sqrtf(1.0f - dx);
+    const float phi = 2.0f * kPiF * dy;
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    const Frame f = frame_from(n);
+    r = Ray{p, to_world(f, f3{cp * st, sp * st, ct})};
+    const f3 power = ld3(L.radiance) * (kPiF * ob.area);
+    weight = f3{power.x / sel, power.y / sel, power.z / sel};
+    return true;
+}
+
+struct PhotonTraceParams {
+    uint32_t first, count, max_depth, rr_depth, kind;
+    uint2 key;
+};
+
+LRT_D float particle_u(uint32_t index, uint32_t tag, uint32_t d, uint2 key) {
+    const uint4 v = philox(make_uint4(index, tag, d >> 2, 0u), key);
+    return u24(pick(v, d & 3u));
+}
+
+__global__ __launch_bounds__(256) void k_photon_trace(const DevScene S, const DevLights Ls, const Mats M, const EmitCdf E,
+                                                      PhotonBuf B, uint32_t* __restrict__ per_particle,
+                                                      PhotonTraceParams prm) {
+    __shared__ uint2 stack[kLdsStack * 256];
+    uint2* lds = stack + threadIdx.x;
+    const uint32_t tag = 0xF0000000u + prm.kind;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t n_round = (prm.count + stride - 1) / stride * stride;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < n_round; t += stride) {
+        const bool valid = t < prm.count;
+        const uint32_t index = prm.first + t;
+        uint32_t d = 0, stored = 0;
+        Ray r;
+        f3 power;
+        bool alive = false;
+        if (valid) {
+            const float sx = particle_u(index, tag, 0, prm.key), sy = particle_u(index, tag, 1, prm.key);
+            const float ex = particle_u(index, tag, 2, prm.key), ey = particle_u(index, tag, 3, prm.key);
+            d = 4;
+            alive = emitter_ray(S, Ls, E, sx, sy, ex, ey, r, power);
+        }
+        f3 T = splat(1.0f);
+        bool delta = false;
+        for (uint32_t depth = 1; depth <= prm.max_depth;) {
+            bool store = false;
+            SurfaceHit sh;
+            if (alive) {
+                HitInfo h;
+                Counters c{0, 0, 0};
+                if (!trace_closest<false>(S, r, 0.001f, kInfF, h, lds, c)) {
+                    alive = false;
+                } else {
+                    sh = surface(S, r, h.obj, h.prim, h.t, h.u, h.v);
+                    if (sh.material < 0) alive = false;
+                }
+                if (alive) {
+                    const uint32_t flags = bsdf_flags(M, sh.material);
+                    const bool smooth = (flags & F_DIFFUSE) != 0 || (flags & F_GLOSSY) != 0;
+                    store = smooth && depth > 1 && (prm.kind == 1 ? delta : !delta);
+                }
+            }
+            // photon append (store_if_accepted, photon_tracer.h:80-96)
+            const uint32_t slot = wave_append(B.count, store);
+            if (store && slot < B.capacity) {
+                const f3 dir = normalize(r.d);
+                const f3 pw = T * power;
+                B.px[slot] = sh.p.x; B.py[slot] = sh.p.y; B.pz[slot] = sh.p.z;
+                B.dx[slot] = dir.x; B.dy[slot] = dir.y; B.dz[slot] = dir.z;
+                B.nx[slot] = sh.n.x; B.ny[slot] = sh.n.y; B.nz[slot] = sh.n.z;
+                B.wr[slot] = pw.x; B.wg[slot] = pw.y; B.wb[slot] = pw.z;
+                B.key[slot] = (index << 8) | depth;
+            }
+            stored += store ? 1u : 0u;
+            if (__ballot(alive) == 0) break;
+            if (alive) {
+                const Frame fr = frame_from(sh.n);
+                BsdfCtx ctx;
+                ctx.mode = LRT_MODE_IMPORTANCE;
+                ctx.lobe = particle_u(index, tag, d, prm.key);
+                SRec sr;
+                sr.wi = to_local(fr, -normalize(r.d));
+                sr.front = sh.front;
+                const float ux = particle_u(index, tag, d + 1, prm.key), uy = particle_u(index, tag, d + 2, prm.key);
+                d += 3;
+                const f3 w = bsdf_sample(M, sh.material, ctx, sr, ux, uy);
+                if (is_zero(w)) {
+                    alive = false;
+                } else {
+                    const f3 outgoing = to_world(fr, sr.wo);
+                    const f3 incoming = to_world(fr, sr.wi);
+                    if (dot(sh.n, incoming) * sr.wi.z <= 0.0f || dot(sh.n, outgoing) * sr.wo.z <= 0.0f) {
+                        alive = false;
+                    } else {
+                        T = T * w;
+                        delta = (sr.type & F_SPECULAR) != 0;
+                        r = Ray{sh.p, outgoing};
+                        if (depth >= prm.rr_depth) {
+                            const float surv = fminf(fmaxf(fmaxf(T.x, T.y), T.z), 0.95f);
+                            if (particle_u(index, tag, d, prm.key) >= surv) alive = false;
+                            else T = f3{T.x / surv, T.y / surv, T.z / surv};
+                            d += 1;
+                        }
+                    }
+                }
+            }
+            ++depth;
+            if (is_zero(T)) alive = false;
+        }
+        if (valid) per_particle[t] = stored;
+    }
+}
+
+// ---- stable LSD radix sort of (u32 key, u32 value), 4-bit digits -------------
+constexpr int kRsBlock = 256, kRsTile = 4096;
+
+__global__ __launch_bounds__(kRsBlock) void k_rs_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift,
+                                                      uint32_t* __restrict__ hist, uint32_t nblocks) {
+    __shared__ uint32_t cnt[16];
+    if (threadIdx.x < 16) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t b0 = blockIdx.x * kRsTile, b1 = min(n, b0 + kRsTile);
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += kRsBlock) atomicAdd(&cnt[(keys[i] >> shift) & 15u], 1u);
+    __syncthreads();
+    if (threadIdx.x < 16) hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// exclusive scan of m counters in one block of 1024 threads
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ a, uint32_t m) {
+    __shared__ uint32_t part[1024];
+    const uint32_t seg = (m + 1023) / 1024, lo = threadIdx.x * seg, hi = min(m, lo + seg);
+    uint32_t s = 0;
+    for (uint32_t i = lo; i < hi; ++i) s += a[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 1024; off <<= 1) {
+        const uint32_t v = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += v;
+        __syncthreads();
+    }
+    uint32_t run = part[threadIdx.x] - s;
+    for (uint32_t i = lo; i < hi; ++i) {
+        const uint32_t v = a[i];
+        a[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kRsBlock) void k_rs_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                         uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n,
+                                                         int shift, const uint32_t* __restrict__ hist, uint32_t nblocks) {
+    __shared__ uint32_t base[16];
+    __shared__ uint32_t wcnt[kRsBlock / 64][16];
+    if (threadIdx.x < 16) base[threadIdx.x] = hist[threadIdx.x * nblocks + blockIdx.x];
+    __syncthreads();
+    const uint32_t b0 = blockIdx.x * kRsTile, b1 = min(n, b0 + kRsTile);
+    const uint32_t w = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (uint32_t off = b0; off < b1; off += kRsBlock) {
+        const uint32_t i = off + threadIdx.x;
+        const bool valid = i < b1;
+        const uint32_t key = valid ? kin[i] : 0u;
+        const uint32_t dg = (key >> shift) & 15u;
+        uint32_t rank = 0;
+#pragma unroll
+        for (uint32_t dd = 0; dd < 16; ++dd) {
+            const uint64_t m = __ballot(valid && dg == dd);
+            if (valid && dg == dd) rank = (uint32_t)__popcll(m & lt);
+            if (lane == 0) wcnt[w][dd] = (uint32_t)__popcll(m);
+        }
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = base[dg] + rank;
+            for (uint32_t ww = 0; ww < w; ++ww) pos += wcnt[ww][dg];
+            kout[pos] = key;
+            vout[pos] = vin[i];
+        }
+        __syncthreads();
+        if (threadIdx.x < 16) {
+            uint32_t tot = 0;
+            for (uint32_t ww = 0; ww < kRsBlock / 64; ++ww) tot += wcnt[ww][threadIdx.x];
+            base[threadIdx.x] += tot;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- photon map: photons in kd-tree order -------------------------------------
+struct PhotonMapDev {
+    const float4* pos;   // xyz + depth (as float), tree order
+    const float* pay;    // 9 floats: direction, normal, power
+    const uint4* nodes;  // interior {split bits, axis, left, right}; leaf {0, 3, first, count}
+    uint32_t n;
+    float scale, radius;
+    int32_t k;
+};
+
+// tree position i <- raw slot perm[order[i]] (perm: particle order)
+__global__ void k_photon_pack(const PhotonBuf B, const uint32_t* __restrict__ perm, const uint32_t* __restrict__ order,
+                              uint32_t n, float4* __restrict__ pos, float* __restrict__ pay) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t s = perm[order[i]];
+        pos[i] = make_float4(B.px[s], B.py[s], B.pz[s], (float)(B.key[s] & 0xffu));
+        float* o = pay + 9 * (size_t)i;
+        o[0] = B.dx[s]; o[1] = B.dy[s]; o[2] = B.dz[s];
+        o[3] = B.nx[s]; o[4] = B.ny[s]; o[5] = B.nz[s];
+        o[6] = B.wr[s]; o[7] = B.wg[s]; o[8] = B.wb[s];
+    }
+}
+
+// kept photons in particle order -> packed positions for the host tree build
+__global__ void k_photon_positions(const PhotonBuf B, const uint32_t* __restrict__ perm, uint32_t n,
+                                   float* __restrict__ xyz) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t s = perm[i];
+        xyz[3 * (size_t)i] = B.px[s];
+        xyz[3 * (size_t)i + 1] = B.py[s];
+        xyz[3 * (size_t)i + 2] = B.pz[s];
+    }
+}
+
+__global__ void k_iota(uint32_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+constexpr int kGatherBlock = 64;
+constexpr int kMaxK = 128;
+constexpr int kPkStack = 32;  // > photon kd depth (checked at build)
+
+// LDS of one gather lane: heap of (d2, index) and the far-child stack, each
+// strided by the block size
+struct GatherLds {
+    float hd[kMaxK * kGatherBlock];
+    uint32_t hi[kMaxK * kGatherBlock];
+    uint32_t st[kPkStack * kGatherBlock];
+};
+
+// heap entries are (d2, index) pairs compared like std::pair (priority_queue)
+LRT_D bool pair_less(float da, uint32_t ia, float db, uint32_t ib) { return da < db || (!(db < da) && ia < ib); }
+
+// remove the heap root: last element to the root, sift down
+LRT_D void heap_pop(float* hd, uint32_t* hi, int& size) {
+    constexpr uint32_t st = kGatherBlock;
+    --size;
+    const float ld = hd[size * st];
+    const uint32_t li = hi[size * st];
+    int j = 0;
+    for (;;) {
+        int c = 2 * j + 1;
+        if (c >= size) break;
+        if (c + 1 < size && pair_less(hd[c * st], hi[c * st], hd[(c + 1) * st], hi[(c + 1) * st])) ++c;
+        if (!pair_less(ld, li, hd[c * st], hi[c * st])) break;
+        hd[j * st] = hd[c * st];
+        hi[j * st] = hi[c * st];
+        j = c;
+    }
+    hd[j * st] = ld;
+    hi[j * st] = li;
+}
+
+// k nearest photons within r of p (photon_map.h:117-165) and the irradiance
+// estimate over them (photon_map.h:38-70).  The recursion's far-child test
+// uses the limit taken when the node was entered, so it is decided on entry
+// and the far child pushed before descending into the near one.
+LRT_D f3 photon_estimate(const PhotonMapDev& Mp, f3 p, f3 n, int max_depth, GatherLds& G) {
+    if (Mp.n == 0 || Mp.radius <= 0.0f || Mp.k <= 0) return splat(0.0f);
+    constexpr uint32_t st = kGatherBlock;
+    float* hd = G.hd + threadIdx.x;
+    uint32_t* hi = G.hi + threadIdx.x;
+    uint32_t* stack = G.st + threadIdx.x;
+    const float r2 = Mp.radius * Mp.radius;
+    const int K = Mp.k;
+    int size = 0, sp = 0;
+    uint32_t node = 0;
+    for (;;) {
+        const uint4 nd = Mp.nodes[node];
+        float limit = r2;
+        if (size == K) limit = fminf(limit, hd[0]);
+        if (nd.y == 3u) {
+            for (uint32_t i = nd.z; i < nd.z + nd.w; ++i) {
+                const float4 q = Mp.pos[i];
+                const f3 dv = f3{q.x, q.y, q.z} - p;
+                const float d = len2(dv);
+                if (d > limit) continue;
+                if (size == K) heap_pop(hd, hi, size);
+                int j = size++;  // push, sift up
+                while (j > 0) {
+                    const int par = (j - 1) / 2;
+                    if (!pair_less(hd[par * st], hi[par * st], d, i)) break;
+                    hd[j * st] = hd[par * st];
+                    hi[j * st] = hi[par * st];
+                    j = par;
+                }
+                hd[j * st] = d;
+                hi[j * st] = i;
+                if (size == K) limit = fminf(r2, hd[0]);
+            }
+            if (sp == 0) break;
+            node = stack[--sp * st];
+            continue;
+        }
+        const float pv = nd.y == 0u ? p.x : (nd.y == 1u ? p.y : p.z);
+        const float delta = pv - __uint_as_float(nd.x);
+        const bool left_near = delta <= 0.0f;
+        if (delta * delta <= limit) stack[sp++ * st] = left_near ? nd.w : nd.z;
+        node = left_near ? nd.z : nd.w;
+    }
+    const float inv_r2 = 1.0f / r2;
+    f3 sum = splat(0.0f);
+    while (size > 0) {  // pop order: farthest first, like estimate_irradiance
+        const float d = hd[0];
+        const uint32_t idx = hi[0];
+        heap_pop(hd, hi, size);
+        const float4 q = Mp.pos[idx];
+        if ((int)q.w > max_depth) continue;
+        const float* pay = Mp.pay + 9 * (size_t)idx;
+        const f3 in = -ld3(pay);
+        const f3 pn = ld3(pay + 3);
+        const float ip = dot(pn, in), is = dot(n, in);
+        if (is <= 0.0f || dot(pn, n) <= 1e-1f || ip <= 1e-2f) continue;
+        const float sq = 1.0f - d * inv_r2;
+        const float w = (is / ip) * (sq * sq);
+        sum = sum + ld3(pay + 6) * fabsf(w);
+    }
+    return sum * (Mp.scale * 3.0f * (1.0f / kPiF) * inv_r2);
+}
+
+__global__ __launch_bounds__(kGatherBlock) void k_photon_estimate(const PhotonMapDev Mp, const float* __restrict__ q6,
+                                                                  uint32_t n, int max_depth, float* __restrict__ out) {
+    __shared__ GatherLds G;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const f3 e = photon_estimate(Mp, ld3(q6 + 6 * (size_t)i), ld3(q6 + 6 * (size_t)i + 3), max_depth, G);
+        out[3 * (size_t)i] = e.x;
+        out[3 * (size_t)i + 1] = e.y;
+        out[3 * (size_t)i + 2] = e.z;
+    }
+}
+
+}  // namespace lrt

@@ -53,6 +53,20 @@ struct CameraBasis {
 };
 CameraBasis camera_basis(const lrt_camera_desc& c);
 
+// Median kd tree of photon positions (photon_map.h:83-114): leaves of at most
+// 8 photons, split axis = largest extent of the range, std::nth_element at
+// count/2, and the split value read from the median slot after both subtrees
+// are built, as the reference does (the children's partitions may have moved
+// another photon there; the search prunes with that value).  nodes: 4 words
+// each; interior {split bits, axis, left, right}, leaf {0, 3, first, count}.
+// order[i] = input index of the photon at tree position i.
+struct PhotonKd {
+    std::vector<uint32_t> nodes;
+    std::vector<uint32_t> order;
+    int depth = 0;  // deepest root-to-leaf path (nodes)
+};
+void photon_kd_build(const float* xyz, uint32_t n, PhotonKd& out, int threads = 0);
+
 // Philox-4x32-10 on the host
 void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

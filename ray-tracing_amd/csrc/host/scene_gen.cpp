@@ -252,11 +252,8 @@ Generated* generate(const lrt_gen_params& p, const float* tables, uint32_t n_tab
         return o;
     };
     lrt_camera_desc& cam = g->desc.camera;
-    if (p.kind == LRT_GEN_BOX_ICOSPHERE) {
-        Rng rng(p.seed, 1);
-        g->meshes.resize(2);
-        Mesh& room = g->meshes[0];
-        // each face of [-1,1]^3 as two quads, normals facing inward
+    // each face of [-1,1]^3 as two quads, normals facing inward (12 quads)
+    auto build_room = [](Mesh& room) {
         for (int axis = 0; axis < 3; ++axis)
             for (int side = -1; side <= 1; side += 2) {
                 const int a1 = (axis + 1) % 3, a2 = (axis + 2) % 3;
@@ -279,20 +276,15 @@ Generated* generate(const lrt_gen_params& p, const float* tables, uint32_t n_tab
                     else add_quad(room, q[0], q[3], q[2], q[1], n);
                 }
             }
-        icosphere(g->meshes[1], (int)p.subdivisions, 0.3f);
-        for (int i = 0; i < 2; ++i) {
-            lrt_material m = blank();
-            m.type = LRT_BSDF_DIFFUSE;
-            for (int c = 0; c < 3; ++c) m.diffuse[c] = 0.2f + 0.6f * rng.next();
-            g->materials.push_back(m);
-            g->objects.push_back(obj_for_mesh(g->meshes[(size_t)i], i));
-        }
+    };
+    // ceiling rectangle light of edge `size` just below y = 1, facing down
+    auto ceiling_light = [&](float size, float radiance, int object) {
         lrt_object_desc L;
         std::memset(&L, 0, sizeof(L));
         L.shape = LRT_SHAPE_RECTANGLE;
         L.material = -1;
-        L.emitter = 0;
-        const float corner[3] = {-0.125f, 0.999f, -0.125f}, eu[3] = {0.25f, 0.0f, 0.0f}, ev[3] = {0.0f, 0.0f, 0.25f};
+        L.emitter = (int)g->lights.size();
+        const float corner[3] = {-0.5f * size, 0.999f, -0.5f * size}, eu[3] = {size, 0.0f, 0.0f}, ev[3] = {0.0f, 0.0f, size};
         std::memcpy(L.corner, corner, 12);
         std::memcpy(L.edge_u, eu, 12);
         std::memcpy(L.edge_v, ev, 12);
@@ -300,49 +292,43 @@ Generated* generate(const lrt_gen_params& p, const float* tables, uint32_t n_tab
         lrt_light_desc ld;
         std::memset(&ld, 0, sizeof(ld));
         ld.kind = LRT_LIGHT_AREA;
-        ld.object = 2;
-        ld.radiance[0] = ld.radiance[1] = ld.radiance[2] = 17.0f;
+        ld.object = object;
+        ld.radiance[0] = ld.radiance[1] = ld.radiance[2] = radiance;
         g->lights.push_back(ld);
-        const float o[3] = {0.0f, 0.0f, 0.95f}, la[3] = {0.0f, -0.05f, 0.0f}, up[3] = {0.0f, 1.0f, 0.0f};
-        std::memcpy(cam.origin, o, 12);
-        std::memcpy(cam.look_at, la, 12);
-        std::memcpy(cam.up, up, 12);
-        cam.fov_deg = 40.0f;
-        cam.aspect = p.aspect > 0 ? p.aspect : 1.0f;
-    } else if (p.kind == LRT_GEN_DISPLACED_SPHERE) {
-        if (!sky) throw std::runtime_error("displaced sphere scene needs a sky light");
-        if (g->tables.size() < 3 * 101) throw std::runtime_error("displaced sphere scene needs 3 rough-transmittance slices");
-        Rng rng(p.seed, 2);
-        float sites[7][3];
+    };
+    // 7 seeded Voronoi sites on the sphere and the region materials:
+    // 0..6 = diffuse, rough dielectric, rough conductor, plastic, rough plastic,
+    // coating, rough coating (README.md:34 order); 7 = the rough glass nested
+    // in the rough coating; 8 = grey diffuse.  Tables 0/1/2 = rough plastic
+    // external/internal and rough coating slices.
+    auto region_setup = [&](Rng& rng, float sites[7][3]) {
+        if (g->tables.size() < 3 * 101) throw std::runtime_error("this scene needs 3 rough-transmittance slices");
         for (int s = 0; s < 7; ++s) {
             const float z = 1.0f - 2.0f * rng.next(), ph = 6.28318530718f * rng.next();
             const float r = std::sqrt(std::max(0.0f, 1.0f - z * z));
             sites[s][0] = r * std::cos(ph); sites[s][1] = z; sites[s][2] = r * std::sin(ph);
         }
-        g->meshes.resize(2);
-        displaced_sphere(g->meshes[0], p.segments ? p.segments : 1024, p.rings ? p.rings : 512, p.seed, sites, 7);
-        // materials 0..6 = regions (README.md:34 order), 7 = nested rough glass, 8 = ground
         const float cu_eta[3] = {0.200438f, 0.924033f, 1.102212f}, cu_k[3] = {3.912948f, 2.452847f, 2.142188f};
-        lrt_material m0 = blank();  // diffuse
+        lrt_material m0 = blank();
         m0.type = LRT_BSDF_DIFFUSE;
         for (int c = 0; c < 3; ++c) m0.diffuse[c] = 0.2f + 0.6f * rng.next();
-        lrt_material m1 = blank();  // rough dielectric
+        lrt_material m1 = blank();
         m1.type = LRT_BSDF_ROUGH_DIELECTRIC; m1.int_ior = 1.5f; m1.ext_ior = 1.0f; m1.alpha = 0.2f;
-        lrt_material m2 = blank();  // rough conductor (Cu)
+        lrt_material m2 = blank();
         m2.type = LRT_BSDF_CONDUCTOR; m2.alpha = 0.1f;
         std::memcpy(m2.cond_eta, cu_eta, 12); std::memcpy(m2.cond_k, cu_k, 12);
-        lrt_material m3 = blank();  // smooth plastic (src/main.cpp:42)
+        lrt_material m3 = blank();  // src/main.cpp:42
         m3.type = LRT_BSDF_PLASTIC; m3.int_ior = 1.49f; m3.ext_ior = 1.000277f;
         m3.diffuse[0] = 0.8f; m3.diffuse[1] = 0.8f; m3.diffuse[2] = 0.5f;
         m3.fdr_int = fresnel_diffuse_reflectance(1.0f / (m3.int_ior / m3.ext_ior));
-        lrt_material m4 = blank();  // rough plastic (src/main.cpp:43 colours)
+        lrt_material m4 = blank();  // src/main.cpp:43 colours
         m4.type = LRT_BSDF_ROUGH_PLASTIC; m4.int_ior = 1.49f; m4.ext_ior = 1.000277f; m4.alpha = 0.3f;
         m4.diffuse[0] = 0.8f; m4.diffuse[1] = 0.1f; m4.diffuse[2] = 0.05f;
         m4.table_ext = 0; m4.table_int = 1;
-        lrt_material m5 = blank();  // smooth coating over the plastic (src/main.cpp:44-45)
+        lrt_material m5 = blank();  // src/main.cpp:44-45
         m5.type = LRT_BSDF_COATING; m5.int_ior = 1.7f; m5.ext_ior = 1.000277f; m5.nested = 3;
         m5.specular_reflectance[0] = 0.3f; m5.specular_reflectance[1] = 0.6f; m5.specular_reflectance[2] = 0.6f;
-        lrt_material m6 = blank();  // rough coating over rough glass alpha 0.05 (src/main.cpp:41,46-47)
+        lrt_material m6 = blank();  // src/main.cpp:41,46-47
         m6.type = LRT_BSDF_ROUGH_COATING; m6.int_ior = 1.7f; m6.ext_ior = 1.000277f; m6.alpha = 0.25f; m6.nested = 7;
         m6.table_ext = 2;
         lrt_material m7 = blank();
@@ -351,6 +337,33 @@ Generated* generate(const lrt_gen_params& p, const float* tables, uint32_t n_tab
         m8.type = LRT_BSDF_DIFFUSE;
         m8.diffuse[0] = m8.diffuse[1] = m8.diffuse[2] = 0.5f;
         for (const auto& m : {m0, m1, m2, m3, m4, m5, m6, m7, m8}) g->materials.push_back(m);
+    };
+    if (p.kind == LRT_GEN_BOX_ICOSPHERE) {
+        Rng rng(p.seed, 1);
+        g->meshes.resize(2);
+        build_room(g->meshes[0]);
+        icosphere(g->meshes[1], (int)p.subdivisions, 0.3f);
+        for (int i = 0; i < 2; ++i) {
+            lrt_material m = blank();
+            m.type = LRT_BSDF_DIFFUSE;
+            for (int c = 0; c < 3; ++c) m.diffuse[c] = 0.2f + 0.6f * rng.next();
+            g->materials.push_back(m);
+            g->objects.push_back(obj_for_mesh(g->meshes[(size_t)i], i));
+        }
+        ceiling_light(0.25f, 17.0f, 2);
+        const float o[3] = {0.0f, 0.0f, 0.95f}, la[3] = {0.0f, -0.05f, 0.0f}, up[3] = {0.0f, 1.0f, 0.0f};
+        std::memcpy(cam.origin, o, 12);
+        std::memcpy(cam.look_at, la, 12);
+        std::memcpy(cam.up, up, 12);
+        cam.fov_deg = 40.0f;
+        cam.aspect = p.aspect > 0 ? p.aspect : 1.0f;
+    } else if (p.kind == LRT_GEN_DISPLACED_SPHERE) {
+        if (!sky) throw std::runtime_error("displaced sphere scene needs a sky light");
+        Rng rng(p.seed, 2);
+        float sites[7][3];
+        region_setup(rng, sites);
+        g->meshes.resize(2);
+        displaced_sphere(g->meshes[0], p.segments ? p.segments : 1024, p.rings ? p.rings : 512, p.seed, sites, 7);
         // ground quad (2 triangles) below the sphere
         Mesh& ground = g->meshes[1];
         const float y = -1.06f, e = 20.0f, n[3] = {0.0f, 1.0f, 0.0f};
@@ -365,6 +378,33 @@ Generated* generate(const lrt_gen_params& p, const float* tables, uint32_t n_tab
         std::memcpy(cam.up, up, 12);
         cam.fov_deg = 40.0f;
         cam.aspect = p.aspect > 0 ? p.aspect : 16.0f / 9.0f;
+    } else if (p.kind == LRT_GEN_PHOTON_BOX) {
+        // the box room around the displaced sphere (radius 0.5, 7 regions),
+        // lit by a 0.5 x 0.5 ceiling light
+        Rng rng(p.seed, 3);
+        float sites[7][3];
+        region_setup(rng, sites);
+        g->meshes.resize(2);
+        build_room(g->meshes[0]);
+        displaced_sphere(g->meshes[1], p.segments ? p.segments : 512, p.rings ? p.rings : 256, p.seed, sites, 7);
+        for (size_t i = 0; i < g->meshes[1].pos.size(); i += 3) {
+            g->meshes[1].pos[i] *= 0.5f;
+            g->meshes[1].pos[i + 1] = g->meshes[1].pos[i + 1] * 0.5f - 0.45f;
+            g->meshes[1].pos[i + 2] *= 0.5f;
+        }
+        lrt_material wall = blank();
+        wall.type = LRT_BSDF_DIFFUSE;
+        for (int c = 0; c < 3; ++c) wall.diffuse[c] = 0.2f + 0.6f * rng.next();
+        g->materials.push_back(wall);  // material 9
+        g->objects.push_back(obj_for_mesh(g->meshes[0], 9));
+        g->objects.push_back(obj_for_mesh(g->meshes[1], 0));
+        ceiling_light(0.5f, 10.0f, 2);
+        const float o[3] = {0.0f, 0.0f, 0.95f}, la[3] = {0.0f, -0.3f, 0.0f}, up[3] = {0.0f, 1.0f, 0.0f};
+        std::memcpy(cam.origin, o, 12);
+        std::memcpy(cam.look_at, la, 12);
+        std::memcpy(cam.up, up, 12);
+        cam.fov_deg = 60.0f;
+        cam.aspect = p.aspect > 0 ? p.aspect : 1.0f;
     } else {
         throw std::runtime_error("unknown generator kind");
     }

@@ -139,6 +139,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(PathState P, const CameraDev 
         counters[0] = n;  // active queue length
         counters[1] = 0;  // next queue
         counters[2] = 0;  // shadow queue
+        counters[3] = 0;  // photon queries
     }
     if (i >= n) return;
     const uint32_t x = i % W, y = row0 + i / W;
@@ -300,7 +301,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(const DevScene S, const DevLig
                         u6[k] = u24((d >> 2) == (d0 >> 2) ? pick(A, d & 3u) : pick(B, d & 3u));
                     }
                     ctx.lobe = u6[2];
-                    if (bsdf_flags(M, sh.material) & (F_DIFFUSE | F_GLOSSY)) {  // NEE (path_integrator.h:86-128)
+                    const uint32_t fl = bsdf_flags(M, sh.material);
+                    if (fl & (F_DIFFUSE | F_GLOSSY)) {  // NEE (path_integrator.h:86-128)
                         DirectSample ds;
                         if (sample_emitter_direction(S, Ls, sh.p, u6[0], u6[1], ds)) {
                             so = sh.p + sh.n * 1e-4f;
@@ -319,7 +321,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(const DevScene S, const DevLig
                                 if (!is_zero(e.f)) {
                                     const float lp = pdf_emitter_direction(S, Ls, sh.p, sd);
                                     if (lp > 0.0f) {
-                                        contrib = T * e.f * ds.radiance * (power_heuristic(lp, e.pdf) / lp);
+                                        const float w = power_heuristic(lp, e.pdf);
+                                        contrib = T * e.f * ds.radiance * (w / lp);
                                         ignore = ds.emitter;
                                         shadow = true;
                                     }
@@ -367,6 +370,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(const DevScene S, const DevLig
     }
 }
 
+
 // shadow rays: add the pending NEE contribution when unoccluded
 __global__ __launch_bounds__(kBlock) void k_shadow(const DevScene S, PathState P, const uint32_t* __restrict__ counters,
                                                    const ShadowBuf Sb) {
@@ -391,6 +395,7 @@ __global__ void k_advance(uint32_t* counters, unsigned long long* stats) {
         counters[0] = counters[1];
         counters[1] = 0;
         counters[2] = 0;
+        counters[3] = 0;
     }
 }
 
@@ -408,3 +413,6 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(const PathState P, uint32
 }
 
 }  // namespace lrt
+
+// photon tracing, sorting and the kd photon map (uses surface() and wave_append)
+#include "device/photon.h"

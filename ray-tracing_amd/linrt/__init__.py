@@ -40,7 +40,8 @@ MODE_RADIANCE, MODE_IMPORTANCE = 0, 1
  BSDF_ROUGH_PLASTIC, BSDF_COATING, BSDF_ROUGH_COATING) = range(9)
 SHAPE_MESH, SHAPE_RECTANGLE, SHAPE_SPHERE = 0, 1, 2
 LIGHT_AREA, LIGHT_SKY = 0, 1
-GEN_BOX_ICOSPHERE, GEN_DISPLACED_SPHERE = 0, 1
+GEN_BOX_ICOSPHERE, GEN_DISPLACED_SPHERE, GEN_PHOTON_BOX = 0, 1, 2
+PHOTON_SURFACE, PHOTON_CAUSTIC = 0, 1
 KERNEL_CLOSEST, KERNEL_OCCLUDED, KERNEL_SHADE = 0, 1, 2
 
 F3 = C.c_float * 3
@@ -81,7 +82,8 @@ class SceneDesc(C.Structure):
 class SceneInfo(C.Structure):
     _fields_ = [("n_triangles", C.c_uint64), ("n_nodes", C.c_uint64), ("n_refs", C.c_uint64),
                 ("device_bytes", C.c_uint64), ("max_tree_depth", C.c_int32),
-                ("build_seconds", C.c_double), ("upload_seconds", C.c_double)]
+                ("build_seconds", C.c_double), ("upload_seconds", C.c_double),
+                ("bounding_radius", C.c_float), ("has_delta_bsdf", C.c_int32)]
 
 
 class RenderParams(C.Structure):
@@ -98,6 +100,17 @@ class RenderStats(C.Structure):
                 ("occluded_ms", C.c_double), ("shade_ms", C.c_double),
                 ("trace_launches", C.c_uint64), ("occluded_launches", C.c_uint64),
                 ("nodes_visited", C.c_uint64), ("refs_read", C.c_uint64), ("tris_tested", C.c_uint64)]
+
+
+class PhotonParams(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("requested", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
+                ("seed", C.c_uint64), ("radius", C.c_float), ("k", C.c_uint32)]
+
+
+class PhotonStats(C.Structure):
+    _fields_ = [("photons", C.c_uint64), ("shot", C.c_uint64), ("particles_traced", C.c_uint64),
+                ("returned", C.c_uint64), ("full", C.c_int32), ("trace_ms", C.c_double), ("sort_ms", C.c_double),
+                ("seconds", C.c_double)]
 
 
 class GenParams(C.Structure):
@@ -127,6 +140,7 @@ EXPORTED = [
     "lrt_scene_release", "lrt_scene_info_get", "lrt_trace_closest", "lrt_trace_occluded", "lrt_trace_count",
     "lrt_scene_sync", "lrt_scene_kernel_timing", "lrt_bsdf_eval_pdf", "lrt_bsdf_sample", "lrt_bsdf_flags",
     "lrt_fresnel_diffuse_reflectance", "lrt_rough_transmittance_slice", "lrt_sky_eval", "lrt_render",
+    "lrt_photon_build", "lrt_photon_export", "lrt_photon_map_load", "lrt_photon_estimate", "lrt_render_photon",
     "lrt_generate_scene", "lrt_generated_desc", "lrt_generated_free", "lrt_philox4x32_10",
 ]
 
@@ -253,6 +267,13 @@ def displaced_sphere(seed=1, segments=1024, rings=512, aspect=16.0 / 9.0):
                      tables=tables, sky=load_sky())
 
 
+def photon_box(seed=1, segments=512, rings=256, aspect=1.0):
+    """Box room around a 262,144-triangle displaced sphere (7 material
+    regions, radius 0.5) under a 0.5 x 0.5 ceiling light (photon workload)."""
+    tables, _ = load_tables()
+    return Generated(GEN_PHOTON_BOX, seed=seed, segments=segments, rings=rings, aspect=aspect, tables=tables)
+
+
 class Scene:
     """Scene uploaded to one GPU (lrt_scene_create)."""
 
@@ -336,6 +357,82 @@ class Scene:
         _check(LIB.lrt_render(self._h, C.byref(prm), C.c_void_p(sum_ptr), C.c_void_p(sumsq_ptr) if sumsq_ptr else None,
                               1, C.byref(st)))
         return st
+
+    # ---- photon mapping (photon_integrator.h:15-123) ----
+    def photon_build(self, requested, max_depth, rr_depth, seed=1, radius=0.05, k=120, kind=PHOTON_SURFACE):
+        """Traces particles until `requested` photons are stored and builds map
+        `kind` (photon_gather.h:15-101, photon_map.h:26-114)."""
+        prm = PhotonParams(kind, requested, max_depth, rr_depth, seed, radius, k)
+        st = PhotonStats()
+        _check(LIB.lrt_photon_build(self._h, C.byref(prm), C.byref(st)))
+        return st
+
+    def photon_export(self, kind=PHOTON_SURFACE):
+        """Map `kind`'s photons in particle order: (photons [n, 12] float32 =
+        position, direction, normal, power; depth_particle [n, 2] uint32; shot)."""
+        n, shot = C.c_uint64(), C.c_uint64()
+        _check(LIB.lrt_photon_export(self._h, C.c_int32(kind), None, None, C.c_uint64(0), C.byref(n), C.byref(shot)))
+        ph = np.zeros((n.value, 12), np.float32)
+        dp = np.zeros((n.value, 2), np.uint32)
+        _check(LIB.lrt_photon_export(self._h, C.c_int32(kind), _p(ph), _p(dp, C.c_uint32), C.c_uint64(n.value),
+                                     C.byref(n), C.byref(shot)))
+        return ph, dp, shot.value
+
+    def photon_map_load(self, photons, depth_particle, scale, radius, k=120, kind=PHOTON_SURFACE):
+        ph = np.ascontiguousarray(photons, np.float32).reshape(-1, 12)
+        dp = np.ascontiguousarray(depth_particle, np.uint32).reshape(-1, 2)
+        _check(LIB.lrt_photon_map_load(self._h, C.c_int32(kind), _p(ph), _p(dp, C.c_uint32), C.c_uint64(ph.shape[0]),
+                                       C.c_float(scale), C.c_float(radius), C.c_uint32(k)))
+
+    def photon_estimate(self, queries, max_depth, kind=PHOTON_SURFACE):
+        """photon_map::estimate_irradiance of map `kind` at (point, normal) rows [n, 6]."""
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 6)
+        out = np.zeros((q.shape[0], 3), np.float32)
+        _check(LIB.lrt_photon_estimate(self._h, C.c_int32(kind), _p(q), C.c_uint32(q.shape[0]), C.c_int(max_depth),
+                                       _p(out)))
+        return out
+
+    def render_photon(self, width, height, spp, max_depth=128, seed=1, spp_offset=0, rows=None, sumsq=True):
+        """photon_integrator::radiance per (pixel, sample) with the scene's maps
+        (photon_integrator.h:39-123); returns (sum, sumsq, stats) like render."""
+        sum_rgb = np.zeros((height, width, 3), np.float64)
+        sumsq_rgb = np.zeros((height, width, 3), np.float64) if sumsq else None
+        rb, re = rows if rows is not None else (0, 0)
+        prm = RenderParams(width, height, spp, spp_offset, max_depth, 0, seed, rb, re, 0)
+        st = RenderStats()
+        _check(LIB.lrt_render_photon(self._h, C.byref(prm), _p(sum_rgb, C.c_double),
+                                     _p(sumsq_rgb, C.c_double) if sumsq else None, 0, C.byref(st)))
+        return sum_rgb, sumsq_rgb, st
+
+
+class PhotonIntegrator:
+    """photon_integrator (photon_integrator.h:13-123) on the GPU: preprocess()
+    builds the global and caustic maps with the reference's constants, and
+    render() runs the eye pass."""
+    MAX_DEPTH, RR_DEPTH, LOOKUP = 128, 5, 120
+    GLOBAL_PHOTONS = CAUSTIC_PHOTONS = 250000
+
+    def __init__(self, global_photons=GLOBAL_PHOTONS, caustic_photons=CAUSTIC_PHOTONS, lookup=LOOKUP,
+                 max_depth=MAX_DEPTH, seed=1):
+        self.global_photons, self.caustic_photons = global_photons, caustic_photons
+        self.lookup, self.max_depth, self.seed = lookup, max_depth, seed
+
+    def preprocess(self, scene):
+        info = scene.info
+        r = info.bounding_radius
+        caustic = self.caustic_photons if info.has_delta_bsdf else 0
+        self.stats = [scene.photon_build(self.global_photons, self.max_depth - 1, self.RR_DEPTH, self.seed,
+                                         0.05 * r, self.lookup, PHOTON_SURFACE)]
+        if caustic:
+            self.stats.append(scene.photon_build(caustic, self.max_depth - 1, self.RR_DEPTH, self.seed,
+                                                 0.0125 * r, self.lookup, PHOTON_CAUSTIC))
+        else:
+            scene.photon_map_load(np.zeros((0, 12), np.float32), np.zeros((0, 2), np.uint32), 1.0,
+                                  0.0125 * r, self.lookup, PHOTON_CAUSTIC)
+        return self.stats
+
+    def render(self, scene, width, height, spp, spp_offset=0):
+        return scene.render_photon(width, height, spp, self.max_depth, self.seed, spp_offset)
 
 
 def _mats_arg(materials):

@@ -116,3 +116,54 @@ def bsdf_tuples(n, seed, n_mats):
     u = rng.random((n, 2))
     lobe = rng.random(n)
     return (mat, wi.astype(np.float32), wo.astype(np.float32), front, u.astype(np.float32), lobe.astype(np.float32))
+
+
+class SphereRoom:
+    """The box room of the 256x256 workload with four analytic spheres (smooth
+    dielectric, rough conductor, smooth plastic, smooth coating over diffuse)
+    under its ceiling light: every photon-integrator branch (specular lobes,
+    glossy recursion, pure-diffuse gathers) with one material per shape, as the
+    reference's scene graph requires."""
+
+    def __init__(self, seed=1):
+        self.gen = linrt.box_icosphere(seed=seed, subdivisions=1)
+        d = self.gen.desc
+        m = linrt.material
+        cu_eta, cu_k = (0.200438, 0.924033, 1.102212), (3.912948, 2.452847, 2.142188)
+        mats = [m(linrt.BSDF_DIFFUSE, diffuse=(0.62, 0.55, 0.45)),
+                m(linrt.BSDF_DIELECTRIC, int_ior=1.5, ext_ior=1.0),
+                m(linrt.BSDF_CONDUCTOR, alpha=0.15, cond_eta=cu_eta, cond_k=cu_k),
+                m(linrt.BSDF_PLASTIC, int_ior=1.49, ext_ior=1.000277, diffuse=(0.2, 0.5, 0.8)),
+                m(linrt.BSDF_COATING, int_ior=1.7, ext_ior=1.000277, nested=5, specular_reflectance=(0.9, 0.9, 0.9)),
+                m(linrt.BSDF_DIFFUSE, diffuse=(0.7, 0.2, 0.2))]
+        mats[3].fdr_int = linrt.fresnel_diffuse_reflectance(np.float32(1.0) / (np.float32(1.49) / np.float32(1.000277)))
+        self.materials = (linrt.Material * len(mats))(*mats)
+        room = linrt.ObjectDesc()
+        C_ = __import__("ctypes")
+        C_.pointer(room)[0] = d.objects[0]
+        room.material = 0
+        spheres = [((-0.45, -0.6, -0.3), 0.3, 1), ((0.45, -0.65, -0.35), 0.3, 2), ((-0.3, -0.75, 0.35), 0.22, 3),
+                   ((0.35, -0.72, 0.3), 0.25, 4)]
+        objs = [room]
+        for c, r, mi in spheres:
+            o = linrt.ObjectDesc()
+            o.shape, o.material, o.emitter, o.radius = linrt.SHAPE_SPHERE, mi, -1, r
+            for i in range(3):
+                o.center[i] = c[i]
+            objs.append(o)
+        light_obj = None
+        for i in range(d.n_objects):
+            if d.objects[i].shape == linrt.SHAPE_RECTANGLE:
+                light_obj = d.objects[i]
+        lo = linrt.ObjectDesc()
+        C_.pointer(lo)[0] = light_obj
+        lo.emitter = 0
+        objs.append(lo)
+        self.objects = (linrt.ObjectDesc * len(objs))(*objs)
+        ld = linrt.LightDesc()
+        C_.pointer(ld)[0] = d.lights[0]
+        ld.object = len(objs) - 1
+        self.lights = (linrt.LightDesc * 1)(ld)
+        self.desc = linrt.SceneDesc(len(objs), self.objects, len(mats), self.materials, 1, self.lights, 0, None,
+                                    d.camera)
+        self.desc_ptr = C_.pointer(self.desc)

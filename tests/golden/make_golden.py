@@ -138,15 +138,58 @@ def make_render_fixture():
     np.savez_compressed(OUT / "render_box_icosphere_16x16x4.npz", per_sample=r["per_sample"], fsum=r["fsum"])
 
 
+PHOTON_EST = ((16, 0.1, 100), (64, 0.3, 100), (64, 0.3, 3))  # (k, radius, max_depth)
+
+
+def make_photon_fixture():
+    """Photons of the reference's particle tracer (photon_tracer.h, reference
+    Sobol sampler, surface map, depth 8, RR from 5) for particles 0..599 of a
+    box-icosphere scene, and photon_map::estimate_irradiance over them."""
+    g = linrt.box_icosphere(seed=1, subdivisions=3)
+    ph, dp = OL.reference.scene(g.desc_ptr).trace_particles(0, 600, 8, 5)
+    rng = np.random.default_rng(5)
+    pick = rng.integers(0, len(ph), 256)
+    q = np.concatenate([ph[pick, :3] + rng.normal(scale=0.02, size=(256, 3)).astype(np.float32),
+                        ph[rng.permutation(pick), 6:9]], 1).astype(np.float32)
+    out = dict(photons=ph, depth_particle=dp, queries=q)
+    for k, r, md in PHOTON_EST:
+        out[f"est_k{k}_r{r}_d{md}"] = OL.photon_estimate(OL.reference, ph, dp, 1.0 / 600, q, r, md, k)
+    np.savez_compressed(OUT / "photons_box_icosphere_l3.npz", **out)
+
+
+def make_eye_fixture():
+    """photon_integrator::radiance of the reference (reference sampler, depth
+    128, k 120, radii 0.05 / 0.0125 x the bounding radius) per sample on the
+    sphere room, with global and caustic maps from the reference's particle
+    tracer (particles 0..1499 each, depth 127, RR from 5)."""
+    from common import SphereRoom
+    sr = SphereRoom()
+    s = OL.reference.scene(sr.desc_ptr)
+    R, delta = s.bounds()
+    assert delta
+    gp, gd = s.trace_particles(0, 1500, 127, 5, kind=0)
+    cp, cd = s.trace_particles(0, 1500, 127, 5, kind=1)
+    maps = [(gp, gd, 1.0 / 1500, 0.05 * R), (cp, cd, 1.0 / 1500, 0.0125 * R)]
+    ps = s.render_photon(16, 16, 2, maps)
+    np.savez_compressed(OUT / "eye_sphere_room_16x16x2.npz", g_photons=gp, g_dp=gd, c_photons=cp, c_dp=cd,
+                        radius=np.float32(R), per_sample=ps)
+
+
 def main():
     DATA.mkdir(parents=True, exist_ok=True)
     assert OL.REF is not None, "oracle/_ref/libref.so is not built (make -C oracle ref)"
+    if sys.argv[1:] == ["photons"]:
+        make_photon_fixture()
+        make_eye_fixture()
+        return
     tables = make_tables()
     make_sky()
     make_bsdf_fixture(tables)
     make_hits_fixture()
     make_misc_fixture(tables)
     make_render_fixture()
+    make_photon_fixture()
+    make_eye_fixture()
     print("data and fixtures written")
 
 

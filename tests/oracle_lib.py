@@ -27,6 +27,9 @@ def _load(path):
     for name in ("orc_scene_create", "ref_scene_create"):
         if hasattr(lib, name):
             getattr(lib, name).restype = C.c_void_p
+    for name in ("orc_trace_particles", "orc_gather_photons", "ref_trace_particles"):
+        if hasattr(lib, name):
+            getattr(lib, name).restype = C.c_int64
     return lib
 
 
@@ -175,6 +178,85 @@ class _SceneHandle:
                               C.c_int(re), _p(s, C.c_double), _p(q, C.c_double), _p(ps), _p(fs), st)
         return dict(sum=s, sumsq=q, per_sample=None if ps is None else ps.reshape(H, W, spp, 3), fsum=fs,
                     ext=st[0], shadow=st[1])
+
+
+    # ---- photon mapping ----
+    def trace_particles(self, first, count, max_depth, rr_depth, kind=0, seed=1, mode=0):
+        """Particles [first, first+count) traced one after another
+        (photon_tracer.h:17-88): (photons [n, 12], depth_particle [n, 2])."""
+        if self.impl.pre == "ref_":
+            args = (self.h, C.c_int(kind), C.c_uint32(first), C.c_uint32(count), C.c_int(max_depth), C.c_int(rr_depth))
+        else:
+            args = (self.h, C.c_int(kind), C.c_uint32(first), C.c_uint32(count), C.c_int(max_depth), C.c_int(rr_depth),
+                    C.c_uint64(seed), C.c_int(mode))
+        f = self.impl.f("trace_particles")
+        n = f(*args, None, None, C.c_int64(0))
+        ph = np.zeros((max(n, 1), 12), np.float32)
+        dp = np.zeros((max(n, 1), 2), np.uint32)
+        f(*args, _p(ph), _p(dp, C.c_uint32), C.c_int64(n))
+        return ph[:n], dp[:n]
+
+    def gather_photons(self, requested, max_depth, rr_depth, kind=0, seed=1):
+        """Oracle quota gather in particle order: (photons, depth_particle, shot)."""
+        f = self.impl.f("gather_photons")
+        shot = C.c_uint64()
+        n = f(self.h, C.c_int(kind), C.c_uint32(requested), C.c_int(max_depth), C.c_int(rr_depth), C.c_uint64(seed),
+              None, None, C.byref(shot))
+        ph = np.zeros((max(n, 1), 12), np.float32)
+        dp = np.zeros((max(n, 1), 2), np.uint32)
+        f(self.h, C.c_int(kind), C.c_uint32(requested), C.c_int(max_depth), C.c_int(rr_depth), C.c_uint64(seed),
+          _p(ph), _p(dp, C.c_uint32), C.byref(shot))
+        return ph[:n], dp[:n], shot.value
+
+    def render_photon(self, W, H, spp, maps, max_depth=128, k=120, seed=1, mode=0, threads=8, rows=None):
+        """photon_integrator::radiance per (pixel, sample) over maps built from
+        maps = [(photons, depth_particle, scale, radius) or None] * 2 (global,
+        caustic).  Returns per-sample radiance [H, W, spp, 3] (float32)."""
+        rb, re = rows if rows is not None else (0, 0)
+        ps = np.zeros(H * W * spp * 3, np.float32)
+        args = []
+        keep = []
+        for m in maps:
+            if m is None:
+                args += [None, None, C.c_int64(0), C.c_float(1.0), C.c_float(1.0)]
+            else:
+                ph = np.ascontiguousarray(m[0], np.float32).reshape(-1, 12)
+                dp = np.ascontiguousarray(m[1], np.uint32).reshape(-1, 2)
+                keep += [ph, dp]
+                args += [_p(ph), _p(dp, C.c_uint32), C.c_int64(ph.shape[0]), C.c_float(m[2]), C.c_float(m[3])]
+        if self.impl.pre == "ref_":
+            assert max_depth == 128 and k == 120, "the reference's photon integrator fixes depth 128 and k 120"
+            self.impl.f("render_photon")(self.h, C.c_int(W), C.c_int(H), C.c_int(spp), *args, C.c_int(rb), C.c_int(re),
+                                         _p(ps), C.c_int(threads))
+        else:
+            self.impl.f("render_photon")(self.h, C.c_int(W), C.c_int(H), C.c_int(spp), C.c_int(max_depth),
+                                         C.c_uint64(seed), C.c_int(mode), *args, C.c_int(k), C.c_int(threads),
+                                         C.c_int(rb), C.c_int(re), None, None, _p(ps))
+        return ps.reshape(H, W, spp, 3)
+
+    def bounds(self):
+        """(bounding sphere radius, has a delta BSDF) of the reference scene."""
+        r, d = C.c_float(), C.c_int()
+        self.impl.f("scene_bounds")(self.h, C.byref(r), C.byref(d))
+        return r.value, bool(d.value)
+
+
+def photon_estimate(impl, photons, depth_particle, scale, queries, radius, max_depth, k, threads=8):
+    """photon_map build + estimate_irradiance through the oracle or the reference."""
+    ph = np.ascontiguousarray(photons, np.float32).reshape(-1, 12)
+    dp = np.ascontiguousarray(depth_particle, np.uint32).reshape(-1, 2)
+    q = np.ascontiguousarray(queries, np.float32).reshape(-1, 6)
+    out = np.zeros((q.shape[0], 3), np.float32)
+    if impl.pre == "ref_":
+        rc = impl.f("photon_estimate")(_p(ph), _p(dp, C.c_uint32), C.c_int64(ph.shape[0]), C.c_float(scale), _p(q),
+                                       C.c_uint32(q.shape[0]), C.c_float(radius), C.c_int(max_depth), C.c_int(k),
+                                       _p(out))
+    else:
+        rc = impl.f("photon_estimate")(_p(ph), _p(dp, C.c_uint32), C.c_int64(ph.shape[0]), C.c_float(scale), _p(q),
+                                       C.c_uint32(q.shape[0]), C.c_float(radius), C.c_int(max_depth), C.c_int(k),
+                                       _p(out), C.c_int(threads))
+    assert rc == 0
+    return out
 
 
 oracle = _Impl(ORC, "orc_") if ORC is not None else None

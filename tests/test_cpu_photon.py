@@ -1,0 +1,140 @@
+"""Photon mapping in the oracle: the particle tracer and the k-nearest
+irradiance estimate against the reference's own outputs (tests/golden/
+photons_box_icosphere_l3.npz, written by make_golden.py), the estimate against
+a brute-force k-nearest search in float64, and the quota semantics of the
+gather (photon_gather.h:15-101).  CPU only."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import linrt
+import oracle_lib as OL
+
+GOLD = Path(__file__).resolve().parent / "golden"
+EST = ((16, 0.1, 100), (64, 0.3, 100), (64, 0.3, 3))
+
+
+@pytest.fixture(scope="module")
+def gold():
+    return np.load(GOLD / "photons_box_icosphere_l3.npz")
+
+
+def test_particles_oracle_is_the_reference(gold):
+    g = linrt.box_icosphere(seed=1, subdivisions=3)
+    ph, dp = OL.oracle.scene(g.desc_ptr).trace_particles(0, 600, 8, 5, mode=1)
+    assert np.array_equal(ph, gold["photons"])
+    assert np.array_equal(dp, gold["depth_particle"])
+    # surface map: no direct (depth-1) photons, depths within the path limit
+    assert dp[:, 0].min() >= 2 and dp[:, 0].max() <= 8
+    assert np.all(np.diff(dp[:, 1].astype(np.int64)) >= 0)
+
+
+@pytest.mark.parametrize("k,r,md", EST)
+def test_estimate_oracle_is_the_reference(gold, k, r, md):
+    e = OL.photon_estimate(OL.oracle, gold["photons"], gold["depth_particle"], 1.0 / 600, gold["queries"], r, md, k)
+    assert np.array_equal(e, gold[f"est_k{k}_r{r}_d{md}"])
+    assert (e > 0).any(1).mean() > 0.1
+
+
+def brute_estimate(ph, dp, scale, q, radius, max_depth, k):
+    """photon_map::estimate_irradiance (photon_map.h:72-115) by exhaustive
+    search and float64 sums: the k nearest photons within radius, filtered by
+    depth and orientation, weighted by the Epanechnikov-squared kernel."""
+    pos, d, n, pw = (ph[:, 0:3].astype(np.float64), ph[:, 3:6].astype(np.float64), ph[:, 6:9].astype(np.float64),
+                     ph[:, 9:12].astype(np.float64))
+    out = np.zeros((len(q), 3))
+    r2 = float(np.float32(radius) * np.float32(radius))
+    for i, row in enumerate(q.astype(np.float64)):
+        d2 = ((pos - row[:3]) ** 2).sum(1)
+        idx = np.nonzero(d2 <= r2)[0]
+        idx = idx[np.argsort(d2[idx], kind="stable")][:k]
+        s = np.zeros(3)
+        for j in idx:
+            wi = -d[j]
+            ip, is_ = n[j] @ wi, row[3:] @ wi
+            if dp[j, 0] > max_depth or is_ <= 0 or n[j] @ row[3:] <= 0.1 or ip <= 0.01:
+                continue
+            s += pw[j] * abs(is_ / ip * (1 - d2[j] / r2) ** 2)
+        out[i] = s * scale * 3.0 / np.pi / r2
+    return out
+
+
+@pytest.mark.parametrize("r,md", ((3.5, 100), (3.5, 4)))
+def test_estimate_against_brute_force(gold, r, md):
+    """With k >= the photon count and a radius that covers the room the search
+    is exhaustive, so the estimate must equal the brute-force one: this pins
+    the kernel weight, the orientation and depth filters and the scale."""
+    ph, dp, q = gold["photons"][:700], gold["depth_particle"][:700], gold["queries"]
+    e = OL.photon_estimate(OL.oracle, ph, dp, 1.0 / 600, q, r, md, len(ph))
+    b = brute_estimate(ph, dp, 1.0 / 600, q, r, md, len(ph))
+    assert (b > 0).any(1).mean() > 0.5
+    assert np.allclose(e, b, rtol=2e-5, atol=1e-6 * np.abs(b).max())
+
+
+def test_brute_force_catches_a_wrong_radius(gold):
+    ph, dp, q = gold["photons"][:700], gold["depth_particle"][:700], gold["queries"]
+    e = OL.photon_estimate(OL.oracle, ph, dp, 1.0 / 600, q, 3.5, 100, len(ph))
+    wrong = brute_estimate(ph, dp, 1.0 / 600, q, 3.6, 100, len(ph)) * (3.6 / 3.5) ** 2
+    assert not np.allclose(e, wrong, rtol=1e-3)
+
+
+def test_reference_search_is_approximate(gold):
+    """photon_map::build_range reads the split value from the median slot
+    after both children have been partitioned (photon_map.h:104-110), so the
+    far-side pruning can skip photons inside the radius: with a small radius
+    the reference's k-nearest sets differ from the exact ones for some
+    queries.  The oracle (pinned to the reference above) keeps this."""
+    ph, dp, q = gold["photons"], gold["depth_particle"], gold["queries"]
+    e = gold["est_k16_r0.1_d100"]
+    b = brute_estimate(ph, dp, 1.0 / 600, q, 0.1, 100, 16)
+    differs = ~np.isclose(e, b, rtol=1e-4, atol=1e-9).all(1)
+    assert 0 < differs.sum() < len(q) // 3
+
+
+def test_gather_quota():
+    """Particles are traced in order until `requested` photons are returned;
+    the first `requested` are kept and `shot` counts every traced particle."""
+    g = linrt.box_icosphere(seed=1, subdivisions=3)
+    s = OL.oracle.scene(g.desc_ptr)
+    req = 5000
+    ph, dp, shot = s.gather_photons(req, 8, 5)
+    assert len(ph) == req
+    allp, alld = s.trace_particles(0, shot, 8, 5)
+    assert len(allp) >= req
+    assert np.array_equal(allp[:req], ph) and np.array_equal(alld[:req], dp)
+    # one particle fewer returns fewer than requested
+    fewer, _ = s.trace_particles(0, shot - 1, 8, 5)
+    assert len(fewer) < req
+
+
+def test_eye_pass_oracle_is_the_reference():
+    """photon_integrator::radiance per sample (emission, gathers, specular
+    lobes, emitter and BSDF samples with count-weighted MIS) on the sphere room
+    with the reference sampler: bit-exact against the reference's values."""
+    from common import SphereRoom
+    d = np.load(GOLD / "eye_sphere_room_16x16x2.npz")
+    sr = SphereRoom()
+    R = float(d["radius"])
+    maps = [(d["g_photons"], d["g_dp"], 1.0 / 1500, 0.05 * R), (d["c_photons"], d["c_dp"], 1.0 / 1500, 0.0125 * R)]
+    ps = OL.oracle.scene(sr.desc_ptr).render_photon(16, 16, 2, maps, mode=1)
+    assert np.array_equal(ps, d["per_sample"])
+    assert (ps > 0).any(-1).mean() > 0.9
+
+
+def test_eye_pass_philox_streams_differ_but_agree():
+    """The device's Philox streams (mode 0) give a different estimate of the
+    same image: pixel means agree within their standard errors."""
+    from common import SphereRoom
+    d = np.load(GOLD / "eye_sphere_room_16x16x2.npz")
+    sr = SphereRoom()
+    R = float(d["radius"])
+    maps = [(d["g_photons"], d["g_dp"], 1.0 / 1500, 0.05 * R), (d["c_photons"], d["c_dp"], 1.0 / 1500, 0.0125 * R)]
+    s = OL.oracle.scene(sr.desc_ptr)
+    a = s.render_photon(8, 8, 64, maps, mode=0)
+    b = s.render_photon(8, 8, 64, maps, mode=1)
+    assert not np.array_equal(a, b)
+    ma, mb = a.mean(2), b.mean(2)
+    se = np.sqrt(a.var(2) / 64 + b.var(2) / 64) + 1e-6
+    z = np.abs(ma - mb) / se
+    assert np.mean(z < 3) > 0.97

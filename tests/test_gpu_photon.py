@@ -1,0 +1,161 @@
+"""GPU photon mapping against the oracle and the reference's own outputs:
+particle tracing (lrt_photon_build), the kd photon map and its k-nearest
+irradiance estimate (lrt_photon_estimate), and the photon-integrator eye pass
+(lrt_render_photon).
+
+Bars: photon estimates on identical photon lists equal to 1e-5 relative
+(the search and the summation order are the reference's, so they are normally
+bit-exact); traced photons equal to the oracle's for >= 99% of particles
+(BSDF sampling agrees to 1e-5 relative, so a few particle paths may fork);
+eye-pass pixel means equal to 1e-4 relative for >= 99% of pixel-channels on
+the same photon maps, the rest within 3 sigma.
+"""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import linrt
+import oracle_lib as OL
+from common import SphereRoom
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module")
+def box():
+    g = linrt.box_icosphere(seed=1, subdivisions=3)
+    return g, linrt.Scene(g.desc_ptr)
+
+
+@pytest.fixture(scope="module")
+def pbox():
+    g = linrt.photon_box(seed=3)
+    return g, linrt.Scene(g.desc_ptr), OL.oracle.scene(g.desc_ptr)
+
+
+@pytest.mark.parametrize("k,r,md", ((16, 0.1, 100), (64, 0.3, 100), (64, 0.3, 3)))
+def test_estimate_matches_reference_fixture(box, k, r, md):
+    """The reference's photons and queries: the device kd search and sum give
+    the reference's estimates."""
+    d = np.load(GOLD / "photons_box_icosphere_l3.npz")
+    _, sc = box
+    sc.photon_map_load(d["photons"], d["depth_particle"], 1.0 / 600, r, k)
+    e = sc.photon_estimate(d["queries"], md)
+    ref = d[f"est_k{k}_r{r}_d{md}"]
+    exact = np.mean(np.all(e == ref, axis=1))
+    print(f"k={k} r={r} depth<={md}: {exact:.4%} of queries bit-exact")
+    assert np.allclose(e, ref, rtol=1e-5, atol=1e-7 * np.abs(ref).max())
+    assert exact >= 0.99
+
+
+def test_estimate_empty_map(box):
+    _, sc = box
+    sc.photon_map_load(np.zeros((0, 12), np.float32), np.zeros((0, 2), np.uint32), 1.0, 0.1, 16)
+    assert not sc.photon_estimate(np.zeros((4, 6), np.float32), 10).any()
+
+
+def test_photon_trace_vs_oracle(pbox):
+    """Quota gather of 60k photons (max depth 8, RR from 5) on the 262,144-tri
+    photon box, global and caustic maps, against the oracle's particle-order
+    gather with the same Philox streams."""
+    g, sc, orc = pbox
+    for kind, req in ((linrt.PHOTON_SURFACE, 60000), (linrt.PHOTON_CAUSTIC, 20000)):
+        st = sc.photon_build(req, 8, 5, seed=7, radius=0.05, k=64, kind=kind)
+        ph, dp, shot = sc.photon_export(kind)
+        oph, odp, oshot = orc.gather_photons(req, 8, 5, kind=kind, seed=7)
+        print(f"kind {kind}: {st.photons} photons, shot {st.shot} (oracle {oshot}), traced {st.particles_traced}, "
+              f"trace {st.trace_ms:.1f} ms, map {st.sort_ms:.1f} ms")
+        assert st.full == 1 and st.photons == req and len(ph) == req and len(oph) == req
+        assert abs(int(shot) - int(oshot)) <= max(2, oshot // 1000)
+        # per particle: same photon records
+        kd = dp[:, 1].astype(np.int64) << 8 | dp[:, 0]
+        ko = odp[:, 1].astype(np.int64) << 8 | odp[:, 0]
+        common, ia, ib = np.intersect1d(kd, ko, return_indices=True)
+        same = np.all(np.isclose(ph[ia], oph[ib], rtol=1e-4, atol=1e-5), axis=1)
+        parts = np.unique(odp[:, 1])
+        bad_parts = np.unique(np.concatenate([odp[ib[~same], 1], np.setxor1d(kd, ko) >> 8]))
+        print(f"  {len(common)} common records, {same.mean():.4%} equal, particles forked: {len(bad_parts)} "
+              f"of {len(parts)}")
+        assert len(bad_parts) <= 0.01 * len(parts)
+        assert np.mean(np.all(ph[ia] == oph[ib], axis=1)) > 0.95
+
+
+def test_estimate_large_map_vs_oracle(pbox):
+    """20,000 queries with k = 120 over the device-built global map, against
+    the oracle's photon_map on the exported photons."""
+    g, sc, orc = pbox
+    sc.photon_build(60000, 8, 5, seed=7, radius=0.05, k=120)
+    ph, dp, shot = sc.photon_export()
+    rng = np.random.default_rng(11)
+    pick = rng.integers(0, len(ph), 20000)
+    q = np.concatenate([ph[pick, :3] + rng.normal(scale=0.02, size=(20000, 3)).astype(np.float32),
+                        ph[rng.permutation(pick), 6:9]], 1).astype(np.float32)
+    for md in (100, 4):
+        e = sc.photon_estimate(q, md)
+        o = OL.photon_estimate(OL.oracle, ph, dp, 1.0 / shot, q, 0.05, md, 120)
+        exact = np.mean(np.all(e == o, axis=1))
+        print(f"depth<={md}: {exact:.4%} bit-exact, {np.mean(o.any(1)):.2%} nonzero")
+        assert np.allclose(e, o, rtol=1e-5, atol=1e-7 * np.abs(o).max())
+        assert exact >= 0.99
+
+
+def compare_images(a_sum, a_sq, b_ps, label):
+    spp = b_ps.shape[2]
+    ma, mb = a_sum / spp, b_ps.astype(np.float64).mean(2)
+    close = np.isclose(ma, mb, rtol=1e-4, atol=1e-6)
+    va = np.maximum(a_sq / spp - ma * ma, 0) / spp
+    vb = b_ps.astype(np.float64).var(2) / spp
+    se = np.sqrt(va + vb) + 1e-9
+    z = np.abs(ma - mb) / se
+    print(f"{label}: {close.mean():.4%} of pixel-channels within 1e-4, worst |z| {np.sort(z[~close])[-3:] if (~close).any() else 0}, "
+          f"mean {ma.mean():.5f} vs {mb.mean():.5f}")
+    assert close.mean() >= 0.99
+    assert np.mean(z[~close] > 3) <= 0.01 * z.size if (~close).any() else True
+
+
+def test_eye_pass_sphere_room_vs_oracle():
+    """Sphere room (smooth dielectric, rough conductor, plastic, coating) with
+    the reference's maps from the fixture: every li() branch runs."""
+    d = np.load(GOLD / "eye_sphere_room_16x16x2.npz")
+    sr = SphereRoom()
+    sc = linrt.Scene(sr.desc_ptr)
+    R = float(d["radius"])
+    assert sc.info.bounding_radius == np.float32(R) and sc.info.has_delta_bsdf == 1
+    sc.photon_map_load(d["g_photons"], d["g_dp"], 1.0 / 1500, 0.05 * R, 120, linrt.PHOTON_SURFACE)
+    sc.photon_map_load(d["c_photons"], d["c_dp"], 1.0 / 1500, 0.0125 * R, 120, linrt.PHOTON_CAUSTIC)
+    W, H, spp = 32, 32, 4
+    s, q, st = sc.render_photon(W, H, spp, 128, seed=1)
+    maps = [(d["g_photons"], d["g_dp"], 1.0 / 1500, 0.05 * R), (d["c_photons"], d["c_dp"], 1.0 / 1500, 0.0125 * R)]
+    ps = OL.oracle.scene(sr.desc_ptr).render_photon(W, H, spp, maps, mode=0, seed=1)
+    print(f"extension {st.extension_rays}, shadow {st.shadow_rays}, queries {st.segments}, {st.seconds * 1e3:.1f} ms")
+    compare_images(s, q, ps, "sphere room")
+
+
+def test_eye_pass_photon_box_vs_oracle(pbox):
+    """The photon box (seven region materials incl. rough coatings) with
+    device-built global and caustic maps, exported to the oracle."""
+    g, sc, orc = pbox
+    R = sc.info.bounding_radius
+    sc.photon_build(40000, 127, 5, seed=5, radius=0.05 * R, k=120, kind=linrt.PHOTON_SURFACE)
+    sc.photon_build(10000, 127, 5, seed=5, radius=0.0125 * R, k=120, kind=linrt.PHOTON_CAUSTIC)
+    gp, gd, gs = sc.photon_export(linrt.PHOTON_SURFACE)
+    cp, cd, cs = sc.photon_export(linrt.PHOTON_CAUSTIC)
+    W, H, spp = 24, 24, 2
+    s, q, st = sc.render_photon(W, H, spp, 128, seed=3)
+    maps = [(gp, gd, 1.0 / gs, 0.05 * R), (cp, cd, 1.0 / cs, 0.0125 * R)]
+    ps = orc.render_photon(W, H, spp, maps, mode=0, seed=3)
+    compare_images(s, q, ps, "photon box")
+
+
+def test_photon_integrator_host_mirror():
+    """PhotonIntegrator.preprocess builds both maps with the reference's
+    constants (250,000 photons, depth 127, radii 0.05 / 0.0125 x R, k 120)."""
+    sr = SphereRoom()
+    sc = linrt.Scene(sr.desc_ptr)
+    pi = linrt.PhotonIntegrator(global_photons=50000, caustic_photons=20000)
+    st = pi.preprocess(sc)
+    assert len(st) == 2 and all(s.full for s in st)
+    s, q, rs = pi.render(sc, 16, 16, 1)
+    assert np.isfinite(s).all() and s.mean() > 0
