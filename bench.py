@@ -109,6 +109,55 @@ def ray_bench(args, torch, linrt):
     print(json.dumps(out), flush=True)
 
 
+def photon_bench(args, torch, linrt):
+    """Photon-mapper workload: 4,194,304 photons traced (max depth 8, RR from
+    5) from the 0.5 x 0.5 ceiling light through the 262,144-tri displaced
+    sphere inside the box room, the kd photon map built over them, then the
+    photon-integrator eye pass at 1024x1024 (k = 64, radius 0.05, eye depth
+    limit 128).  One step = photon pass + map build + eye pass."""
+    gen = linrt.photon_box(seed=SEED)
+    scene = linrt.Scene(gen.desc_ptr, device=0)
+    n_ph, res, spp = 4194304, 1024, args.spp if args.spp != SPP else 16
+    scene.photon_map_load(np.zeros((0, 12), np.float32), np.zeros((0, 2), np.uint32), 1.0, 0.05, 64,
+                          linrt.PHOTON_CAUSTIC)
+
+    def step():
+        st = scene.photon_build(n_ph, 8, 5, seed=SEED, radius=0.05, k=64)
+        s, q, rs = scene.render_photon(res, res, spp, 128, seed=SEED, sumsq=False)
+        return st, rs
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rec = []
+    for k in range(args.steps):
+        rec.append(step())
+        log(f"step {k}: photons {rec[-1][0].seconds:.3f}s (trace {rec[-1][0].trace_ms:.1f} ms, map "
+            f"{rec[-1][0].sort_ms:.1f} ms), eye {rec[-1][1].seconds:.3f}s")
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    ph_s = sum(r[0].seconds for r in rec)
+    eye_s = sum(r[1].seconds for r in rec)
+    eye = rec[-1][1]
+    samples = res * res * spp
+    out = {"metric": "photon-mapped frame: eye Msamples/s (1024x1024, 4M photons, 262k tris)",
+           "value": round(samples * args.steps / eye_s / 1e6, 3), "unit": "Msamples/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 2),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic (seeded scene generator)",
+           "config": {"workload": f"photon box: {n_ph} photons (depth 8), {res}x{res} at {spp} spp, k 64, r 0.05",
+                      "photons": n_ph, "width": res, "height": res, "spp": spp},
+           "photon_pass_s": round(ph_s / args.steps, 4),
+           "mphotons_per_s": round(n_ph * args.steps / ph_s / 1e6, 2),
+           "eye_pass_s": round(eye_s / args.steps, 4),
+           "eye_rays_per_sample": round((eye.extension_rays + eye.shadow_rays) / samples, 2),
+           "eye_mrays_per_s": round((eye.extension_rays + eye.shadow_rays) / eye.seconds / 1e6, 1),
+           "eye_kernel_ms": {"extend": round(eye.trace_ms, 1), "shadow": round(eye.occluded_ms, 1),
+                             "shade": round(eye.shade_ms, 1)}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +166,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--spp", type=int, default=SPP)
     ap.add_argument("--rays", type=int, default=0, help="ray-traversal benchmark with this many rays instead")
+    ap.add_argument("--photon", action="store_true", help="photon-mapper benchmark instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by pixel/sample ranges "
@@ -128,6 +178,8 @@ def main():
 
     if args.rays:
         return ray_bench(args, torch, linrt)
+    if args.photon:
+        return photon_bench(args, torch, linrt)
 
     if linrt.device_count() < 1 or not torch.cuda.is_available():
         log("no GPU visible")

@@ -1210,7 +1210,9 @@ int lrt_photon_estimate(lrt_scene* sc, int32_t kind, const float* queries6, uint
         upload(sc->q_rays, queries6, (size_t)n * 24, sc->stream);
         sc->q_out0.alloc((size_t)n * 12);
         const uint32_t g = std::min<uint32_t>((n + kGatherBlock - 1) / kGatherBlock, (uint32_t)sc->cus * 16);
-        hipLaunchKernelGGL(k_photon_estimate, dim3(g), dim3(kGatherBlock), 0, sc->stream, S.map, sc->q_rays.as<float>(), n,
+        const size_t lds = gather_lds_bytes(S.map.k);
+        HIPCHECK(hipFuncSetAttribute((const void*)k_photon_estimate, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_photon_estimate, dim3(g), dim3(kGatherBlock), lds, sc->stream, S.map, sc->q_rays.as<float>(), n,
                            max_depth, sc->q_out0.as<float>());
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(out3, sc->q_out0.p, (size_t)n * 12, hipMemcpyDeviceToHost, sc->stream));
@@ -1320,6 +1322,8 @@ static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, doub
     PhotonMapDev G0 = sc->pm[0].map, G1 = sc->pm[1].map;
     if (!(maps & 1u)) std::memset(&G0, 0, sizeof(G0));
     if (!(maps & 2u)) std::memset(&G1, 0, sizeof(G1));
+    const size_t gather_lds = gather_lds_bytes(std::max(G0.k, G1.k));
+    HIPCHECK(hipFuncSetAttribute((const void*)k_eye_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)gather_lds));
     const uint64_t pixels = (uint64_t)(re - rb) * W;
     const uint32_t chunk = (uint32_t)std::min<uint64_t>(pixels, 1u << 18);
     sc->counters.alloc(256);
@@ -1410,7 +1414,7 @@ static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, doub
                     hipLaunchKernelGGL(k_eye_gather,
                                        dim3((uint32_t)std::min<uint64_t>((cnt[2] + kGatherBlock - 1) / kGatherBlock,
                                                                          (uint64_t)sc->cus * 16)),
-                                       dim3(kGatherBlock), 0, sc->stream, G0, G1, Q, cnt[2], acc);
+                                       dim3(kGatherBlock), gather_lds, sc->stream, G0, G1, Q, cnt[2], acc);
                 HIPCHECK(hipGetLastError());
                 cur = nxt;
                 std::swap(A, B);

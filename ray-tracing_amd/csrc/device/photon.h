@@ -283,13 +283,23 @@ constexpr int kGatherBlock = 64;
 constexpr int kMaxK = 128;
 constexpr int kPkStack = 32;  // > photon kd depth (checked at build)
 
-// LDS of one gather lane: heap of (d2, index) and the far-child stack, each
-// strided by the block size
+// LDS of a gather block (dynamic shared memory, sized for the launch's k):
+// per lane a heap of (d2, index) pairs of kcap entries and the far-child
+// stack, each strided by the block size
 struct GatherLds {
-    float hd[kMaxK * kGatherBlock];
-    uint32_t hi[kMaxK * kGatherBlock];
-    uint32_t st[kPkStack * kGatherBlock];
+    float* hd;
+    uint32_t* hi;
+    uint32_t* st;
 };
+inline size_t gather_lds_bytes(int kcap) { return (size_t)(2 * kcap + kPkStack) * kGatherBlock * 4; }
+LRT_D GatherLds gather_lds(int kcap) {
+    extern __shared__ uint32_t gather_smem[];
+    GatherLds G;
+    G.hd = (float*)gather_smem + threadIdx.x;
+    G.hi = gather_smem + (size_t)kcap * kGatherBlock + threadIdx.x;
+    G.st = gather_smem + (size_t)2 * kcap * kGatherBlock + threadIdx.x;
+    return G;
+}
 
 // heap entries are (d2, index) pairs compared like std::pair (priority_queue)
 LRT_D bool pair_less(float da, uint32_t ia, float db, uint32_t ib) { return da < db || (!(db < da) && ia < ib); }
@@ -318,12 +328,12 @@ LRT_D void heap_pop(float* hd, uint32_t* hi, int& size) {
 // estimate over them (photon_map.h:38-70).  The recursion's far-child test
 // uses the limit taken when the node was entered, so it is decided on entry
 // and the far child pushed before descending into the near one.
-LRT_D f3 photon_estimate(const PhotonMapDev& Mp, f3 p, f3 n, int max_depth, GatherLds& G) {
+LRT_D f3 photon_estimate(const PhotonMapDev& Mp, f3 p, f3 n, int max_depth, const GatherLds& G) {
     if (Mp.n == 0 || Mp.radius <= 0.0f || Mp.k <= 0) return splat(0.0f);
     constexpr uint32_t st = kGatherBlock;
-    float* hd = G.hd + threadIdx.x;
-    uint32_t* hi = G.hi + threadIdx.x;
-    uint32_t* stack = G.st + threadIdx.x;
+    float* hd = G.hd;
+    uint32_t* hi = G.hi;
+    uint32_t* stack = G.st;
     const float r2 = Mp.radius * Mp.radius;
     const int K = Mp.k;
     int size = 0, sp = 0;
@@ -383,7 +393,7 @@ LRT_D f3 photon_estimate(const PhotonMapDev& Mp, f3 p, f3 n, int max_depth, Gath
 
 __global__ __launch_bounds__(kGatherBlock) void k_photon_estimate(const PhotonMapDev Mp, const float* __restrict__ q6,
                                                                   uint32_t n, int max_depth, float* __restrict__ out) {
-    __shared__ GatherLds G;
+    const GatherLds G = gather_lds(Mp.k);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const f3 e = photon_estimate(Mp, ld3(q6 + 6 * (size_t)i), ld3(q6 + 6 * (size_t)i + 3), max_depth, G);
         out[3 * (size_t)i] = e.x;

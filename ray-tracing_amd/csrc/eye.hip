@@ -313,7 +313,7 @@ __global__ __launch_bounds__(kBlock) void k_eye_shadow(const DevScene S, const E
 
 __global__ __launch_bounds__(kGatherBlock) void k_eye_gather(const PhotonMapDev G0, const PhotonMapDev G1, const EyeQuery Q,
                                                              uint32_t n, unsigned long long* __restrict__ acc) {
-    __shared__ GatherLds L;
+    const GatherLds L = gather_lds(max(G0.k, G1.k));
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t meta = Q.meta[i];
         const f3 E = photon_estimate((meta & 1u) ? G1 : G0, f3{Q.px[i], Q.py[i], Q.pz[i]}, f3{Q.nx[i], Q.ny[i], Q.nz[i]},
