@@ -330,6 +330,41 @@ int lrt_generated_free(lrt_generated* g);
 /* ---- Philox-4x32-10 (the counter-based sampler of every kernel) -------- */
 int lrt_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+/* ---- first-hit features for the denoiser (no counterpart in the reference:
+ *      its renderer has no denoiser) --------------------------------------
+ * Per pixel, averaged over the frame's jittered camera rays (the same rays
+ * lrt_render traces first): albedo (3: diffuse colour, normal-incidence
+ * Fresnel for conductors, 1 for dielectrics, emitters and misses), shading
+ * normal (3, 0 on a miss) and hit distance (1, 0 on a miss); 7 floats per
+ * pixel, rows bottom-up like the radiance.  device_ptrs: aov is device memory. */
+int lrt_render_aov(lrt_scene* scene, const lrt_render_params* params, float* aov, int device_ptrs);
+
+/* ---- learned denoiser (no counterpart in the reference) -------------------
+ * A 3-level U-Net (32 / 64 / 128 channels, 3x3 convolutions) over radiance +
+ * first-hit features, run and trained with the library's BF16 matrix-core
+ * kernels (FP32 accumulation, FP32 master weights, Adam).  Images are
+ * B x H x W pixels (H, W multiples of 4), radiance 3 floats and aux 7 floats
+ * (lrt_render_aov) per pixel; the output is the denoised radiance. */
+typedef struct lrt_denoiser lrt_denoiser;
+int lrt_denoiser_create(int hip_device, uint64_t seed, lrt_denoiser** out);
+int lrt_denoiser_release(lrt_denoiser* d);
+int lrt_denoiser_param_count(const lrt_denoiser* d, uint64_t* n);
+/* copies the FP32 parameters out (get) and/or in (set; resets Adam) */
+int lrt_denoiser_params(lrt_denoiser* d, float* get, const float* set);
+int lrt_denoiser_forward(lrt_denoiser* d, const float* radiance, const float* aux, uint32_t B, uint32_t H, uint32_t W,
+                         float* out, int device_ptrs);
+/* L1 loss in log(1 + radiance) against target and its parameter gradient
+ * (kept for lrt_denoiser_adam_step; copied to grads when non-null) */
+int lrt_denoiser_loss_grad(lrt_denoiser* d, const float* radiance, const float* aux, const float* target, uint32_t B,
+                           uint32_t H, uint32_t W, float* loss, float* grads, int device_ptrs);
+int lrt_denoiser_adam_step(lrt_denoiser* d, float lr);
+/* one 3x3 convolution layer on host FP32 arrays (rounded to BF16 on the
+ * device): op 0 out = act(conv(x, w) + b) [B,H,W,cout]; op 1 the data
+ * gradient out = conv^T(g, w) [B,H,W,cin]; op 2 the weight and bias gradient
+ * out = (dW [9][cin][cout], db [cout]) of x and g.  w: [9][cin][cout]. */
+int lrt_conv3x3(int hip_device, int op, const float* x, const float* w, const float* b, const float* g, uint32_t B,
+                uint32_t H, uint32_t W, uint32_t cin, uint32_t cout, int relu, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -983,6 +983,32 @@ int lrt_render(lrt_scene* sc, const lrt_render_params* prm, double* sum_rgb, dou
     return guarded([&] { render_impl(sc, prm, sum_rgb, sumsq_rgb, device_ptrs, stats); });
 }
 
+int lrt_render_aov(lrt_scene* sc, const lrt_render_params* prm, float* aov, int device_ptrs) {
+    return guarded([&] {
+        require(sc && prm && aov, "null pointer");
+        require(prm->width >= 2 && prm->height >= 2 && prm->spp >= 1, "bad render parameters");
+        const uint32_t W = prm->width, H = prm->height;
+        const uint32_t rb = prm->row_end > prm->row_begin ? prm->row_begin : 0;
+        const uint32_t re = prm->row_end > prm->row_begin ? prm->row_end : H;
+        require(re <= H, "row range outside the image");
+        HIPCHECK(hipSetDevice(sc->device));
+        const uint32_t n = (re - rb) * W;
+        const size_t bytes = (size_t)W * H * 7 * 4;
+        float* d = aov;
+        if (!device_ptrs) {
+            sc->q_out0.alloc(bytes);
+            d = sc->q_out0.as<float>();
+            HIPCHECK(hipMemcpyAsync(d, aov, bytes, hipMemcpyHostToDevice, sc->stream));
+        }
+        const uint2 key = make_uint2((uint32_t)prm->seed, (uint32_t)(prm->seed >> 32));
+        hipLaunchKernelGGL(k_aov, dim3(grid_for(*sc, n, 20)), dim3(kBlock), 0, sc->stream, sc->S, sc->M, sc->cam, W, H, rb, n,
+                           prm->spp, prm->spp_offset, key, d);
+        HIPCHECK(hipGetLastError());
+        if (!device_ptrs) HIPCHECK(hipMemcpyAsync(aov, d, bytes, hipMemcpyDeviceToHost, sc->stream));
+        HIPCHECK(hipStreamSynchronize(sc->stream));
+    });
+}
+
 // ---- photon mapping -----------------------------------------------------------------
 static void radix_sort(lrt_scene& sc, PhotonState& S, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t n,
                        int bits, uint32_t*& out_k, uint32_t*& out_v) {
@@ -1473,3 +1499,5 @@ int lrt_generated_free(lrt_generated* g) {
 namespace lrt {
 void set_error(const std::string& msg) { g_error = msg; }
 }  // namespace lrt
+
+#include "denoise.hip"

@@ -371,7 +371,70 @@ __global__ __launch_bounds__(kBlock) void k_shade(const DevScene S, const DevLig
 }
 
 
-// shadow rays: add the pending NEE contribution when unoccluded
+// first-hit albedo for the denoiser's features: diffuse colour, the
+// normal-incidence Fresnel reflectance of a conductor, 1 for dielectrics
+LRT_D f3 albedo_of(const Mats& M, int m) {
+    const lrt_material& A = M.m[m];
+    switch (A.type) {
+        case LRT_BSDF_DIFFUSE:
+        case LRT_BSDF_PLASTIC:
+        case LRT_BSDF_ROUGH_PLASTIC: return col(A.diffuse);
+        case LRT_BSDF_CONDUCTOR: {
+            f3 f;
+            float* o = &f.x;
+            for (int c = 0; c < 3; ++c) {
+                const float e = A.cond_eta[c], k = A.cond_k[c];
+                o[c] = ((e - 1.0f) * (e - 1.0f) + k * k) / ((e + 1.0f) * (e + 1.0f) + k * k) * A.specular_reflectance[c];
+            }
+            return f;
+        }
+        case LRT_BSDF_COATING:
+        case LRT_BSDF_ROUGH_COATING: {
+            const lrt_material& N = M.m[A.nested];
+            if (N.type == LRT_BSDF_DIFFUSE || N.type == LRT_BSDF_PLASTIC || N.type == LRT_BSDF_ROUGH_PLASTIC) return col(N.diffuse);
+            return splat(1.0f);
+        }
+        default: return splat(1.0f);
+    }
+}
+
+// per-pixel average over the frame's camera rays of (albedo, normal, depth)
+__global__ __launch_bounds__(kBlock) void k_aov(const DevScene S, const Mats M, const CameraDev cam, uint32_t W,
+                                                uint32_t H, uint32_t row0, uint32_t n, uint32_t spp, uint32_t spp0,
+                                                uint2 key, float* __restrict__ aov) {
+    __shared__ uint2 stack[kLdsStack * kBlock];
+    uint2* lds = stack + threadIdx.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t x = i % W, y = row0 + i / W;
+        const uint32_t pix = y * W + x;
+        f3 al = splat(0.0f), nr = splat(0.0f);
+        float dp = 0.0f;
+        for (uint32_t s = 0; s < spp; ++s) {
+            const uint4 v = philox(make_uint4(pix, spp0 + s, 0u, 0u), key);
+            const float u = ((float)x + u24(v.x)) / (float)(W - 1);
+            const float vv = ((float)y + u24(v.y)) / (float)(H - 1);
+            const Ray r{ld3(cam.origin),
+                        normalize(ld3(cam.llc) + u * ld3(cam.horizontal) + vv * ld3(cam.vertical) - ld3(cam.origin))};
+            HitInfo h;
+            Counters c{0, 0, 0};
+            if (trace_closest<false>(S, r, 0.001f, kInfF, h, lds, c)) {
+                const SurfaceHit sh = surface(S, r, h.obj, h.prim, h.t, h.u, h.v);
+                al = al + (sh.material >= 0 ? albedo_of(M, sh.material) : splat(1.0f));
+                nr = nr + sh.n;
+                dp += h.t;
+            } else {
+                al = al + splat(1.0f);
+            }
+        }
+        const float inv = 1.0f / (float)spp;
+        float* o = aov + 7 * (size_t)pix;
+        o[0] = al.x * inv; o[1] = al.y * inv; o[2] = al.z * inv;
+        o[3] = nr.x * inv; o[4] = nr.y * inv; o[5] = nr.z * inv;
+        o[6] = dp * inv;
+    }
+}
+
+
 __global__ __launch_bounds__(kBlock) void k_shadow(const DevScene S, PathState P, const uint32_t* __restrict__ counters,
                                                    const ShadowBuf Sb) {
     __shared__ uint2 stack[kLdsStack * kBlock];

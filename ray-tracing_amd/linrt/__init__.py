@@ -141,7 +141,9 @@ EXPORTED = [
     "lrt_scene_sync", "lrt_scene_kernel_timing", "lrt_bsdf_eval_pdf", "lrt_bsdf_sample", "lrt_bsdf_flags",
     "lrt_fresnel_diffuse_reflectance", "lrt_rough_transmittance_slice", "lrt_sky_eval", "lrt_render",
     "lrt_photon_build", "lrt_photon_export", "lrt_photon_map_load", "lrt_photon_estimate", "lrt_render_photon",
-    "lrt_generate_scene", "lrt_generated_desc", "lrt_generated_free", "lrt_philox4x32_10",
+    "lrt_generate_scene", "lrt_generated_desc", "lrt_generated_free", "lrt_philox4x32_10", "lrt_render_aov",
+    "lrt_denoiser_create", "lrt_denoiser_release", "lrt_denoiser_param_count", "lrt_denoiser_params",
+    "lrt_denoiser_forward", "lrt_denoiser_loss_grad", "lrt_denoiser_adam_step", "lrt_conv3x3",
 ]
 
 
@@ -358,6 +360,19 @@ class Scene:
                               1, C.byref(st)))
         return st
 
+    def render_aov(self, width, height, spp, seed=1, spp_offset=0, rows=None):
+        """First-hit (albedo, normal, depth) averaged over the frame's camera
+        rays: [H, W, 7] float32 (rows bottom-up like render)."""
+        aov = np.zeros((height, width, 7), np.float32)
+        rb, re = rows if rows is not None else (0, 0)
+        prm = RenderParams(width, height, spp, spp_offset, 1, 0, seed, rb, re, 0)
+        _check(LIB.lrt_render_aov(self._h, C.byref(prm), _p(aov), 0))
+        return aov
+
+    def render_aov_device(self, width, height, spp, aov_ptr, seed=1, spp_offset=0):
+        prm = RenderParams(width, height, spp, spp_offset, 1, 0, seed, 0, 0, 0)
+        _check(LIB.lrt_render_aov(self._h, C.byref(prm), C.c_void_p(aov_ptr), 1))
+
     # ---- photon mapping (photon_integrator.h:15-123) ----
     def photon_build(self, requested, max_depth, rr_depth, seed=1, radius=0.05, k=120, kind=PHOTON_SURFACE):
         """Traces particles until `requested` photons are stored and builds map
@@ -433,6 +448,97 @@ class PhotonIntegrator:
 
     def render(self, scene, width, height, spp, spp_offset=0):
         return scene.render_photon(width, height, spp, self.max_depth, self.seed, spp_offset)
+
+
+class Denoiser:
+    """The BF16 U-Net denoiser (lrt_denoiser_*): forward on host or device
+    arrays, loss + gradient, Adam steps, parameter get/set."""
+
+    def __init__(self, device=0, seed=1):
+        h = C.c_void_p()
+        _check(LIB.lrt_denoiser_create(C.c_int(device), C.c_uint64(seed), C.byref(h)))
+        self._h = h
+        n = C.c_uint64()
+        _check(LIB.lrt_denoiser_param_count(self._h, C.byref(n)))
+        self.n_params = n.value
+
+    def close(self):
+        if self._h:
+            LIB.lrt_denoiser_release(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        _check(LIB.lrt_denoiser_params(self._h, _p(p), None))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        _check(LIB.lrt_denoiser_params(self._h, None, _p(p)))
+
+    def forward(self, rad, aux):
+        """rad [B,H,W,3], aux [B,H,W,7] float32 -> denoised [B,H,W,3]."""
+        rad = np.ascontiguousarray(rad, np.float32)
+        aux = np.ascontiguousarray(aux, np.float32)
+        if rad.ndim == 3:
+            rad, aux = rad[None], aux[None]
+        B, H, W, _ = rad.shape
+        out = np.zeros((B, H, W, 3), np.float32)
+        _check(LIB.lrt_denoiser_forward(self._h, _p(rad), _p(aux), C.c_uint32(B), C.c_uint32(H), C.c_uint32(W), _p(out), 0))
+        return out
+
+    def forward_device(self, rad_ptr, aux_ptr, B, H, W, out_ptr):
+        _check(LIB.lrt_denoiser_forward(self._h, C.c_void_p(rad_ptr), C.c_void_p(aux_ptr), C.c_uint32(B), C.c_uint32(H),
+                                        C.c_uint32(W), C.c_void_p(out_ptr), 1))
+
+    def loss_grad(self, rad, aux, target, grads=True):
+        rad = np.ascontiguousarray(rad, np.float32)
+        aux = np.ascontiguousarray(aux, np.float32)
+        target = np.ascontiguousarray(target, np.float32)
+        B, H, W, _ = rad.shape
+        loss = C.c_float()
+        g = np.zeros(self.n_params, np.float32) if grads else None
+        _check(LIB.lrt_denoiser_loss_grad(self._h, _p(rad), _p(aux), _p(target), C.c_uint32(B), C.c_uint32(H),
+                                          C.c_uint32(W), C.byref(loss), _p(g), 0))
+        return loss.value, g
+
+    def loss_grad_device(self, rad_ptr, aux_ptr, tgt_ptr, B, H, W):
+        loss = C.c_float()
+        _check(LIB.lrt_denoiser_loss_grad(self._h, C.c_void_p(rad_ptr), C.c_void_p(aux_ptr), C.c_void_p(tgt_ptr),
+                                          C.c_uint32(B), C.c_uint32(H), C.c_uint32(W), C.byref(loss), None, 1))
+        return loss.value
+
+    def adam_step(self, lr):
+        _check(LIB.lrt_denoiser_adam_step(self._h, C.c_float(lr)))
+
+
+def conv3x3(op, w, x=None, b=None, g=None, relu=False, device=0):
+    """One 3x3 convolution layer through lrt_conv3x3 (op 0 forward, 1 data
+    gradient, 2 weight + bias gradient); arrays NHWC float32, w [9][cin][cout]."""
+    w = np.ascontiguousarray(w, np.float32)
+    cin, cout = w.shape[1], w.shape[2]
+    ref = x if x is not None else g
+    B, H, W = ref.shape[:3]
+    x = None if x is None else np.ascontiguousarray(x, np.float32)
+    g = None if g is None else np.ascontiguousarray(g, np.float32)
+    b = None if b is None else np.ascontiguousarray(b, np.float32)
+    if op == 0:
+        out = np.zeros((B, H, W, cout), np.float32)
+    elif op == 1:
+        out = np.zeros((B, H, W, cin), np.float32)
+    else:
+        out = np.zeros(9 * cin * cout + cout, np.float32)
+    _check(LIB.lrt_conv3x3(C.c_int(device), C.c_int(op), _p(x), _p(w), _p(b), _p(g), C.c_uint32(B), C.c_uint32(H),
+                           C.c_uint32(W), C.c_uint32(cin), C.c_uint32(cout), C.c_int(int(relu)), _p(out)))
+    if op == 2:
+        return out[:9 * cin * cout].reshape(9, cin, cout), out[9 * cin * cout:]
+    return out
 
 
 def _mats_arg(materials):

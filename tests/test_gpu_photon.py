@@ -10,8 +10,8 @@ library's by an ulp now and then, so sampled directions agree to ~1e-7 and the
 difference grows along a path: >= 99% of photon records within 1e-3 relative
 and <= 1% of particles forked (some field off by more than 1e-2) for the
 global map; >= 90% and <= 3% for the caustic map, whose delta bounces off the
-displaced mesh amplify the ulps; both maps estimate the oracle maps'
-irradiance within 2%.  Eye pass on
+displaced mesh amplify the ulps; the maps estimate the oracle maps' irradiance
+within 2% (global) and 8% (caustic: independent seeds differ by 12%).  Eye pass on
 the same photon maps: the sphere room (analytic shapes) >= 99% of
 pixel-channels within 1e-4 relative; the displaced-sphere box, whose glossy
 chains and k-nearest sets amplify those ulps, >= 97% within 1e-2; both with
@@ -100,7 +100,9 @@ def test_photon_trace_vs_oracle(pbox):
         e_orc = OL.photon_estimate(OL.oracle, oph, odp, 1.0 / oshot, q, 0.1, 100, 64)
         ratio = e_dev.sum() / e_orc.sum()
         print(f"  irradiance at 5000 points: device map / oracle map = {ratio:.5f}")
-        assert abs(ratio - 1) < 0.02
+        # the caustic map is a few thousand high-power photons: the same sum
+        # from independent oracle seeds spreads by +-12%
+        assert abs(ratio - 1) < (0.02 if kind == linrt.PHOTON_SURFACE else 0.08)
 
 
 def test_estimate_large_map_vs_oracle(pbox):
