@@ -167,6 +167,7 @@ def main():
     ap.add_argument("--spp", type=int, default=SPP)
     ap.add_argument("--rays", type=int, default=0, help="ray-traversal benchmark with this many rays instead")
     ap.add_argument("--photon", action="store_true", help="photon-mapper benchmark instead")
+    ap.add_argument("--no-denoise", action="store_true", help="skip the time-to-relMSE leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by pixel/sample ranges "
@@ -240,6 +241,21 @@ def main():
         traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
 
     cpu = None if args.no_cpu else cpu_baseline(gen)
+    ttq = None
+    if not args.no_denoise:
+        # the second half of the metric: wall time to relMSE <= 1e-2 on the
+        # evaluation band of this frame, denoised low-spp frames vs plain
+        from linrt import denoise as dn
+        r = dn.time_to_quality(log=log, spp_list=(1, 2, 4))
+        ttq = {"target_relmse": r["target"], "eps": r["eps"], "band_rows": r["band_rows"],
+               "reference": "16384 spp of the same frame (band rows), committed data",
+               "denoised": [{k: (round(v, 5) if isinstance(v, float) else v) for k, v in d.items()}
+                            for d in r["denoised"]],
+               "plain_best": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in r["plain_best"].items()},
+               "denoised_best": None if r["denoised_best"] is None else
+               {k: (round(v, 5) if isinstance(v, float) else v) for k, v in r["denoised_best"].items()},
+               "speedup": None if r["speedup"] is None else round(r["speedup"], 2),
+               "speedup_4spp": None if r["speedup_4spp"] is None else round(r["speedup_4spp"], 2)}
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 2), "higher_is_better": True,
@@ -257,6 +273,7 @@ def main():
                      "traffic": traffic, "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(avg_ms, 4),
                      "launches": int(n_c)},
         "cpu_baseline": cpu,
+        "time_to_relmse": ttq,
     }
     print(json.dumps(out), flush=True)
 

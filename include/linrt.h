@@ -313,7 +313,8 @@ typedef struct lrt_generated lrt_generated;
 enum {
     LRT_GEN_BOX_ICOSPHERE = 0,   /* closed box room + icosphere L5 + ceiling light */
     LRT_GEN_DISPLACED_SPHERE = 1, /* displaced sphere, 7 material regions, ground, sky */
-    LRT_GEN_PHOTON_BOX = 2        /* box room + displaced sphere (7 regions, radius 0.5) + 0.5x0.5 light */
+    LRT_GEN_PHOTON_BOX = 2,       /* box room + displaced sphere (7 regions, radius 0.5) + 0.5x0.5 light */
+    LRT_GEN_DISPLACED_SPHERE_VARIANT = 3 /* kind 1 with seeded material colours, roughness and glass index */
 };
 typedef struct lrt_gen_params {
     int32_t kind;

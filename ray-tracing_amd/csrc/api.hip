@@ -851,9 +851,13 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         const uint32_t rb = prm->row_end > prm->row_begin ? prm->row_begin : 0;
         const uint32_t re = prm->row_end > prm->row_begin ? prm->row_end : H;
         require(re <= H, "row range outside the image");
-        const uint64_t n64 = (uint64_t)W * (re - rb);
-        require(n64 < (1ull << 31), "too many paths per pass");
-        const uint32_t n = (uint32_t)n64;
+        const uint64_t npix64 = (uint64_t)W * (re - rb);
+        require(npix64 < (1ull << 31), "too many pixels");
+        const uint32_t npix = (uint32_t)npix64;
+        // samples per pass: small frames run several samples of every pixel
+        // side by side so each launch has enough paths (up to 2^22)
+        const uint32_t slots = std::max<uint32_t>(1, std::min<uint32_t>(prm->spp, (uint32_t)((1u << 22) / npix)));
+        const uint32_t n = npix * slots;
         HIPCHECK(hipSetDevice(sc->device));
         const auto t0 = std::chrono::steady_clock::now();
         // work buffers: path state (14 floats), hits (5), shadow queue (12), two queues
@@ -871,6 +875,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         float** pf[14] = {&P.ox, &P.oy, &P.oz, &P.dx, &P.dy, &P.dz, &P.tr, &P.tg, &P.tb, &P.lr, &P.lg, &P.lb, &P.eta, &P.pdf};
         for (float** f : pf) *f = alloc_carve<float>(c, n);
         P.pixel0 = rb * W;
+        P.npix = npix;
         HitBuf Hb;
         c = sc->hits.as<char>();
         Hb.obj = alloc_carve<int32_t>(c, n);
@@ -915,9 +920,12 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         Timer& tc = sc->timers[LRT_KERNEL_CLOSEST];
         Timer& to = sc->timers[LRT_KERNEL_OCCLUDED];
         Timer& ts = sc->timers[LRT_KERNEL_SHADE];
-        for (uint32_t s = 0; s < prm->spp; ++s) {
+        for (uint32_t s = 0; s < prm->spp; s += slots) {
             const uint32_t smp = prm->spp_offset + s;
-            hipLaunchKernelGGL(k_raygen, dim3(g_all), dim3(kBlock), 0, sc->stream, P, sc->cam, W, H, rb, n, smp, key, qa, counters);
+            const uint32_t ns = std::min(slots, prm->spp - s);  // samples in this pass
+            const uint32_t np = npix * ns;
+            hipLaunchKernelGGL(k_raygen, dim3((np + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, P, sc->cam, W, H, rb,
+                               np, smp, key, qa, counters);
             uint32_t* q = qa;
             uint32_t* qn = qb;
             for (uint32_t depth = 0; depth < prm->max_depth; ++depth) {
@@ -941,7 +949,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
                 hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, sc->stream, counters, dstats);
                 std::swap(q, qn);
             }
-            hipLaunchKernelGGL(k_accumulate, dim3(g_all), dim3(kBlock), 0, sc->stream, P, n, dsum, dsq);
+            hipLaunchKernelGGL(k_accumulate, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, P, ns, dsum, dsq);
             HIPCHECK(hipGetLastError());
         }
         unsigned long long hst[4] = {0, 0, 0, 0}, work[4] = {0, 0, 0, 0};
@@ -961,7 +969,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         tc.items += hst[0];
         to.items += hst[1];
         if (stats) {
-            stats->samples = (uint64_t)n * prm->spp;
+            stats->samples = (uint64_t)npix * prm->spp;
             stats->extension_rays = hst[0];
             stats->shadow_rays = hst[1];
             stats->segments = hst[0];

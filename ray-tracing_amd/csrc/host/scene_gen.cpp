@@ -336,6 +336,23 @@ Generated* generate(const lrt_gen_params& p, const float* tables, uint32_t n_tab
         lrt_material m8 = blank();
         m8.type = LRT_BSDF_DIFFUSE;
         m8.diffuse[0] = m8.diffuse[1] = m8.diffuse[2] = 0.5f;
+        if (p.kind == LRT_GEN_DISPLACED_SPHERE_VARIANT) {
+            // colours, the conductor's and the rough glass's roughness and the
+            // glass's index drawn from the seed (the rough plastic and rough
+            // coating keep their table-bound alpha and eta)
+            Rng vr(p.seed, 4);
+            for (int c = 0; c < 3; ++c) {
+                m0.diffuse[c] = 0.05f + 0.85f * vr.next();
+                m3.diffuse[c] = 0.05f + 0.85f * vr.next();
+                m4.diffuse[c] = 0.05f + 0.85f * vr.next();
+                m5.specular_reflectance[c] = 0.2f + 0.8f * vr.next();
+                m8.diffuse[c] = 0.2f + 0.6f * vr.next();
+            }
+            m1.alpha = 0.05f + 0.35f * vr.next();
+            m1.int_ior = 1.3f + 0.5f * vr.next();
+            m2.alpha = 0.03f + 0.3f * vr.next();
+            m3.fdr_int = fresnel_diffuse_reflectance(1.0f / (m3.int_ior / m3.ext_ior));
+        }
         for (const auto& m : {m0, m1, m2, m3, m4, m5, m6, m7, m8}) g->materials.push_back(m);
     };
     if (p.kind == LRT_GEN_BOX_ICOSPHERE) {
@@ -357,7 +374,7 @@ Generated* generate(const lrt_gen_params& p, const float* tables, uint32_t n_tab
         std::memcpy(cam.up, up, 12);
         cam.fov_deg = 40.0f;
         cam.aspect = p.aspect > 0 ? p.aspect : 1.0f;
-    } else if (p.kind == LRT_GEN_DISPLACED_SPHERE) {
+    } else if (p.kind == LRT_GEN_DISPLACED_SPHERE || p.kind == LRT_GEN_DISPLACED_SPHERE_VARIANT) {
         if (!sky) throw std::runtime_error("displaced sphere scene needs a sky light");
         Rng rng(p.seed, 2);
         float sites[7][3];

@@ -19,6 +19,7 @@ struct PathState {
     float *tr, *tg, *tb, *lr, *lg, *lb;
     float *eta, *pdf;   // pdf < 0: no solid-angle pdf (camera ray or delta lobe)
     uint32_t pixel0;    // pixel index of path 0
+    uint32_t npix;      // pixels per sample slice: path p is pixel pixel0 + p % npix of sample slot p / npix
 };
 
 // hit records of the active queue, by queue slot

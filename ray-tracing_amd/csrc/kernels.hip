@@ -142,9 +142,10 @@ __global__ __launch_bounds__(kBlock) void k_raygen(PathState P, const CameraDev 
         counters[3] = 0;  // photon queries
     }
     if (i >= n) return;
-    const uint32_t x = i % W, y = row0 + i / W;
+    const uint32_t ip = i % P.npix, slot = i / P.npix;
+    const uint32_t x = ip % W, y = row0 + ip / W;
     const uint32_t pix = y * W + x;
-    const uint4 v = philox(make_uint4(pix, smp, 0u, 0u), key);
+    const uint4 v = philox(make_uint4(pix, smp + slot, 0u, 0u), key);
     const float jx = u24(v.x), jy = u24(v.y);
     const float u = ((float)x + jx) / (float)(W - 1);
     const float vv = ((float)y + jy) / (float)(H - 1);
@@ -291,9 +292,10 @@ __global__ __launch_bounds__(kBlock) void k_shade(const DevScene S, const DevLig
                     sr.front = sh.front;
                     BsdfCtx ctx;
                     // six sampler dimensions per bounce (path_integrator.h:80-83)
-                    const uint32_t pix = P.pixel0 + p, d0 = 2u + 6u * prm.depth;
-                    const uint4 A = philox(make_uint4(pix, prm.sample, d0 >> 2, 0u), prm.key);
-                    const uint4 B = philox(make_uint4(pix, prm.sample, (d0 >> 2) + 1u, 0u), prm.key);
+                    const uint32_t pix = P.pixel0 + p % P.npix, smp = prm.sample + p / P.npix;
+                    const uint32_t d0 = 2u + 6u * prm.depth;
+                    const uint4 A = philox(make_uint4(pix, smp, d0 >> 2, 0u), prm.key);
+                    const uint4 B = philox(make_uint4(pix, smp, (d0 >> 2) + 1u, 0u), prm.key);
                     float u6[6];
 #pragma unroll
                     for (int k = 0; k < 6; ++k) {
@@ -462,16 +464,28 @@ __global__ void k_advance(uint32_t* counters, unsigned long long* stats) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_accumulate(const PathState P, uint32_t n, double* __restrict__ sum,
+// per pixel, the pass's samples in sample order
+__global__ __launch_bounds__(kBlock) void k_accumulate(const PathState P, uint32_t slots, double* __restrict__ sum,
                                                        double* __restrict__ sumsq) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const size_t pix = (size_t)P.pixel0 + i;
-    const float l[3] = {P.lr[i], P.lg[i], P.lb[i]};
+    const uint32_t ip = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= P.npix) return;
+    const size_t pix = (size_t)P.pixel0 + ip;
+    double s[3] = {sum[3 * pix], sum[3 * pix + 1], sum[3 * pix + 2]};
+    double q[3] = {0.0, 0.0, 0.0};
+    if (sumsq) { q[0] = sumsq[3 * pix]; q[1] = sumsq[3 * pix + 1]; q[2] = sumsq[3 * pix + 2]; }
+    for (uint32_t j = 0; j < slots; ++j) {
+        const size_t i = (size_t)j * P.npix + ip;
+        const float l[3] = {P.lr[i], P.lg[i], P.lb[i]};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            s[c] += (double)l[c];
+            q[c] += (double)l[c] * (double)l[c];
+        }
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-        sum[3 * pix + c] += (double)l[c];
-        if (sumsq) sumsq[3 * pix + c] += (double)l[c] * (double)l[c];
+        sum[3 * pix + c] = s[c];
+        if (sumsq) sumsq[3 * pix + c] = q[c];
     }
 }
 

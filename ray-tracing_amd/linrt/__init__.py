@@ -40,7 +40,7 @@ MODE_RADIANCE, MODE_IMPORTANCE = 0, 1
  BSDF_ROUGH_PLASTIC, BSDF_COATING, BSDF_ROUGH_COATING) = range(9)
 SHAPE_MESH, SHAPE_RECTANGLE, SHAPE_SPHERE = 0, 1, 2
 LIGHT_AREA, LIGHT_SKY = 0, 1
-GEN_BOX_ICOSPHERE, GEN_DISPLACED_SPHERE, GEN_PHOTON_BOX = 0, 1, 2
+GEN_BOX_ICOSPHERE, GEN_DISPLACED_SPHERE, GEN_PHOTON_BOX, GEN_DISPLACED_SPHERE_VARIANT = 0, 1, 2, 3
 PHOTON_SURFACE, PHOTON_CAUSTIC = 0, 1
 KERNEL_CLOSEST, KERNEL_OCCLUDED, KERNEL_SHADE = 0, 1, 2
 
@@ -127,6 +127,14 @@ class LinrtError(RuntimeError):
 def _load():
     if not LIB_PATH.exists():
         raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    # torch ships its own copy of the HIP runtime, found by a different file
+    # name than the system copy this library links against: loading torch
+    # first makes the library bind to that same copy, so both share one
+    # runtime in a process that uses torch for device buffers
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(str(LIB_PATH))
     lib.lrt_last_error.restype = C.c_char_p
     lib.lrt_generated_desc.restype = C.POINTER(SceneDesc)
@@ -269,6 +277,32 @@ def displaced_sphere(seed=1, segments=1024, rings=512, aspect=16.0 / 9.0):
                      tables=tables, sky=load_sky())
 
 
+def displaced_sphere_variant(seed=1, segments=1024, rings=512, aspect=16.0 / 9.0, sky=None):
+    """The displaced-sphere scene with seeded materials (and mesh) and a given
+    sky (default: the one of the 1080p workload): denoiser training scenes."""
+    tables, _ = load_tables()
+    return Generated(GEN_DISPLACED_SPHERE_VARIANT, seed=seed, segments=segments, rings=rings, aspect=aspect,
+                     tables=tables, sky=sky if sky is not None else load_sky())
+
+
+def load_sky_set(name="skies_train.json"):
+    """Cooked Hosek skies of several sun positions and turbidities."""
+    out = []
+    for d in json.loads((DATA_DIR / name).read_text())["skies"]:
+        L = LightDesc()
+        L.kind = LIGHT_SKY
+        L.object = -1
+        for i in range(3):
+            L.sun_dir[i] = d["sun_dir"][i]
+        L.turbidity, L.ground_albedo, L.scale = d["turbidity"], d["ground_albedo"], d["scale"]
+        for c in range(3):
+            for k in range(9):
+                L.config[c][k] = d["config"][c][k]
+            L.channel_radiance[c] = d["channel_radiance"][c]
+        out.append(L)
+    return out
+
+
 def photon_box(seed=1, segments=512, rings=256, aspect=1.0):
     """Box room around a 262,144-triangle displaced sphere (7 material
     regions, radius 0.5) under a 0.5 x 0.5 ceiling light (photon workload)."""
@@ -280,6 +314,7 @@ class Scene:
     """Scene uploaded to one GPU (lrt_scene_create)."""
 
     def __init__(self, desc_ptr, device=0):
+        self._h = None
         h = C.c_void_p()
         _check(LIB.lrt_scene_create(C.c_int(device), desc_ptr, C.byref(h)))
         self._h = h
@@ -369,8 +404,9 @@ class Scene:
         _check(LIB.lrt_render_aov(self._h, C.byref(prm), _p(aov), 0))
         return aov
 
-    def render_aov_device(self, width, height, spp, aov_ptr, seed=1, spp_offset=0):
-        prm = RenderParams(width, height, spp, spp_offset, 1, 0, seed, 0, 0, 0)
+    def render_aov_device(self, width, height, spp, aov_ptr, seed=1, spp_offset=0, rows=None):
+        rb, re = rows if rows is not None else (0, 0)
+        prm = RenderParams(width, height, spp, spp_offset, 1, 0, seed, rb, re, 0)
         _check(LIB.lrt_render_aov(self._h, C.byref(prm), C.c_void_p(aov_ptr), 1))
 
     # ---- photon mapping (photon_integrator.h:15-123) ----
@@ -455,6 +491,7 @@ class Denoiser:
     arrays, loss + gradient, Adam steps, parameter get/set."""
 
     def __init__(self, device=0, seed=1):
+        self._h = None
         h = C.c_void_p()
         _check(LIB.lrt_denoiser_create(C.c_int(device), C.c_uint64(seed), C.byref(h)))
         self._h = h

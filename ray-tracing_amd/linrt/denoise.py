@@ -1,0 +1,271 @@
+"""Training, evaluation and the time-to-quality measurement of the learned
+denoiser (lrt_denoiser_*), all on one GPU.
+
+Data: frames of seeded displaced-sphere scenes (random materials, mesh and
+one of eight cooked skies) rendered by the library's own path tracer: a few
+noisy low-spp frames with their first-hit features per scene and one
+high-spp frame as the target.  Frames stay in device memory (torch tensors
+are used only as allocations); every network step runs the library's BF16
+matrix-core kernels through the C ABI.
+
+  python -m linrt.denoise train [--scenes N] [--steps K] [--out path]
+  python -m linrt.denoise reference [--spp S] [--out path]
+"""
+from __future__ import annotations
+
+import argparse
+import math
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+import linrt
+
+DATA = Path(__file__).resolve().parent.parent / "data"
+WEIGHTS = DATA / "denoiser_weights.npz"
+REFERENCE = DATA / "reference_1080p_band.npz"
+MAX_DEPTH, RR_DEPTH = 16, 5
+# the evaluation band of the 1920x1080 frame (rows bottom-up) and the margin
+# the network sees around it
+BAND = (476, 604)
+MARGIN = 64
+EPS = 1e-2  # relMSE = mean((x - ref)^2 / (ref^2 + EPS))
+
+
+def _torch():
+    """torch (device allocations only), with its GPU runtime initialised: it
+    must start before the library's first HIP call in the process."""
+    import torch
+    if not torch.cuda.is_initialized():
+        torch.cuda.init()
+    return torch
+
+
+def radiance(scene, W, H, spp, seed, offset, torch_dev="cuda:0", rows=None):
+    """Mean radiance of spp samples per pixel, float32 [H, W, 3] on the device
+    (rows outside `rows` stay 0)."""
+    torch = _torch()
+    acc = torch.zeros((H, W, 3), dtype=torch.float64, device=torch_dev)
+    rb, re = rows if rows is not None else (0, 0)
+    prm = linrt.RenderParams(W, H, spp, offset, MAX_DEPTH, RR_DEPTH, seed, rb, re, 0)
+    st = linrt.RenderStats()
+    linrt._check(linrt.LIB.lrt_render(scene._h, linrt.C.byref(prm), linrt.C.c_void_p(acc.data_ptr()), None, 1,
+                                      linrt.C.byref(st)))
+    scene.sync()
+    return (acc / spp).float()
+
+
+def features(scene, W, H, spp, seed, offset, torch_dev="cuda:0", rows=None):
+    torch = _torch()
+    aov = torch.zeros((H, W, 7), dtype=torch.float32, device=torch_dev)
+    scene.render_aov_device(W, H, spp, aov.data_ptr(), seed=seed, spp_offset=offset, rows=rows)
+    return aov
+
+
+def relmse(x, ref, eps=EPS):
+    """Relative MSE (per pixel-channel, against ref^2 + eps), float64, over
+    the pixels whose reference is finite; a non-finite pixel of x counts as
+    black, as the reference's image writer stores it (image_write.cpp:11-13)."""
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    ok = np.isfinite(ref).all(axis=-1)
+    x = np.where(np.isfinite(x), x, 0.0)
+    return float(np.mean(((x - ref) ** 2 / (ref * ref + eps))[ok]))
+
+
+def make_dataset(n_scenes=160, W=1920, H=1080, band=128, spp_in=(2, 4, 8), spp_tgt=512, seed0=100, log=print):
+    """Per scene (seeded materials, mesh and sky): a band of `band` rows of the
+    W x H frame at a seeded height, rendered once per entry of spp_in (noisy
+    input + features) and once at spp_tgt (target), so the network sees the
+    pixel scale of full-resolution frames.  Device tensors rad [N*len(spp_in),
+    band, W, 3], aux [.., 7], tgt [N, band, W, 3] and each input's target index."""
+    torch = _torch()
+    skies = linrt.load_sky_set()
+    rad, aux, tgt, idx = [], [], [], []
+    t0 = time.perf_counter()
+    rng = np.random.default_rng(seed0)
+    for s in range(n_scenes):
+        seed = seed0 + s
+        gen = linrt.displaced_sphere_variant(seed=seed, aspect=W / H, sky=skies[s % len(skies)])
+        sc = linrt.Scene(gen.desc_ptr)
+        r0 = int(rng.integers(0, (H - band) // 4 + 1)) * 4
+        rows = (r0, r0 + band)
+        off = 0
+        for k, spp in enumerate(spp_in):
+            rad.append(radiance(sc, W, H, spp, seed, off, rows=rows)[r0:r0 + band].clone())
+            aux.append(features(sc, W, H, spp, seed, off, rows=rows)[r0:r0 + band].clone())
+            idx.append(s)
+            off += spp
+        tgt.append(radiance(sc, W, H, spp_tgt, seed, 1 << 20, rows=rows)[r0:r0 + band].clone())
+        sc.close()
+        gen.close()
+        if (s + 1) % 16 == 0:
+            log(f"  dataset: {s + 1}/{n_scenes} scenes, {time.perf_counter() - t0:.1f}s")
+    return torch.stack(rad), torch.stack(aux), torch.stack(tgt), torch.tensor(idx, device="cuda:0")
+
+
+def train(den, data, steps=1500, batch=8, crop=128, lr=2e-3, seed=0, log=print):
+    """Adam on random crops (with flips) of the dataset; cosine learning rate."""
+    torch = _torch()
+    rad, aux, tgt, idx = data
+    N, H, W, _ = rad.shape
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    t0 = time.perf_counter()
+    hist = []
+    for it in range(steps):
+        f = torch.randint(0, N, (batch,), generator=g)
+        ys = torch.randint(0, H - crop + 1, (batch,), generator=g)
+        xs = torch.randint(0, W - crop + 1, (batch,), generator=g)
+        r = torch.stack([rad[f[i], ys[i]:ys[i] + crop, xs[i]:xs[i] + crop] for i in range(batch)])
+        a = torch.stack([aux[f[i], ys[i]:ys[i] + crop, xs[i]:xs[i] + crop] for i in range(batch)])
+        t = torch.stack([tgt[idx[f[i]], ys[i]:ys[i] + crop, xs[i]:xs[i] + crop] for i in range(batch)])
+        if it % 2:
+            r, a, t = r.flip(2), a.flip(2), t.flip(2)
+            a[..., 3] = -a[..., 3]  # mirrored normals
+        r, a, t = r.contiguous(), a.contiguous(), t.contiguous()
+        loss = den.loss_grad_device(r.data_ptr(), a.data_ptr(), t.data_ptr(), batch, crop, crop)
+        den.adam_step(lr * 0.5 * (1 + math.cos(math.pi * it / steps)))
+        hist.append(loss)
+        if (it + 1) % 250 == 0:
+            log(f"  step {it + 1}/{steps}: loss {np.mean(hist[-250:]):.4f} ({time.perf_counter() - t0:.1f}s)")
+    torch.cuda.synchronize()
+    return hist
+
+
+def save(den, path=WEIGHTS):
+    np.savez_compressed(path, params=den.params.astype(np.float16), layers=np.array(linrt_layers()))
+
+
+def load(den, path=WEIGHTS):
+    p = np.load(path)["params"].astype(np.float32)
+    den.params = p
+
+
+def linrt_layers():
+    return [10, 32, 32, 32, 32, 64, 64, 64, 64, 128, 128, 128, 192, 64, 64, 64, 96, 32, 32, 3]
+
+
+def c1_scene():
+    gen = linrt.displaced_sphere(seed=1)
+    return gen, linrt.Scene(gen.desc_ptr)
+
+
+def make_reference(spp=16384, out=REFERENCE, chunk=1024, log=print):
+    """The band of the 1080p frame at spp samples (float64 sum on the device),
+    stored as float32 [rows, 1920, 3]."""
+    torch = _torch()
+    gen, sc = c1_scene()
+    W, H = 1920, 1080
+    acc = torch.zeros((H, W, 3), dtype=torch.float64, device="cuda:0")
+    t0 = time.perf_counter()
+    for off in range(0, spp, chunk):
+        prm = linrt.RenderParams(W, H, chunk, (1 << 24) + off, MAX_DEPTH, RR_DEPTH, 1, BAND[0], BAND[1], 0)
+        st = linrt.RenderStats()
+        linrt._check(linrt.LIB.lrt_render(sc._h, linrt.C.byref(prm), linrt.C.c_void_p(acc.data_ptr()), None, 1,
+                                          linrt.C.byref(st)))
+        sc.sync()
+        log(f"  reference: {off + chunk}/{spp} spp, {time.perf_counter() - t0:.1f}s")
+    band = (acc[BAND[0]:BAND[1]] / spp).float().cpu().numpy()
+    np.savez_compressed(out, band=band.astype(np.float16), rows=np.array(BAND), spp=spp)
+    return band
+
+
+def time_to_quality(den=None, ref=None, target=0.01, log=print, spp_list=(1, 2, 4, 8, 16), weights=WEIGHTS):
+    """Time for the plain path tracer and for denoised frames to reach
+    relMSE <= target on the band of the 1080p frame.  Returns a dict."""
+    torch = _torch()
+    if ref is None:
+        ref = np.load(REFERENCE)["band"].astype(np.float32)
+    if den is None:
+        den = linrt.Denoiser()
+        load(den, weights)
+    gen, sc = c1_scene()
+    W, H = 1920, 1080
+    out = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0")
+    # warm-up of every kernel
+    r = radiance(sc, W, H, 1, 1, 0)
+    a = features(sc, W, H, 1, 1, 0)
+    den.forward_device(r.data_ptr(), a.data_ptr(), 1, H, W, out.data_ptr())
+    torch.cuda.synchronize()
+    res = {"band_rows": list(BAND), "eps": EPS, "target": target, "denoised": [], "plain": []}
+    best = None
+    for spp in spp_list:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = radiance(sc, W, H, spp, 1, 0)
+        a = features(sc, W, H, spp, 1, 0)
+        den.forward_device(r.data_ptr(), a.data_ptr(), 1, H, W, out.data_ptr())
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        e = relmse(out[BAND[0]:BAND[1]].cpu().numpy(), ref)
+        e0 = relmse(r[BAND[0]:BAND[1]].cpu().numpy(), ref)
+        res["denoised"].append({"spp": spp, "seconds": dt, "relmse": e, "relmse_input": e0})
+        log(f"  denoised {spp} spp: {dt * 1e3:.1f} ms, relMSE {e:.5f} (input {e0:.4f})")
+        if e <= target and best is None:
+            best = (spp, dt, e)
+    res["denoised_best"] = None if best is None else {"spp": best[0], "seconds": best[1], "relmse": best[2]}
+    # plain: relMSE ~ c / spp; measure at 64 spp, render the predicted spp for
+    # the target and step up by 10% until it is reached
+    t0 = time.perf_counter()
+    r = radiance(sc, W, H, 64, 1, 0)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    e64 = relmse(r[BAND[0]:BAND[1]].cpu().numpy(), ref)
+    res["plain"].append({"spp": 64, "seconds": t64, "relmse": e64})
+    need = max(2, int(math.ceil(64 * e64 / target / 2)) * 2)
+    for _ in range(6):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = radiance(sc, W, H, need, 1, 0)
+        torch.cuda.synchronize()
+        tn = time.perf_counter() - t0
+        en = relmse(r[BAND[0]:BAND[1]].cpu().numpy(), ref)
+        res["plain"].append({"spp": need, "seconds": tn, "relmse": en})
+        log(f"  plain {need} spp: {tn:.2f} s, relMSE {en:.5f}")
+        if en <= target:
+            break
+        need = int(math.ceil(need * 1.1 / 2)) * 2
+    res["plain_best"] = {"spp": need, "seconds": tn, "relmse": en}
+    res["speedup"] = tn / best[1] if (best is not None and en <= target) else None
+    four = [d for d in res["denoised"] if d["spp"] == 4]
+    res["speedup_4spp"] = tn / four[0]["seconds"] if (four and four[0]["relmse"] <= target and en <= target) else None
+    sc.close()
+    gen.close()
+    return res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cmd", choices=["train", "reference", "evaluate"])
+    ap.add_argument("--scenes", type=int, default=160)
+    ap.add_argument("--steps", type=int, default=12000)
+    ap.add_argument("--spp-target", type=int, default=512)
+    ap.add_argument("--spp", type=int, default=16384)
+    ap.add_argument("--out", type=str, default=None)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--weights", type=str, default=None)
+    args = ap.parse_args(argv)
+    log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
+    _torch()
+    if args.cmd == "reference":
+        make_reference(args.spp, Path(args.out) if args.out else REFERENCE, log=log)
+        return
+    if args.cmd == "evaluate":
+        import json
+        print(json.dumps(time_to_quality(log=log, weights=Path(args.weights) if args.weights else WEIGHTS)))
+        return
+    den = linrt.Denoiser(seed=args.seed)
+    t0 = time.perf_counter()
+    data = make_dataset(args.scenes, spp_tgt=args.spp_target, log=log)
+    log(f"dataset {time.perf_counter() - t0:.1f}s")
+    t0 = time.perf_counter()
+    train(den, data, steps=args.steps, batch=args.batch, log=log)
+    log(f"training {time.perf_counter() - t0:.1f}s")
+    save(den, Path(args.out) if args.out else WEIGHTS)
+
+
+if __name__ == "__main__":
+    main()

@@ -81,6 +81,27 @@ def make_sky():
     (DATA / "sky_t3_a0.3_e30.json").write_text(json.dumps(doc, indent=1))
 
 
+def make_sky_set():
+    """Skies for the denoiser's training scenes: sun elevations 15..60 degrees
+    at several azimuths, turbidity 2 and 4, ground albedo 0.3, cooked by the
+    reference's Hosek-Wilkie implementation."""
+    skies = []
+    for i, (el, tb) in enumerate([(15, 2), (25, 4), (35, 2), (45, 4), (60, 2), (20, 4), (40, 2), (55, 4)]):
+        az = np.radians(45 + 40 * i)
+        e = np.radians(el)
+        sun = (C.c_float * 3)(float(np.cos(e) * np.cos(az)), float(np.sin(e)), float(np.cos(e) * np.sin(az)))
+        elev = C.c_double()
+        OL.REF.ref_sky_elevation(sun, C.byref(elev))
+        cfg = np.zeros(27, np.float64)
+        rad = np.zeros(3, np.float64)
+        OL.REF.ref_hosek_cook(C.c_float(tb), C.c_float(0.3), elev, OL._p(cfg, C.c_double), OL._p(rad, C.c_double))
+        skies.append({"turbidity": float(tb), "ground_albedo": 0.3, "elevation_rad": elev.value,
+                      "sun_dir": [float(v) for v in sun], "scale": 0.05, "config": cfg.reshape(3, 9).tolist(),
+                      "channel_radiance": rad.tolist()})
+    doc = {"source": "ArHosekSkyModel v1.4a RGB, cooked by the reference", "skies": skies}
+    (DATA / "skies_train.json").write_text(json.dumps(doc, indent=1))
+
+
 def make_bsdf_fixture(tables):
     """Reference eval/pdf (both measures) and sample outputs on seeded tuples."""
     from common import bsdf_tuples, material_zoo
@@ -178,12 +199,16 @@ def make_eye_fixture():
 def main():
     DATA.mkdir(parents=True, exist_ok=True)
     assert OL.REF is not None, "oracle/_ref/libref.so is not built (make -C oracle ref)"
+    if sys.argv[1:] == ["skies"]:
+        make_sky_set()
+        return
     if sys.argv[1:] == ["photons"]:
         make_photon_fixture()
         make_eye_fixture()
         return
     tables = make_tables()
     make_sky()
+    make_sky_set()
     make_bsdf_fixture(tables)
     make_hits_fixture()
     make_misc_fixture(tables)
