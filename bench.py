@@ -168,6 +168,8 @@ def main():
     ap.add_argument("--rays", type=int, default=0, help="ray-traversal benchmark with this many rays instead")
     ap.add_argument("--photon", action="store_true", help="photon-mapper benchmark instead")
     ap.add_argument("--no-denoise", action="store_true", help="skip the time-to-relMSE leg")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="only copy this many MiB device-to-device 5 times (HBM counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by pixel/sample ranges "
@@ -177,6 +179,16 @@ def main():
     import torch
     import linrt
 
+    if args.copy_calibration:
+        n = args.copy_calibration * 2**20 // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_mib": args.copy_calibration, "launches": 5, "bytes_read_per_launch": n * 4,
+                          "bytes_written_per_launch": n * 4}))
+        return
     if args.rays:
         return ray_bench(args, torch, linrt)
     if args.photon:
