@@ -340,6 +340,33 @@ int lrt_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
  * pixel, rows bottom-up like the radiance.  device_ptrs: aov is device memory. */
 int lrt_render_aov(lrt_scene* scene, const lrt_render_params* params, float* aov, int device_ptrs);
 
+/* ---- differentiable rendering (no counterpart in the reference) ---------
+ * The path tracer of lrt_render as one kernel thread per pixel (identical
+ * radiance per sample), with parameters theta: an albedo texture on one
+ * diffuse material (spherical mapping about the origin, nearest texel), the
+ * GGX alpha of up to two materials and the interior IOR of one.  theta holds
+ * tex_height x tex_width RGB texels (row-major), then alpha[0], alpha[1] and
+ * the IOR (3 slots always; unused ones are ignored).  lrt_diff_adjoint
+ * returns d(sum(d_image * image)) / d theta by path replay (sampling, MIS
+ * and roulette detached); images are width x height x 3 floats, bottom-up. */
+typedef struct lrt_diff_params {
+    int32_t tex_material;          /* diffuse material textured by theta, -1: none */
+    uint32_t tex_width, tex_height;
+    int32_t alpha_material[2];     /* conductor / rough dielectric whose alpha is a parameter, -1: none */
+    int32_t ior_material;          /* dielectric / rough dielectric whose interior IOR is a parameter, -1: none */
+} lrt_diff_params;
+int lrt_diff_param_count(const lrt_diff_params* dp, uint32_t* n);
+int lrt_diff_render(lrt_scene* scene, const lrt_diff_params* dp, const float* theta, const lrt_render_params* params,
+                    float* image, int device_ptrs);
+int lrt_diff_adjoint(lrt_scene* scene, const lrt_diff_params* dp, const float* theta, const lrt_render_params* params,
+                     const float* d_image, float* grad, int device_ptrs);
+/* the estimator lrt_diff_adjoint differentiates: BSDF values and texture at
+ * theta, directions, pdfs and MIS weights sampled with theta_sampling's
+ * scalars; its derivative in theta at theta == theta_sampling is the
+ * adjoint's gradient for the same samples (host arrays; for gradient checks) */
+int lrt_diff_render_held(lrt_scene* scene, const lrt_diff_params* dp, const float* theta, const float* theta_sampling,
+                         const lrt_render_params* params, float* image);
+
 /* ---- learned denoiser (no counterpart in the reference) -------------------
  * A 3-level U-Net (32 / 64 / 128 channels, 3x3 convolutions) over radiance +
  * first-hit features, run and trained with the library's BF16 matrix-core

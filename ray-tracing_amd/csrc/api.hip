@@ -12,6 +12,7 @@
 #include "host/host.hpp"
 #include "kernels.hip"
 #include "eye.hip"
+#include "device/nn.h"
 
 namespace lrt {
 
@@ -255,6 +256,8 @@ struct lrt_scene {
     EmitCdf emit;
     float bounding_radius = 0.0f;
     bool has_delta = false;
+    DevBuf diff_mats, diff_tex, diff_theta, diff_img, diff_dimg, diff_grad, diff_part;
+    std::vector<lrt_material> host_mats;
     int max_spec_components = 0;
     Timer timers[LRT_KERNEL_COUNT_];
     ~lrt_scene() {
@@ -464,6 +467,7 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
         sc.bounding_radius = d.n_objects ? std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]) : 0.0f;
         sc.has_delta = delta;
         sc.max_spec_components = d.n_materials ? host_max_specular_components(mats) : 0;
+        sc.host_mats = mats;
     }
     const CameraBasis cb = camera_basis(d.camera);
     std::memcpy(&sc.cam, &cb, sizeof(sc.cam));
@@ -1013,6 +1017,174 @@ int lrt_render_aov(lrt_scene* sc, const lrt_render_params* prm, float* aov, int 
                            prm->spp, prm->spp_offset, key, d);
         HIPCHECK(hipGetLastError());
         if (!device_ptrs) HIPCHECK(hipMemcpyAsync(aov, d, bytes, hipMemcpyDeviceToHost, sc->stream));
+        HIPCHECK(hipStreamSynchronize(sc->stream));
+    });
+}
+
+// ---- differentiable rendering --------------------------------------------------------
+static uint32_t diff_count(const lrt_diff_params& dp) { return dp.tex_width * dp.tex_height * 3 + 3; }
+
+// samples per thread and chunk count so a launch has ~2M threads
+static void diff_chunks(const lrt_render_params& prm, uint32_t& chunk, uint32_t& nchunks) {
+    const uint32_t npix = prm.width * prm.height;
+    const uint32_t want = std::max<uint32_t>(1, std::min<uint32_t>(prm.spp, (1u << 21) / std::max<uint32_t>(npix, 1)));
+    chunk = (prm.spp + want - 1) / want;
+    nchunks = (prm.spp + chunk - 1) / chunk;
+}
+
+static void diff_launch_render(lrt_scene& sc, const Mats& M, const DiffParams& D, const lrt_render_params& prm,
+                               float* img) {
+    uint32_t chunk, nchunks;
+    diff_chunks(prm, chunk, nchunks);
+    const uint32_t npix = prm.width * prm.height;
+    sc.diff_part.alloc((size_t)npix * nchunks * 12);
+    hipLaunchKernelGGL(k_diff_render, dim3(grid_for(sc, (uint64_t)npix * nchunks, 16)), dim3(kBlock), 0, sc.stream, sc.S,
+                       sc.Ls, M, D, sc.cam, chunk, nchunks, sc.diff_part.as<float>());
+    hipLaunchKernelGGL(k_diff_resolve, dim3(grid_for(sc, npix, 8)), dim3(kBlock), 0, sc.stream, sc.diff_part.as<float>(),
+                       npix, nchunks, prm.spp, img);
+    HIPCHECK(hipGetLastError());
+}
+
+// validates dp, applies theta's scalars to a copy of the materials and
+// uploads it; returns the kernel parameters (texture pointer set by caller)
+static DiffParams diff_setup(lrt_scene& sc, const lrt_diff_params& dp, const float* theta_host_scalars,
+                             const lrt_render_params& prm) {
+    const int nm = (int)sc.host_mats.size();
+    require(prm.width >= 2 && prm.height >= 2 && prm.spp >= 1 && prm.max_depth >= 1, "bad render parameters");
+    require(dp.tex_material < 0 || (dp.tex_material < nm && sc.host_mats[dp.tex_material].type == LRT_BSDF_DIFFUSE),
+            "tex_material must be a diffuse material");
+    require(dp.tex_material < 0 || (dp.tex_width >= 1 && dp.tex_height >= 1), "bad texture size");
+    for (int k = 0; k < 2; ++k) {
+        const int m = dp.alpha_material[k];
+        require(m < 0 || (m < nm && (sc.host_mats[m].type == LRT_BSDF_CONDUCTOR || sc.host_mats[m].type == LRT_BSDF_ROUGH_DIELECTRIC)),
+                "alpha_material must be a conductor or a rough dielectric");
+    }
+    require(dp.ior_material < 0 || (dp.ior_material < nm && (sc.host_mats[dp.ior_material].type == LRT_BSDF_DIELECTRIC ||
+                                                             sc.host_mats[dp.ior_material].type == LRT_BSDF_ROUGH_DIELECTRIC)),
+            "ior_material must be a (rough) dielectric");
+    std::vector<lrt_material> mats = sc.host_mats;
+    for (int k = 0; k < 2; ++k)
+        if (dp.alpha_material[k] >= 0) mats[dp.alpha_material[k]].alpha = theta_host_scalars[k];
+    if (dp.ior_material >= 0) mats[dp.ior_material].int_ior = theta_host_scalars[2];
+    upload(sc.diff_mats, mats.data(), mats.size() * sizeof(lrt_material), sc.stream);
+    DiffParams D;
+    D.tex = nullptr;
+    D.eval_mats = nullptr;
+    D.tex_w = (int)dp.tex_width;
+    D.tex_h = (int)dp.tex_height;
+    D.tex_mat = dp.tex_material;
+    D.alpha_mat[0] = dp.alpha_material[0];
+    D.alpha_mat[1] = dp.alpha_material[1];
+    D.ior_mat = dp.ior_material;
+    D.max_depth = prm.max_depth;
+    D.rr_depth = prm.rr_depth;
+    D.W = prm.width;
+    D.H = prm.height;
+    D.spp = prm.spp;
+    D.spp0 = prm.spp_offset;
+    D.key = make_uint2((uint32_t)prm.seed, (uint32_t)(prm.seed >> 32));
+    return D;
+}
+
+// theta (host or device) -> device copy and host scalars
+static const float* diff_theta(lrt_scene& sc, const lrt_diff_params& dp, const float* theta, int device_ptrs, float s[3]) {
+    const uint32_t n = diff_count(dp);
+    if (device_ptrs) {
+        HIPCHECK(hipMemcpyAsync(s, theta + (n - 3), 12, hipMemcpyDeviceToHost, sc.stream));
+        HIPCHECK(hipStreamSynchronize(sc.stream));
+        return theta;
+    }
+    s[0] = theta[n - 3];
+    s[1] = theta[n - 2];
+    s[2] = theta[n - 1];
+    upload(sc.diff_theta, theta, (size_t)n * 4, sc.stream);
+    return sc.diff_theta.as<float>();
+}
+
+int lrt_diff_param_count(const lrt_diff_params* dp, uint32_t* n) {
+    return guarded([&] {
+        require(dp && n, "null pointer");
+        *n = diff_count(*dp);
+    });
+}
+
+int lrt_diff_render(lrt_scene* sc, const lrt_diff_params* dp, const float* theta, const lrt_render_params* prm,
+                    float* image, int device_ptrs) {
+    return guarded([&] {
+        require(sc && dp && theta && prm && image, "null pointer");
+        HIPCHECK(hipSetDevice(sc->device));
+        float sv[3];
+        const float* dth = diff_theta(*sc, *dp, theta, device_ptrs, sv);
+        DiffParams D = diff_setup(*sc, *dp, sv, *prm);
+        D.tex = dp->tex_material >= 0 ? dth : nullptr;
+        const Mats M{sc->diff_mats.as<lrt_material>(), sc->M.tables};
+        const size_t bytes = (size_t)prm->width * prm->height * 12;
+        float* img = image;
+        if (!device_ptrs) {
+            sc->diff_img.alloc(bytes);
+            img = sc->diff_img.as<float>();
+        }
+        diff_launch_render(*sc, M, D, *prm, img);
+        if (!device_ptrs) HIPCHECK(hipMemcpyAsync(image, img, bytes, hipMemcpyDeviceToHost, sc->stream));
+        HIPCHECK(hipStreamSynchronize(sc->stream));
+    });
+}
+
+int lrt_diff_render_held(lrt_scene* sc, const lrt_diff_params* dp, const float* theta, const float* theta_sampling,
+                         const lrt_render_params* prm, float* image) {
+    return guarded([&] {
+        require(sc && dp && theta && theta_sampling && prm && image, "null pointer");
+        HIPCHECK(hipSetDevice(sc->device));
+        const uint32_t np = diff_count(*dp);
+        // BSDF values at theta (texture and scalars), sampling at theta_sampling's scalars
+        float se[3] = {theta[np - 3], theta[np - 2], theta[np - 1]};
+        diff_setup(*sc, *dp, se, *prm);  // materials at theta -> copied aside for the BSDF values
+        DevBuf eval;
+        eval.alloc(sc->host_mats.size() * sizeof(lrt_material));
+        HIPCHECK(hipMemcpyAsync(eval.p, sc->diff_mats.p, sc->host_mats.size() * sizeof(lrt_material), hipMemcpyDeviceToDevice,
+                                sc->stream));
+        float ss[3] = {theta_sampling[np - 3], theta_sampling[np - 2], theta_sampling[np - 1]};
+        DiffParams D = diff_setup(*sc, *dp, ss, *prm);
+        upload(sc->diff_theta, theta, (size_t)np * 4, sc->stream);
+        D.tex = dp->tex_material >= 0 ? sc->diff_theta.as<float>() : nullptr;
+        D.eval_mats = eval.as<lrt_material>();
+        const Mats M{sc->diff_mats.as<lrt_material>(), sc->M.tables};
+        const size_t bytes = (size_t)prm->width * prm->height * 12;
+        sc->diff_img.alloc(bytes);
+        diff_launch_render(*sc, M, D, *prm, sc->diff_img.as<float>());
+        HIPCHECK(hipMemcpyAsync(image, sc->diff_img.p, bytes, hipMemcpyDeviceToHost, sc->stream));
+        HIPCHECK(hipStreamSynchronize(sc->stream));
+    });
+}
+
+int lrt_diff_adjoint(lrt_scene* sc, const lrt_diff_params* dp, const float* theta, const lrt_render_params* prm,
+                     const float* d_image, float* grad, int device_ptrs) {
+    return guarded([&] {
+        require(sc && dp && theta && prm && d_image && grad, "null pointer");
+        HIPCHECK(hipSetDevice(sc->device));
+        float sv[3];
+        const float* dth = diff_theta(*sc, *dp, theta, device_ptrs, sv);
+        DiffParams D = diff_setup(*sc, *dp, sv, *prm);
+        D.tex = dp->tex_material >= 0 ? dth : nullptr;
+        const Mats M{sc->diff_mats.as<lrt_material>(), sc->M.tables};
+        const uint32_t np = diff_count(*dp);
+        const size_t bytes = (size_t)prm->width * prm->height * 12;
+        const float* dimg = d_image;
+        float* g = grad;
+        if (!device_ptrs) {
+            upload(sc->diff_dimg, d_image, bytes, sc->stream);
+            dimg = sc->diff_dimg.as<float>();
+            sc->diff_grad.alloc((size_t)np * 4);
+            g = sc->diff_grad.as<float>();
+        }
+        hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(*sc, np, 4)), dim3(kBlock), 0, sc->stream, g, (long long)np);
+        uint32_t chunk, nchunks;
+        diff_chunks(*prm, chunk, nchunks);
+        const uint64_t n = (uint64_t)prm->width * prm->height * nchunks;
+        hipLaunchKernelGGL(k_diff_adjoint, dim3(grid_for(*sc, n, 16)), dim3(kBlock), 0, sc->stream, sc->S, sc->Ls, M, D,
+                           sc->cam, chunk, nchunks, dimg, g, g + (np - 3));
+        HIPCHECK(hipGetLastError());
+        if (!device_ptrs) HIPCHECK(hipMemcpyAsync(grad, g, (size_t)np * 4, hipMemcpyDeviceToHost, sc->stream));
         HIPCHECK(hipStreamSynchronize(sc->stream));
     });
 }

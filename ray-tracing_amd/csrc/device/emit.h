@@ -38,8 +38,11 @@ LRT_D double hosek(const double* c, double rad, double theta, double gamma) {
 LRT_D f3 sky_eval(const DevLight& L, f3 direction) {
     const f3 dir = normalize(direction);
     if (dir.y <= 0.0f) return splat(1.0f);  // below the horizon: 1, unscaled
-    const float theta = acosf(fminf(fmaxf(dir.y, 0.0f), 1.0f));
-    const float gamma = acosf(fminf(fmaxf(dot(dir, ld3(L.sun)), -1.0f), 1.0f));
+    // the angles rounded once from double precision, as the host's acosf
+    // rounds them: a device acosf one ulp above pi/2 at a grazing direction
+    // would make cos(theta) negative and the zenith term sqrt(cos) NaN
+    const float theta = (float)acos((double)fminf(fmaxf(dir.y, 0.0f), 1.0f));
+    const float gamma = (float)acos((double)fminf(fmaxf(dot(dir, ld3(L.sun)), -1.0f), 1.0f));
     return f3{(float)hosek(L.cfg[0], L.rad[0], theta, gamma), (float)hosek(L.cfg[1], L.rad[1], theta, gamma),
               (float)hosek(L.cfg[2], L.rad[2], theta, gamma)} * L.scale;
 }

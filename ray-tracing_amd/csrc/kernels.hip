@@ -357,6 +357,13 @@ __global__ __launch_bounds__(kBlock) void k_shade(const DevScene S, const DevLig
                 }
             }
             P.lr[p] = L.x; P.lg[p] = L.y; P.lb[p] = L.z;
+#ifdef LRT_NAN_PROBE
+            // diagnostic build only: report the first bounce that turns a path non-finite
+            if (!isfinite(L.x + L.y + L.z) || !isfinite(T.x + T.y + T.z) || (shadow && !isfinite(contrib.x + contrib.y + contrib.z)))
+                printf("nan: path %u depth %u obj %d prim %d t %g L %g %g %g T %g %g %g contrib %g %g %g dir %g %g %g\n", p,
+                       prm.depth, Hb.obj[i], Hb.prim[i], Hb.t[i], L.x, L.y, L.z, T.x, T.y, T.z, contrib.x, contrib.y,
+                       contrib.z, P.dx[p], P.dy[p], P.dz[p]);
+#endif
         }
         const uint32_t slot = wave_append(&counters[1], cont);
         if (cont) next_queue[slot] = p;
@@ -493,3 +500,6 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(const PathState P, uint32
 
 // photon tracing, sorting and the kd photon map (uses surface() and wave_append)
 #include "device/photon.h"
+
+// differentiable rendering (uses surface())
+#include "device/diff.h"

@@ -152,6 +152,7 @@ EXPORTED = [
     "lrt_generate_scene", "lrt_generated_desc", "lrt_generated_free", "lrt_philox4x32_10", "lrt_render_aov",
     "lrt_denoiser_create", "lrt_denoiser_release", "lrt_denoiser_param_count", "lrt_denoiser_params",
     "lrt_denoiser_forward", "lrt_denoiser_loss_grad", "lrt_denoiser_adam_step", "lrt_conv3x3",
+    "lrt_diff_param_count", "lrt_diff_render", "lrt_diff_adjoint", "lrt_diff_render_held",
 ]
 
 
@@ -251,9 +252,9 @@ class Generated:
         self.desc = self.desc_ptr.contents
 
     def close(self):
-        if self._h:
+        if self._h and LIB is not None:
             LIB.lrt_generated_free(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         self.close()
@@ -320,9 +321,9 @@ class Scene:
         self._h = h
 
     def close(self):
-        if self._h:
+        if self._h and LIB is not None:
             LIB.lrt_scene_release(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         self.close()
@@ -500,9 +501,9 @@ class Denoiser:
         self.n_params = n.value
 
     def close(self):
-        if self._h:
+        if self._h and LIB is not None:
             LIB.lrt_denoiser_release(self._h)
-            self._h = None
+        self._h = None
 
     def __del__(self):
         self.close()
