@@ -12,7 +12,9 @@
 //
 // d f / d alpha and d f / d eta come from central differences of the BSDF
 // value at the sampled directions (step 1e-3); the texture's derivative is
-// exact (a diffuse value is linear in its albedo).
+// exact (a diffuse value is linear in its albedo).  A sample whose radiance is
+// not finite (a few per 10^9) counts as black and adds no gradient, so one
+// cannot poison a loss.
 #pragma once
 
 namespace lrt {
@@ -156,12 +158,18 @@ LRT_D f3 diff_sample(const DevScene& S, const DevLights& Ls, const Mats& M, cons
                                     const float* rho = dm.m.diffuse;
                                     const float tv[3] = {term.x, term.y, term.z}, dv[3] = {delta.x, delta.y, delta.z};
                                     for (int c = 0; c < 3; ++c)
-                                        if (rho[c] > 0.0f && tv[c] != 0.0f)
+                                        if (rho[c] > 1e-20f && tv[c] != 0.0f && isfinite(dv[c] * tv[c] / rho[c]))
                                             atomicAdd(G.tex + 3 * (size_t)dm.texel + c, dv[c] * tv[c] / rho[c]);
                                 }
                                 const f3 base = T * ds.radiance * (w / lp);
-                                if (pk >= 0) G.s[pk] += dot(delta, base * diff_df(M, sh.material, 0, ctx, sr));
-                                if (pe) G.s[2] += dot(delta, base * diff_df(M, sh.material, 1, ctx, sr));
+                                if (pk >= 0) {
+                                    const float gv = dot(delta, base * diff_df(M, sh.material, 0, ctx, sr));
+                                    if (isfinite(gv)) G.s[pk] += gv;
+                                }
+                                if (pe) {
+                                    const float gv = dot(delta, base * diff_df(M, sh.material, 1, ctx, sr));
+                                    if (isfinite(gv)) G.s[2] += gv;
+                                }
                             }
                         }
                     }
@@ -184,14 +192,22 @@ LRT_D f3 diff_sample(const DevScene& S, const DevLights& Ls, const Mats& M, cons
                 const float* rho = dm.m.diffuse;
                 const float rv[3] = {R.x, R.y, R.z}, dv[3] = {delta.x, delta.y, delta.z};
                 for (int c = 0; c < 3; ++c)
-                    if (rho[c] > 0.0f && rv[c] != 0.0f) atomicAdd(G.tex + 3 * (size_t)dm.texel + c, dv[c] * rv[c] / rho[c]);
+                    if (rho[c] > 1e-20f && rv[c] != 0.0f && isfinite(dv[c] * rv[c] / rho[c]))
+                        atomicAdd(G.tex + 3 * (size_t)dm.texel + c, dv[c] * rv[c] / rho[c]);
             }
             if (pk >= 0 || pe) {
+                // d log f: channels whose value is too small to divide by carry no radiance
                 const EP e = bsdf_ep(dm.view, dm.index, ctx, sr, M_SOLID);
-                const f3 inv = f3{e.f.x > 0.0f ? 1.0f / e.f.x : 0.0f, e.f.y > 0.0f ? 1.0f / e.f.y : 0.0f,
-                                  e.f.z > 0.0f ? 1.0f / e.f.z : 0.0f};
-                if (pk >= 0) G.s[pk] += dot(delta, R * diff_df(M, sh.material, 0, ctx, sr) * inv);
-                if (pe) G.s[2] += dot(delta, R * diff_df(M, sh.material, 1, ctx, sr) * inv);
+                const f3 inv = f3{e.f.x > 1e-20f ? 1.0f / e.f.x : 0.0f, e.f.y > 1e-20f ? 1.0f / e.f.y : 0.0f,
+                                  e.f.z > 1e-20f ? 1.0f / e.f.z : 0.0f};
+                if (pk >= 0) {
+                    const float gv = dot(delta, R * diff_df(M, sh.material, 0, ctx, sr) * inv);
+                    if (isfinite(gv)) G.s[pk] += gv;
+                }
+                if (pe) {
+                    const float gv = dot(delta, R * diff_df(M, sh.material, 1, ctx, sr) * inv);
+                    if (isfinite(gv)) G.s[2] += gv;
+                }
             }
         }
         T = T * bw;
@@ -222,8 +238,10 @@ __global__ __launch_bounds__(kBlock) void k_diff_render(const DevScene S, const 
         const uint32_t x = i % D.W, y = i / D.W;
         const uint32_t s0 = j * chunk, s1 = min(D.spp, s0 + chunk);
         f3 acc = splat(0.0f);
-        for (uint32_t s = s0; s < s1; ++s)
-            acc = acc + diff_sample<false>(S, Ls, M, D, cam, x, y, D.spp0 + s, lds, splat(0.0f), splat(0.0f), G);
+        for (uint32_t s = s0; s < s1; ++s) {
+            const f3 L = diff_sample<false>(S, Ls, M, D, cam, x, y, D.spp0 + s, lds, splat(0.0f), splat(0.0f), G);
+            if (isfinite(L.x + L.y + L.z)) acc = acc + L;  // a non-finite sample counts as black
+        }
         float* o = part + 3 * (size_t)t;
         o[0] = acc.x; o[1] = acc.y; o[2] = acc.z;
     }
@@ -265,10 +283,10 @@ __global__ __launch_bounds__(kBlock) void k_diff_adjoint(const DevScene S, const
             const uint32_t s0 = j * chunk, s1 = min(D.spp, s0 + chunk);
             const float inv = 1.0f / (float)D.spp;
             const f3 delta = f3{d_img[3 * (size_t)i], d_img[3 * (size_t)i + 1], d_img[3 * (size_t)i + 2]} * inv;
-            if (!is_zero(delta))
+            if (!is_zero(delta) && isfinite(delta.x + delta.y + delta.z))
                 for (uint32_t s = s0; s < s1; ++s) {
                     const f3 L = diff_sample<false>(S, Ls, M, D, cam, x, y, D.spp0 + s, lds, splat(0.0f), splat(0.0f), G);
-                    diff_sample<true>(S, Ls, M, D, cam, x, y, D.spp0 + s, lds, L, delta, G);
+                    if (isfinite(L.x + L.y + L.z)) diff_sample<true>(S, Ls, M, D, cam, x, y, D.spp0 + s, lds, L, delta, G);
                 }
         }
     }

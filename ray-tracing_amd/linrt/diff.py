@@ -152,9 +152,16 @@ def optimize(iters=200, res=512, spp=32, max_depth=8, lr=0.02, seed=7, log=print
         img = Render.apply(theta, R)
         loss = ((img - target) ** 2).mean()
         loss.backward()
+        bad = int((~torch.isfinite(theta.grad)).sum())
+        if bad:  # a non-finite gradient would poison Adam's moments
+            theta.grad.nan_to_num_(nan=0.0, posinf=0.0, neginf=0.0)
+            log(f"  iter {it}: {bad} non-finite gradient entries zeroed")
         opt.step()
         clamp_(theta)
         hist.append(loss.item())
+        if not np.isfinite(hist[-1]):
+            log(f"  iter {it}: non-finite loss, stopping")
+            break
         if it % 10 == 0 or it == iters - 1:
             t = theta.detach()
             log(f"  iter {it}: loss {hist[-1]:.6f}, alpha {t[-3]:.3f}/{t[-2]:.3f} (true {truth[-3]:.3f}/{truth[-2]:.3f}), "
