@@ -2,6 +2,7 @@
 // Host work here: validating descriptions, kd-tree build and flattening,
 // device uploads, launch sequencing and HIP-event timing.
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -13,6 +14,20 @@
 #include "kernels.hip"
 #include "eye.hip"
 #include "device/nn.h"
+
+// paths per render pass (2^26: ~8.6 GiB of path state, hit and shadow
+// queues); the environment variable LINRT_PASS_PATHS overrides it
+#ifndef LRT_PASS_PATHS
+#define LRT_PASS_PATHS (1u << 26)
+#endif
+static uint32_t pass_paths() {
+    static const uint32_t v = [] {
+        const char* e = std::getenv("LINRT_PASS_PATHS");
+        const long long x = e ? std::atoll(e) : 0;
+        return x >= 1024 && x < (1ll << 31) ? (uint32_t)x : (uint32_t)LRT_PASS_PATHS;
+    }();
+    return v;
+}
 
 namespace lrt {
 
@@ -858,9 +873,11 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         const uint64_t npix64 = (uint64_t)W * (re - rb);
         require(npix64 < (1ull << 31), "too many pixels");
         const uint32_t npix = (uint32_t)npix64;
-        // samples per pass: small frames run several samples of every pixel
-        // side by side so each launch has enough paths (up to 2^22)
-        const uint32_t slots = std::max<uint32_t>(1, std::min<uint32_t>(prm->spp, (uint32_t)((1u << 22) / npix)));
+        // samples per pass: several samples of every pixel run side by side
+        // (up to LRT_PASS_PATHS paths), so the late bounces, whose launches
+        // last as long as their slowest ray whatever their size, serve many
+        // samples at once
+        const uint32_t slots = std::max<uint32_t>(1, std::min<uint32_t>(prm->spp, pass_paths() / npix));
         const uint32_t n = npix * slots;
         HIPCHECK(hipSetDevice(sc->device));
         const auto t0 = std::chrono::steady_clock::now();
