@@ -38,9 +38,12 @@ LRT_D double hosek(const double* c, double rad, double theta, double gamma) {
 LRT_D f3 sky_eval(const DevLight& L, f3 direction) {
     const f3 dir = normalize(direction);
     if (dir.y <= 0.0f) return splat(1.0f);  // below the horizon: 1, unscaled
-    // the angles rounded once from double precision, as the host's acosf
-    // rounds them: a device acosf one ulp above pi/2 at a grazing direction
-    // would make cos(theta) negative and the zenith term sqrt(cos) NaN
+    // the angles as env_emitter.h:79-80 computes them: the float acos of a
+    // float, correctly rounded (here from double precision; the device acosf
+    // can be an ulp off).  For 0 < y < 1.6e-8 the rounded theta is
+    // float(pi/2), which lies above pi/2: cos(theta) < 0 and the zenith term
+    // sqrt(cos(theta)) (ArHosekSkyModel.cc:300) is NaN, here as in the
+    // reference, and the NaN reaches the sample's radiance
     const float theta = (float)acos((double)fminf(fmaxf(dir.y, 0.0f), 1.0f));
     const float gamma = (float)acos((double)fminf(fmaxf(dot(dir, ld3(L.sun)), -1.0f), 1.0f));
     return f3{(float)hosek(L.cfg[0], L.rad[0], theta, gamma), (float)hosek(L.cfg[1], L.rad[1], theta, gamma),

@@ -61,6 +61,32 @@ def test_sky_matches_reference():
     assert np.array_equal(mine, d["sky_rgb"])
 
 
+# escaped directions the path tracer produced on the 1080p displaced-sphere
+# frame whose samples came out NaN (grazing, 0 < y < 1.6e-8), and two just above
+GRAZING = np.array([[0.948603, 1.39698e-08, -0.31647], [-0.189868, 1.49012e-08, 0.98181],
+                    [-0.569782, 2.79397e-09, 0.821796], [0.999903, 7.45058e-09, -0.0139272],
+                    [0.948603, 2.0e-08, -0.31647], [0.6, 1.0e-6, 0.8]], np.float32)
+
+
+def test_sky_grazing_nan_is_the_reference():
+    """For 0 < y < 1.6e-8 the float acos of y rounds to float(pi/2), which is
+    above pi/2, so cos(theta) < 0 and the zenith term sqrt(cos(theta)) of the
+    Hosek model is NaN (env_emitter.h:79, ArHosekSkyModel.cc:300)."""
+    assert np.cos(np.float64(np.float32(np.pi / 2))) < 0.0
+    assert np.float32(np.arccos(np.float64(np.float32(1.39698e-08)))) == np.float32(np.pi / 2)
+    assert np.float32(np.arccos(np.float64(np.float32(2.0e-08)))) < np.float32(np.pi / 2)
+    sky = linrt.load_sky()
+    mine = OL.sky_eval(sky, GRAZING)
+    assert np.isnan(mine[:4]).all() and np.isfinite(mine[4:]).all()
+    if OL.REF is not None:
+        ref = np.zeros_like(GRAZING)
+        sun = np.array(sky.sun_dir[:], np.float32)
+        assert OL.REF.ref_sky_eval(OL._p(sun), C.c_float(sky.turbidity), C.c_float(sky.ground_albedo),
+                                   C.c_float(sky.scale), OL._p(GRAZING), C.c_uint32(len(GRAZING)), OL._p(ref)) == 0
+        assert np.array_equal(np.isnan(ref), np.isnan(mine))
+        assert np.array_equal(ref[4:], mine[4:])
+
+
 def test_bsdf_oracle_is_the_reference(tables):
     d = np.load(GOLD / "bsdf_reference.npz")
     mats = material_zoo(tables)

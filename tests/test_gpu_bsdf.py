@@ -10,6 +10,7 @@ import pytest
 import linrt
 from common import NAMES, bsdf_tuples, material_zoo
 from oracle_lib import oracle
+from oracle_lib import sky_eval as oracle_sky
 
 pytestmark = pytest.mark.gpu
 GOLD = Path(__file__).resolve().parent / "golden"
@@ -86,6 +87,18 @@ def test_sky_gpu_vs_reference_vectors():
     sky = linrt.load_sky()
     rgb = linrt.sky_eval(sky, d["sky_dirs"])
     assert np.all(np.abs(rgb - d["sky_rgb"]) <= 1e-5 * np.abs(d["sky_rgb"]) + 1e-9)
+
+
+def test_sky_gpu_grazing_nan():
+    """Grazing escapes (0 < y < 1.6e-8) are NaN on the device exactly where
+    the oracle and the reference are NaN; just above they agree."""
+    from test_cpu_golden import GRAZING
+    sky = linrt.load_sky()
+    rgb = linrt.sky_eval(sky, GRAZING)
+    want = oracle_sky(sky, GRAZING)
+    assert np.array_equal(np.isnan(rgb), np.isnan(want))
+    ok = ~np.isnan(want)
+    assert np.all(np.abs(rgb[ok] - want[ok]) <= 1e-5 * np.abs(want[ok]))
 
 
 def test_invalid_material_rejected(zoo):
