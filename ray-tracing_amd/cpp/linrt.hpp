@@ -82,6 +82,61 @@ private:
     uint64_t seed_;
 };
 
+// LinRT::photon_integrator (photon_integrator.h:13-123): preprocess() builds
+// the global map and, when the scene has a delta BSDF, the caustic map with
+// the reference's constants (250,000 photons each, lookup 120, radii 0.05
+// and 0.0125 x the bounding-sphere radius, photon depth max_depth - 1, RR
+// from 5); render() is the eye pass over those maps.
+class gpu_photon_integrator {
+public:
+    explicit gpu_photon_integrator(uint32_t max_depth = 128, uint64_t seed = 1, uint32_t global_photons = 250000,
+                                   uint32_t caustic_photons = 250000, uint32_t lookup = 120)
+        : max_depth_(max_depth), seed_(seed), global_(global_photons), caustic_(caustic_photons), lookup_(lookup) {}
+
+    void preprocess(const scene& s) const {
+        const lrt_scene_info info = s.info();
+        lrt_photon_params p{};
+        p.kind = 0;
+        p.requested = global_;
+        p.max_depth = max_depth_ - 1;
+        p.rr_depth = 5;
+        p.seed = seed_;
+        p.radius = 0.05f * info.bounding_radius;
+        p.k = lookup_;
+        check(lrt_photon_build(s.handle(), &p, nullptr));
+        if (info.has_delta_bsdf && caustic_) {
+            p.kind = 1;
+            p.requested = caustic_;
+            p.radius = 0.0125f * info.bounding_radius;
+            check(lrt_photon_build(s.handle(), &p, nullptr));
+        } else {
+            check(lrt_photon_map_load(s.handle(), 1, nullptr, nullptr, 0, 1.0f, 0.0125f * info.bounding_radius, lookup_));
+        }
+    }
+
+    // pixels[y*width + x] = mean radiance of spp eye samples, row 0 at the bottom
+    lrt_render_stats render(const scene& s, uint32_t width, uint32_t height, uint32_t spp, std::vector<color>& pixels) const {
+        std::vector<double> sum((size_t)width * height * 3, 0.0);
+        lrt_render_params p{};
+        p.width = width;
+        p.height = height;
+        p.spp = spp;
+        p.max_depth = max_depth_;
+        p.seed = seed_;
+        lrt_render_stats st{};
+        check(lrt_render_photon(s.handle(), &p, sum.data(), nullptr, 0, &st));
+        pixels.resize((size_t)width * height);
+        for (size_t i = 0; i < pixels.size(); ++i)
+            pixels[i] = color{(float)(sum[3 * i] / spp), (float)(sum[3 * i + 1] / spp), (float)(sum[3 * i + 2] / spp)};
+        return st;
+    }
+
+private:
+    uint32_t max_depth_;
+    uint64_t seed_;
+    uint32_t global_, caustic_, lookup_;
+};
+
 // owner of a seeded generated scene (lrt_generate_scene)
 class generated {
 public:

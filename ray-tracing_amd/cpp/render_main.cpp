@@ -1,9 +1,11 @@
 // linrt_render: C++ driver in the shape of the reference's src/main.cpp
 // (scene -> integrator -> timed frame loop -> image), rendering the seeded
-// box-icosphere scene on the GPU through linrt::gpu_path_integrator and
-// writing a little-endian PFM (FP32 RGB, bottom row first).
+// box-icosphere scene on the GPU through linrt::gpu_path_integrator (or,
+// with --photon, linrt::gpu_photon_integrator: the reference's other
+// integrator, selected there by #ifdef, src/main.cpp:92-101) and writing a
+// little-endian PFM (FP32 RGB, bottom row first).
 //
-//   linrt_render OUT.pfm [width height spp max_depth rr_depth seed]
+//   linrt_render [--photon] OUT.pfm [width height spp max_depth rr_depth seed]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -13,8 +15,13 @@
 #include "linrt.hpp"
 
 int main(int argc, char** argv) {
+    const bool photon = argc > 1 && std::strcmp(argv[1], "--photon") == 0;
+    if (photon) {
+        --argc;
+        ++argv;
+    }
     if (argc < 2) {
-        std::cerr << "usage: linrt_render OUT.pfm [width height spp max_depth rr_depth seed]\n";
+        std::cerr << "usage: linrt_render [--photon] OUT.pfm [width height spp max_depth rr_depth seed]\n";
         return 2;
     }
     const uint32_t W = argc > 2 ? (uint32_t)std::atoi(argv[2]) : 256;
@@ -32,10 +39,17 @@ int main(int argc, char** argv) {
         gp.aspect = (float)W / (float)H;
         linrt::generated gen(gp);
         linrt::scene scene(gen.desc());
-        const linrt::gpu_path_integrator integrator(max_depth, rr_depth, seed);
         std::vector<linrt::color> pixels;
         const auto t0 = std::chrono::steady_clock::now();
-        const lrt_render_stats st = integrator.render(scene, W, H, spp, pixels);
+        lrt_render_stats st;
+        if (photon) {
+            const linrt::gpu_photon_integrator integrator(argc > 5 ? max_depth : 128, seed);
+            integrator.preprocess(scene);
+            st = integrator.render(scene, W, H, spp, pixels);
+        } else {
+            const linrt::gpu_path_integrator integrator(max_depth, rr_depth, seed);
+            st = integrator.render(scene, W, H, spp, pixels);
+        }
         const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         std::printf("%u x %u x %u spp: %.3f s, %.2f Msamples/s, %llu extension + %llu shadow rays\n", W, H, spp, s,
                     (double)st.samples / s / 1e6, (unsigned long long)st.extension_rays,
