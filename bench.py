@@ -128,7 +128,12 @@ def photon_bench(args, torch, linrt):
 
     for _ in range(args.warmup):
         step()
+    # untimed 1-spp counting pass: the gathers' algorithmic work per query
+    if args.warmup == 0:
+        step()
+    _, _, cnt = scene.render_photon(res, res, 1, 128, seed=SEED, sumsq=False, count=True)
     torch.cuda.synchronize()
+    scene.kernel_timing(linrt.KERNEL_GATHER, reset=True)
     t0 = time.perf_counter()
     rec = []
     for k in range(args.steps):
@@ -141,6 +146,13 @@ def photon_bench(args, torch, linrt):
     eye_s = sum(r[1].seconds for r in rec)
     eye = rec[-1][1]
     samples = res * res * spp
+    # gather roofline: B_query = 56 (query) + 16 n_node + 16 n_tested + 52 n_summed + 24 (sum out) bytes
+    g_ms, g_n, _ = scene.kernel_timing(linrt.KERNEL_GATHER)
+    q_c = max(cnt.segments, 1)
+    per_query = 56 + 24 + (16 * cnt.nodes_visited + 16 * cnt.refs_read + 52 * cnt.tris_tested) / q_c
+    queries_per_launch = eye.segments * args.steps / max(g_n, 1)
+    g_avg_ms = g_ms / max(g_n, 1)
+    g_achieved = per_query * queries_per_launch / (g_avg_ms * 1e-3) / 1e9
     out = {"metric": "photon-mapped frame: eye Msamples/s (1024x1024, 4M photons, 262k tris)",
            "value": round(samples * args.steps / eye_s / 1e6, 3), "unit": "Msamples/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 2),
@@ -153,8 +165,17 @@ def photon_bench(args, torch, linrt):
            "eye_pass_s": round(eye_s / args.steps, 4),
            "eye_rays_per_sample": round((eye.extension_rays + eye.shadow_rays) / samples, 2),
            "eye_mrays_per_s": round((eye.extension_rays + eye.shadow_rays) / eye.seconds / 1e6, 1),
+           "eye_gathers_per_sample": round(eye.segments / samples, 2),
            "eye_kernel_ms": {"extend": round(eye.trace_ms, 1), "shadow": round(eye.occluded_ms, 1),
-                             "shade": round(eye.shade_ms, 1)}}
+                             "shade": round(eye.shade_ms, 1), "gather": round(g_ms / args.steps, 1)},
+           "gather_work_per_query": {"nodes": round(cnt.nodes_visited / q_c, 2),
+                                     "photons_tested": round(cnt.refs_read / q_c, 2),
+                                     "photons_summed": round(cnt.tris_tested / q_c, 2)},
+           "roofline": {"bound": "hbm", "kernel": "k_eye_gather (k-nearest photon gathers)",
+                        "achieved": round(g_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(g_achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                        "bytes_per_launch": int(per_query * queries_per_launch), "avg_launch_ms": round(g_avg_ms, 4),
+                        "launches": int(g_n)}}
     print(json.dumps(out), flush=True)
 
 

@@ -192,7 +192,7 @@ int lrt_scene_sync(lrt_scene* scene);
  * timed with HIP events on the scene's stream */
 int lrt_scene_kernel_timing(lrt_scene* scene, int kernel, int reset,
                             double* total_ms, uint64_t* launches, uint64_t* items);
-enum { LRT_KERNEL_CLOSEST = 0, LRT_KERNEL_OCCLUDED = 1, LRT_KERNEL_SHADE = 2, LRT_KERNEL_COUNT_ = 3 };
+enum { LRT_KERNEL_CLOSEST = 0, LRT_KERNEL_OCCLUDED = 1, LRT_KERNEL_SHADE = 2, LRT_KERNEL_GATHER = 3, LRT_KERNEL_COUNT_ = 4 };
 
 /* ---- BSDF batch evaluation (bsdf.h:78-88) -----------------------------
  * For n tuples i: material index mat[i] into the scene's (or the given)
@@ -302,7 +302,10 @@ int lrt_photon_map_load(lrt_scene* scene, int32_t kind, const float* photons12, 
 int lrt_photon_estimate(lrt_scene* scene, int32_t kind, const float* queries6, uint32_t n, int max_depth, float* out3);
 /* photon_integrator::radiance (photon_integrator.h:39-123) for every (pixel,
  * sample) with the scene's maps (an empty map is skipped); max_depth is the
- * integrator's depth limit (photon_integrator.h:57: 128), rr_depth unused. */
+ * integrator's depth limit (photon_integrator.h:57: 128), rr_depth unused.
+ * stats->segments = photon gathers; with LRT_RENDER_COUNT, nodes_visited /
+ * refs_read / tris_tested count the gathers' kd nodes, photon positions tested
+ * and photons summed (the k-nearest sets).  Gather kernel time: timer LRT_KERNEL_GATHER. */
 int lrt_render_photon(lrt_scene* scene, const lrt_render_params* params, double* sum_rgb, double* sumsq_rgb,
                       int device_ptrs, lrt_render_stats* stats);
 

@@ -266,10 +266,11 @@ struct lrt_scene {
     DevBuf q_rays, q_out0, q_out1, q_out2, q_aux;
     // photon mapping
     PhotonState pm[2];   // global, caustic
-    DevBuf eye_items[2], eye_hits, eye_shadow, eye_query, eye_acc;
+    DevBuf eye_items[2], eye_hits, eye_shadow, eye_query, eye_acc, eye_sort, eye_hist;
     size_t eye_cap[2] = {0, 0}, eye_hit_cap = 0, eye_shadow_cap = 0, eye_query_cap = 0;
     EmitCdf emit;
     float bounding_radius = 0.0f;
+    float box_lo[3] = {0.0f, 0.0f, 0.0f}, box_hi[3] = {0.0f, 0.0f, 0.0f};  // scene bounds
     bool has_delta = false;
     DevBuf diff_mats, diff_tex, diff_theta, diff_img, diff_dimg, diff_grad, diff_part;
     std::vector<lrt_material> host_mats;
@@ -479,6 +480,8 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
         }
         float h[3];
         for (int a = 0; a < 3; ++a) h[a] = (hi[a] - lo[a]) * 0.5f;
+        if (d.n_objects)
+            for (int a = 0; a < 3; ++a) { sc.box_lo[a] = lo[a]; sc.box_hi[a] = hi[a]; }
         sc.bounding_radius = d.n_objects ? std::sqrt(h[0] * h[0] + h[1] * h[1] + h[2] * h[2]) : 0.0f;
         sc.has_delta = delta;
         sc.max_spec_components = d.n_materials ? host_max_specular_components(mats) : 0;
@@ -1207,11 +1210,11 @@ int lrt_diff_adjoint(lrt_scene* sc, const lrt_diff_params* dp, const float* thet
 }
 
 // ---- photon mapping -----------------------------------------------------------------
-static void radix_sort(lrt_scene& sc, PhotonState& S, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t n,
+static void radix_sort(lrt_scene& sc, DevBuf& hbuf, uint32_t* k0, uint32_t* v0, uint32_t* k1, uint32_t* v1, uint32_t n,
                        int bits, uint32_t*& out_k, uint32_t*& out_v) {
     const uint32_t nb = std::max<uint32_t>(1, (n + kRsTile - 1) / kRsTile);
-    S.hist.alloc((size_t)nb * 16 * 4);
-    uint32_t* hist = S.hist.as<uint32_t>();
+    hbuf.alloc((size_t)nb * 16 * 4);
+    uint32_t* hist = hbuf.as<uint32_t>();
     for (int shift = 0; shift < bits; shift += 4) {
         hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(kRsBlock), 0, sc.stream, k0, n, shift, hist, nb);
         hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, sc.stream, hist, nb * 16);
@@ -1241,7 +1244,7 @@ static void build_photon_map(lrt_scene& sc, PhotonState& S, uint32_t n_raw, uint
     if (n_raw) {
         HIPCHECK(hipMemcpyAsync(ka, S.buf.key, (size_t)n_raw * 4, hipMemcpyDeviceToDevice, sc.stream));
         hipLaunchKernelGGL(k_iota, dim3(g), dim3(256), 0, sc.stream, va, n_raw);
-        radix_sort(sc, S, ka, va, kb, vb, n_raw, 32, sk, sv);
+        radix_sort(sc, S.hist, ka, va, kb, vb, n_raw, 32, sk, sv);
     }
     S.perm.alloc((size_t)std::max<uint32_t>(n_keep, 1) * 4);
     if (n_keep) HIPCHECK(hipMemcpyAsync(S.perm.p, sv, (size_t)n_keep * 4, hipMemcpyDeviceToDevice, sc.stream));
@@ -1257,6 +1260,8 @@ static void build_photon_map(lrt_scene& sc, PhotonState& S, uint32_t n_raw, uint
     PhotonKd kd;
     photon_kd_build(xyz.data(), n_keep, kd);
     require(kd.depth < kPkStack, "photon kd tree deeper than the search stack");
+    for (size_t i = 0; i < kd.nodes.size(); i += 4)  // the device search loads a leaf's photons 8 at a time
+        require(kd.nodes[i + 1] != 3u || kd.nodes[i + 3] <= 8u, "photon kd leaf with more than 8 photons");
     // 3. photons in tree order
     S.nodes.alloc(std::max<size_t>(kd.nodes.size(), 4) * 4);
     S.order.alloc((size_t)std::max<uint32_t>(n_keep, 1) * 4);
@@ -1432,10 +1437,10 @@ int lrt_photon_estimate(lrt_scene* sc, int32_t kind, const float* queries6, uint
         HIPCHECK(hipSetDevice(sc->device));
         upload(sc->q_rays, queries6, (size_t)n * 24, sc->stream);
         sc->q_out0.alloc((size_t)n * 12);
-        const uint32_t g = std::min<uint32_t>((n + kGatherBlock - 1) / kGatherBlock, (uint32_t)sc->cus * 16);
-        const size_t lds = gather_lds_bytes(S.map.k);
-        HIPCHECK(hipFuncSetAttribute((const void*)k_photon_estimate, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_photon_estimate, dim3(g), dim3(kGatherBlock), lds, sc->stream, S.map, sc->q_rays.as<float>(), n,
+        const uint32_t per_block = kGatherBlock / kGroup;
+        const uint32_t g = std::min<uint32_t>((n + per_block - 1) / per_block, (uint32_t)sc->cus * 32);
+        hipLaunchKernelGGL(gather_slots(S.map.k) == 8 ? k_photon_estimate<8> : k_photon_estimate<16>, dim3(g),
+                           dim3(kGatherBlock), 0, sc->stream, S.map, sc->q_rays.as<float>(), n,
                            max_depth, sc->q_out0.as<float>());
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(out3, sc->q_out0.p, (size_t)n * 12, hipMemcpyDeviceToHost, sc->stream));
@@ -1545,8 +1550,12 @@ static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, doub
     PhotonMapDev G0 = sc->pm[0].map, G1 = sc->pm[1].map;
     if (!(maps & 1u)) std::memset(&G0, 0, sizeof(G0));
     if (!(maps & 2u)) std::memset(&G1, 0, sizeof(G1));
-    const size_t gather_lds = gather_lds_bytes(std::max(G0.k, G1.k));
-    HIPCHECK(hipFuncSetAttribute((const void*)k_eye_gather, hipFuncAttributeMaxDynamicSharedMemorySize, (int)gather_lds));
+    const bool slots8 = gather_slots(std::max(G0.k, G1.k)) == 8;
+    const bool counting = (prm->flags & LRT_RENDER_COUNT) != 0;
+    if (counting) {
+        sc->totals.alloc(256);
+        HIPCHECK(hipMemsetAsync(sc->totals.p, 0, 32, sc->stream));
+    }
     const uint64_t pixels = (uint64_t)(re - rb) * W;
     const uint32_t chunk = (uint32_t)std::min<uint64_t>(pixels, 1u << 18);
     sc->counters.alloc(256);
@@ -1554,14 +1563,22 @@ static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, doub
     sc->eye_acc.alloc((size_t)chunk * 3 * 8);
     unsigned long long* acc = sc->eye_acc.as<unsigned long long>();
     const uint2 key = make_uint2((uint32_t)prm->seed, (uint32_t)(prm->seed >> 32));
+    const f3 box_lo{sc->box_lo[0], sc->box_lo[1], sc->box_lo[2]};
+    auto cells = [&](int a) {  // Morton cells per unit length along axis a
+        const float ext = sc->box_hi[a] - sc->box_lo[a];
+        return ext > 0.0f ? 512.0f / ext : 0.0f;
+    };
+    const f3 box_scale{cells(0), cells(1), cells(2)};
     uint64_t n_ext = 0, n_shadow = 0, n_query = 0;
     Timer& tc = sc->timers[LRT_KERNEL_CLOSEST];
     Timer& to = sc->timers[LRT_KERNEL_OCCLUDED];
     Timer& ts = sc->timers[LRT_KERNEL_SHADE];
+    Timer& tg = sc->timers[LRT_KERNEL_GATHER];
     HIPCHECK(hipStreamSynchronize(sc->stream));
     tc.collect();
     to.collect();
     ts.collect();
+    tg.collect();
     const double before_c = tc.total_ms, before_o = to.total_ms, before_s = ts.total_ms;
     const uint64_t lc0 = tc.launches, lo0 = to.launches;
     for (uint32_t s = 0; s < prm->spp; ++s) {
@@ -1633,11 +1650,33 @@ static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, doub
                                        dim3(kBlock), 0, sc->stream, sc->S, Sh, cnt[1], acc);
                     HIPCHECK(hipEventRecord(e3.second, sc->stream));
                 }
-                if (cnt[2])
-                    hipLaunchKernelGGL(k_eye_gather,
-                                       dim3((uint32_t)std::min<uint64_t>((cnt[2] + kGatherBlock - 1) / kGatherBlock,
-                                                                         (uint64_t)sc->cus * 16)),
-                                       dim3(kGatherBlock), gather_lds, sc->stream, G0, G1, Q, cnt[2], acc);
+                if (cnt[2]) {
+                    // queries in Morton order of their positions (the estimates
+                    // and the fixed-point sums do not depend on the order)
+                    const uint32_t nq = cnt[2];
+                    sc->eye_sort.alloc((size_t)nq * 16 + 4 * 256);
+                    char* c = sc->eye_sort.as<char>();
+                    uint32_t* ka = alloc_carve<uint32_t>(c, nq);
+                    uint32_t* va = alloc_carve<uint32_t>(c, nq);
+                    uint32_t* kb = alloc_carve<uint32_t>(c, nq);
+                    uint32_t* vb = alloc_carve<uint32_t>(c, nq);
+                    hipLaunchKernelGGL(k_eye_query_keys,
+                                       dim3((uint32_t)std::min<uint64_t>((nq + kBlock - 1) / kBlock, (uint64_t)sc->cus * 16)),
+                                       dim3(kBlock), 0, sc->stream, Q, nq, box_lo, box_scale, ka, va);
+                    uint32_t *sk = ka, *sv = va;
+                    radix_sort(*sc, sc->eye_hist, ka, va, kb, vb, nq, 28, sk, sv);
+                    const uint32_t gq = (uint32_t)std::min<uint64_t>(
+                        (nq + kGatherBlock / kGroup - 1) / (kGatherBlock / kGroup), (uint64_t)sc->cus * 32);
+                    if (counting)
+                        hipLaunchKernelGGL(slots8 ? k_eye_gather_count<8> : k_eye_gather_count<16>, dim3(gq),
+                                           dim3(kGatherBlock), 0, sc->stream, G0, G1, Q, nq,
+                                           sc->totals.as<unsigned long long>());
+                    auto e4 = tg.next();
+                    HIPCHECK(hipEventRecord(e4.first, sc->stream));
+                    hipLaunchKernelGGL(slots8 ? k_eye_gather<8> : k_eye_gather<16>, dim3(gq), dim3(kGatherBlock), 0,
+                                       sc->stream, G0, G1, Q, sv, nq, acc);
+                    HIPCHECK(hipEventRecord(e4.second, sc->stream));
+                }
                 HIPCHECK(hipGetLastError());
                 cur = nxt;
                 std::swap(A, B);
@@ -1657,6 +1696,7 @@ static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, doub
     tc.collect();
     to.collect();
     ts.collect();
+    tg.collect();
     if (stats) {
         std::memset(stats, 0, sizeof(*stats));
         stats->samples = pixels * prm->spp;
@@ -1669,6 +1709,13 @@ static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, doub
         stats->shade_ms = ts.total_ms - before_s;
         stats->trace_launches = tc.launches - lc0;
         stats->occluded_launches = to.launches - lo0;
+        if (counting) {
+            unsigned long long t[3];
+            HIPCHECK(hipMemcpy(t, sc->totals.p, sizeof(t), hipMemcpyDeviceToHost));
+            stats->nodes_visited = t[0];
+            stats->refs_read = t[1];
+            stats->tris_tested = t[2];
+        }
     }
 }
 

@@ -279,127 +279,177 @@ __global__ void k_fill_u32(uint32_t* p, uint32_t n, uint32_t v) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
 
-constexpr int kGatherBlock = 64;
 constexpr int kMaxK = 128;
-constexpr int kPkStack = 32;  // > photon kd depth (checked at build)
+constexpr int kPkStack = 32;       // > photon kd depth (checked at build)
+constexpr int kGroup = 8;          // lanes per photon query (a leaf holds <= 8 photons)
+constexpr int kGatherBlock = 256;  // 32 queries per block
 
-// LDS of a gather block (dynamic shared memory, sized for the launch's k):
-// per lane a heap of (d2, index) pairs of kcap entries and the far-child
-// stack, each strided by the block size
-struct GatherLds {
-    float* hd;
-    uint32_t* hi;
-    uint32_t* st;
-};
-inline size_t gather_lds_bytes(int kcap) { return (size_t)(2 * kcap + kPkStack) * kGatherBlock * 4; }
-LRT_D GatherLds gather_lds(int kcap) {
-    extern __shared__ uint32_t gather_smem[];
-    GatherLds G;
-    G.hd = (float*)gather_smem + threadIdx.x;
-    G.hi = gather_smem + (size_t)kcap * kGatherBlock + threadIdx.x;
-    G.st = gather_smem + (size_t)2 * kcap * kGatherBlock + threadIdx.x;
-    return G;
+// lanes of the wave: value of v in lane src (0..63)
+LRT_D uint32_t lane_read(uint32_t v, int src) { return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v); }
+LRT_D float lane_read(float v, int src) { return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(v))); }
+
+// maximum of k over the 8 lanes of a query's group (every lane gets it)
+LRT_D uint64_t group_max(uint64_t k, int lane) {
+#pragma unroll
+    for (int m = 1; m < kGroup; m <<= 1) {
+        const int src = lane ^ m;
+        const uint64_t o = ((uint64_t)lane_read((uint32_t)(k >> 32), src) << 32) | lane_read((uint32_t)k, src);
+        k = o > k ? o : k;
+    }
+    return k;
 }
 
-// heap entries are (d2, index) pairs compared like std::pair (priority_queue)
-LRT_D bool pair_less(float da, uint32_t ia, float db, uint32_t ib) { return da < db || (!(db < da) && ia < ib); }
-
-// remove the heap root: last element to the root, sift down
-LRT_D void heap_pop(float* hd, uint32_t* hi, int& size) {
-    constexpr uint32_t st = kGatherBlock;
-    --size;
-    const float ld = hd[size * st];
-    const uint32_t li = hi[size * st];
-    int j = 0;
-    for (;;) {
-        int c = 2 * j + 1;
-        if (c >= size) break;
-        if (c + 1 < size && pair_less(hd[c * st], hi[c * st], hd[(c + 1) * st], hi[(c + 1) * st])) ++c;
-        if (!pair_less(ld, li, hd[c * st], hi[c * st])) break;
-        hd[j * st] = hd[c * st];
-        hi[j * st] = hi[c * st];
-        j = c;
-    }
-    hd[j * st] = ld;
-    hi[j * st] = li;
+template <int S>
+LRT_D uint64_t slots_max(const uint64_t (&key)[S]) {
+    uint64_t m = key[0];
+#pragma unroll
+    for (int s = 1; s < S; ++s) m = key[s] > m ? key[s] : m;
+    return m;
 }
 
 // k nearest photons within r of p (photon_map.h:117-165) and the irradiance
-// estimate over them (photon_map.h:38-70).  The recursion's far-child test
-// uses the limit taken when the node was entered, so it is decided on entry
-// and the far child pushed before descending into the near one.
-LRT_D f3 photon_estimate(const PhotonMapDev& Mp, f3 p, f3 n, int max_depth, const GatherLds& G) {
+// estimate over them (photon_map.h:38-70), computed by the 8 lanes of a group
+// together; every lane returns the estimate.
+//
+// The search keeps the reference's order: the far child is pushed when the
+// node is entered if delta^2 <= the limit at that moment (r^2 until k photons
+// are held, then the largest held distance, capped at r^2); a leaf's photons
+// are tested in index order, each against the limit after the previous
+// insertion.  The k-nearest set lives in registers as 64-bit keys
+// (distance^2 bits << 32 | index + 1; 0 = empty), S slots in each of the 8
+// lanes: the reference's max-heap pops the largest (distance^2, index) pair and
+// pushes the new one, which leaves the same set as replacing the largest key.
+// The estimate sums the set in the heap's pop order (largest key first).
+// COUNT: also count the work into c (nodes visited, photon positions tested,
+// photons summed) in lane 0 of the group.
+template <int S, bool COUNT>
+LRT_D f3 photon_estimate_group(const PhotonMapDev& Mp, f3 p, f3 n, int max_depth, uint32_t* stack, Counters& c) {
     if (Mp.n == 0 || Mp.radius <= 0.0f || Mp.k <= 0) return splat(0.0f);
-    constexpr uint32_t st = kGatherBlock;
-    float* hd = G.hd;
-    uint32_t* hi = G.hi;
-    uint32_t* stack = G.st;
+    const int lane = (int)(threadIdx.x & 63u), gl = lane & (kGroup - 1), gb = lane - gl;
+    const bool counter = COUNT && gl == 0;
     const float r2 = Mp.radius * Mp.radius;
     const int K = Mp.k;
+    uint64_t key[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) key[s] = 0;
     int size = 0, sp = 0;
+    float maxd = 0.0f;
+    uint64_t maxkey = 0;
     uint32_t node = 0;
     for (;;) {
         const uint4 nd = Mp.nodes[node];
+        if (counter) ++c.nodes;
         float limit = r2;
-        if (size == K) limit = fminf(limit, hd[0]);
+        if (size == K) limit = fminf(limit, maxd);
         if (nd.y == 3u) {
-            for (uint32_t i = nd.z; i < nd.z + nd.w; ++i) {
-                const float4 q = Mp.pos[i];
-                const f3 dv = f3{q.x, q.y, q.z} - p;
-                const float d = len2(dv);
-                if (d > limit) continue;
-                if (size == K) heap_pop(hd, hi, size);
-                int j = size++;  // push, sift up
-                while (j > 0) {
-                    const int par = (j - 1) / 2;
-                    if (!pair_less(hd[par * st], hi[par * st], d, i)) break;
-                    hd[j * st] = hd[par * st];
-                    hi[j * st] = hi[par * st];
-                    j = par;
+            if (counter) c.refs += nd.w;
+            float d = 0.0f;
+            bool cand = false;
+            if ((uint32_t)gl < nd.w) {
+                const float4 q = Mp.pos[nd.z + (uint32_t)gl];
+                d = len2(f3{q.x, q.y, q.z} - p);
+                cand = !(d > limit);  // the limit only shrinks: a photon beyond it now stays out
+            }
+            uint32_t m = (uint32_t)(__ballot(cand) >> gb) & 0xffu;
+            while (m) {
+                const int ci = __builtin_ctz(m);
+                m &= m - 1u;
+                const float dc = lane_read(d, gb + ci);
+                if (dc > limit) continue;
+                const uint64_t nk = ((uint64_t)__float_as_uint(dc) << 32) | (uint64_t)(nd.z + (uint32_t)ci + 1u);
+                if (size < K) {
+                    const int t = size++;
+#pragma unroll
+                    for (int s = 0; s < S; ++s)
+                        if (s == (t >> 3) && gl == (t & 7)) key[s] = nk;
+                    if (size < K) continue;
+                } else {
+#pragma unroll
+                    for (int s = 0; s < S; ++s)
+                        if (key[s] == maxkey) key[s] = nk;
                 }
-                hd[j * st] = d;
-                hi[j * st] = i;
-                if (size == K) limit = fminf(r2, hd[0]);
+                maxkey = group_max(slots_max(key), lane);
+                maxd = __uint_as_float((uint32_t)(maxkey >> 32));
+                limit = fminf(r2, maxd);
             }
             if (sp == 0) break;
-            node = stack[--sp * st];
+            node = stack[--sp];
             continue;
         }
         const float pv = nd.y == 0u ? p.x : (nd.y == 1u ? p.y : p.z);
         const float delta = pv - __uint_as_float(nd.x);
         const bool left_near = delta <= 0.0f;
-        if (delta * delta <= limit) stack[sp++ * st] = left_near ? nd.w : nd.z;
+        if (delta * delta <= limit) stack[sp++] = left_near ? nd.w : nd.z;
         node = left_near ? nd.z : nd.w;
+    }
+    if (counter) c.tris += (uint32_t)size;
+    // the set in descending key order: entry t in lane t % 8, slot t / 8
+    uint64_t ord[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) ord[s] = 0;
+    for (int t = 0; t < size; ++t) {
+        const uint64_t mk = group_max(slots_max(key), lane);
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            if (key[s] == mk) key[s] = 0;
+            if (s == (t >> 3) && gl == (t & 7)) ord[s] = mk;
+        }
     }
     const float inv_r2 = 1.0f / r2;
     f3 sum = splat(0.0f);
-    while (size > 0) {  // pop order: farthest first, like estimate_irradiance
-        const float d = hd[0];
-        const uint32_t idx = hi[0];
-        heap_pop(hd, hi, size);
-        const float4 q = Mp.pos[idx];
-        if ((int)q.w > max_depth) continue;
-        const float* pay = Mp.pay + 9 * (size_t)idx;
-        const f3 in = -ld3(pay);
-        const f3 pn = ld3(pay + 3);
-        const float ip = dot(pn, in), is = dot(n, in);
-        if (is <= 0.0f || dot(pn, n) <= 1e-1f || ip <= 1e-2f) continue;
-        const float sq = 1.0f - d * inv_r2;
-        const float w = (is / ip) * (sq * sq);
-        sum = sum + ld3(pay + 6) * fabsf(w);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        if (s * kGroup >= size) break;
+        // this lane's entry of the slot, then the slot's 8 entries summed in order
+        f3 term = splat(0.0f);
+        uint32_t use = 0;
+        if (s * kGroup + gl < size) {
+            const uint32_t idx = (uint32_t)ord[s] - 1u;
+            const float d = __uint_as_float((uint32_t)(ord[s] >> 32));
+            const float4 q = Mp.pos[idx];
+            if (!((int)q.w > max_depth)) {
+                const float* pay = Mp.pay + 9 * (size_t)idx;
+                const f3 in = -ld3(pay);
+                const f3 pn = ld3(pay + 3);
+                const float ip = dot(pn, in), is = dot(n, in);
+                if (!(is <= 0.0f || dot(pn, n) <= 1e-1f || ip <= 1e-2f)) {
+                    const float sq = 1.0f - d * inv_r2;
+                    const float w = (is / ip) * (sq * sq);
+                    term = ld3(pay + 6) * fabsf(w);
+                    use = 1u;
+                }
+            }
+        }
+        const int cnt = min(kGroup, size - s * kGroup);
+        for (int j = 0; j < cnt; ++j) {
+            const uint32_t u = lane_read(use, gb + j);
+            const f3 v{lane_read(term.x, gb + j), lane_read(term.y, gb + j), lane_read(term.z, gb + j)};
+            if (u) sum = sum + v;
+        }
     }
     return sum * (Mp.scale * 3.0f * (1.0f / kPiF) * inv_r2);
 }
 
+// queries handled by groups of 8 lanes; S = key slots per lane (k <= 8 S)
+template <int S>
 __global__ __launch_bounds__(kGatherBlock) void k_photon_estimate(const PhotonMapDev Mp, const float* __restrict__ q6,
                                                                   uint32_t n, int max_depth, float* __restrict__ out) {
-    const GatherLds G = gather_lds(Mp.k);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const f3 e = photon_estimate(Mp, ld3(q6 + 6 * (size_t)i), ld3(q6 + 6 * (size_t)i + 3), max_depth, G);
-        out[3 * (size_t)i] = e.x;
-        out[3 * (size_t)i + 1] = e.y;
-        out[3 * (size_t)i + 2] = e.z;
+    __shared__ uint32_t stacks[(kGatherBlock / kGroup) * kPkStack];
+    uint32_t* stack = stacks + (threadIdx.x / kGroup) * kPkStack;
+    const uint32_t groups = gridDim.x * (kGatherBlock / kGroup);
+    for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / kGroup; i < n; i += groups) {
+        Counters c{0, 0, 0};
+        const f3 e = photon_estimate_group<S, false>(Mp, ld3(q6 + 6 * (size_t)i), ld3(q6 + 6 * (size_t)i + 3), max_depth,
+                                                     stack, c);
+        if ((threadIdx.x & (kGroup - 1)) == 0) {
+            out[3 * (size_t)i] = e.x;
+            out[3 * (size_t)i + 1] = e.y;
+            out[3 * (size_t)i + 2] = e.z;
+        }
     }
 }
+
+// key slots per lane for k photons per lookup
+inline int gather_slots(int k) { return k <= 64 ? 8 : 16; }
 
 }  // namespace lrt

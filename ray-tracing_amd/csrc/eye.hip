@@ -311,15 +311,70 @@ __global__ __launch_bounds__(kBlock) void k_eye_shadow(const DevScene S, const E
     }
 }
 
-__global__ __launch_bounds__(kGatherBlock) void k_eye_gather(const PhotonMapDev G0, const PhotonMapDev G1, const EyeQuery Q,
-                                                             uint32_t n, unsigned long long* __restrict__ acc) {
-    const GatherLds L = gather_lds(max(G0.k, G1.k));
+// sort key of a gather query: its map in bit 27 above the 27-bit Morton code
+// of its position in the scene box (9 bits per axis), so that the queries of
+// one gather wave lie close together and walk the same part of one kd tree
+LRT_D uint32_t spread3(uint32_t v) {  // bit i of v -> bit 3 i
+    v &= 0x1ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__global__ __launch_bounds__(kBlock) void k_eye_query_keys(const EyeQuery Q, uint32_t n, f3 lo, f3 scale,
+                                                           uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t meta = Q.meta[i];
-        const f3 E = photon_estimate((meta & 1u) ? G1 : G0, f3{Q.px[i], Q.py[i], Q.pz[i]}, f3{Q.nx[i], Q.ny[i], Q.nz[i]},
-                                     (int)(meta >> 1), L);
-        eye_add(acc, Q.sid[i], f3{Q.tr[i], Q.tg[i], Q.tb[i]} * (E * f3{Q.wr[i], Q.wg[i], Q.wb[i]}));
+        const f3 u = (f3{Q.px[i], Q.py[i], Q.pz[i]} - lo) * scale;
+        const uint32_t x = (uint32_t)fminf(fmaxf(u.x, 0.0f), 511.0f);
+        const uint32_t y = (uint32_t)fminf(fmaxf(u.y, 0.0f), 511.0f);
+        const uint32_t z = (uint32_t)fminf(fmaxf(u.z, 0.0f), 511.0f);
+        keys[i] = ((Q.meta[i] & 1u) << 27) | (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
+        vals[i] = i;
     }
+}
+
+// gather queries in the given order (a permutation of 0..n-1), one query per
+// group of 8 lanes; S = key slots per lane (k <= 8 S)
+template <int S>
+__global__ __launch_bounds__(kGatherBlock) void k_eye_gather(const PhotonMapDev G0, const PhotonMapDev G1, const EyeQuery Q,
+                                                             const uint32_t* __restrict__ order, uint32_t n,
+                                                             unsigned long long* __restrict__ acc) {
+    __shared__ uint32_t stacks[(kGatherBlock / kGroup) * kPkStack];
+    uint32_t* stack = stacks + (threadIdx.x / kGroup) * kPkStack;
+    const uint32_t groups = gridDim.x * (kGatherBlock / kGroup);
+    for (uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) / kGroup; t < n; t += groups) {
+        const uint32_t i = order[t];
+        const uint32_t meta = Q.meta[i];
+        Counters c{0, 0, 0};
+        const f3 E = photon_estimate_group<S, false>((meta & 1u) ? G1 : G0, f3{Q.px[i], Q.py[i], Q.pz[i]},
+                                                     f3{Q.nx[i], Q.ny[i], Q.nz[i]}, (int)(meta >> 1), stack, c);
+        if ((threadIdx.x & (kGroup - 1)) == 0)
+            eye_add(acc, Q.sid[i], f3{Q.tr[i], Q.tg[i], Q.tb[i]} * (E * f3{Q.wr[i], Q.wg[i], Q.wb[i]}));
+    }
+}
+
+// the same gathers with work counters (nodes visited, photon positions
+// tested, photons summed) instead of contributions
+template <int S>
+__global__ __launch_bounds__(kGatherBlock) void k_eye_gather_count(const PhotonMapDev G0, const PhotonMapDev G1,
+                                                                   const EyeQuery Q, uint32_t n,
+                                                                   unsigned long long* __restrict__ totals) {
+    __shared__ uint32_t stacks[(kGatherBlock / kGroup) * kPkStack];
+    uint32_t* stack = stacks + (threadIdx.x / kGroup) * kPkStack;
+    const uint32_t groups = gridDim.x * (kGatherBlock / kGroup);
+    uint64_t a = 0, b = 0, c3 = 0;
+    for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / kGroup; i < n; i += groups) {
+        const uint32_t meta = Q.meta[i];
+        Counters c{0, 0, 0};
+        photon_estimate_group<S, true>((meta & 1u) ? G1 : G0, f3{Q.px[i], Q.py[i], Q.pz[i]}, f3{Q.nx[i], Q.ny[i], Q.nz[i]},
+                                       (int)(meta >> 1), stack, c);
+        a += c.nodes; b += c.refs; c3 += c.tris;
+    }
+    atomicAdd(&totals[0], (unsigned long long)a);
+    atomicAdd(&totals[1], (unsigned long long)b);
+    atomicAdd(&totals[2], (unsigned long long)c3);
 }
 
 // per-sample radiance (rounded to FP32 like the reference's color) into the

@@ -42,7 +42,7 @@ SHAPE_MESH, SHAPE_RECTANGLE, SHAPE_SPHERE = 0, 1, 2
 LIGHT_AREA, LIGHT_SKY = 0, 1
 GEN_BOX_ICOSPHERE, GEN_DISPLACED_SPHERE, GEN_PHOTON_BOX, GEN_DISPLACED_SPHERE_VARIANT = 0, 1, 2, 3
 PHOTON_SURFACE, PHOTON_CAUSTIC = 0, 1
-KERNEL_CLOSEST, KERNEL_OCCLUDED, KERNEL_SHADE = 0, 1, 2
+KERNEL_CLOSEST, KERNEL_OCCLUDED, KERNEL_SHADE, KERNEL_GATHER = 0, 1, 2, 3
 
 F3 = C.c_float * 3
 
@@ -444,13 +444,16 @@ class Scene:
                                        _p(out)))
         return out
 
-    def render_photon(self, width, height, spp, max_depth=128, seed=1, spp_offset=0, rows=None, sumsq=True):
+    def render_photon(self, width, height, spp, max_depth=128, seed=1, spp_offset=0, rows=None, sumsq=True,
+                      count=False):
         """photon_integrator::radiance per (pixel, sample) with the scene's maps
-        (photon_integrator.h:39-123); returns (sum, sumsq, stats) like render."""
+        (photon_integrator.h:39-123); returns (sum, sumsq, stats) like render.
+        count: stats also count the gathers' work (nodes, photons tested,
+        photons summed)."""
         sum_rgb = np.zeros((height, width, 3), np.float64)
         sumsq_rgb = np.zeros((height, width, 3), np.float64) if sumsq else None
         rb, re = rows if rows is not None else (0, 0)
-        prm = RenderParams(width, height, spp, spp_offset, max_depth, 0, seed, rb, re, 0)
+        prm = RenderParams(width, height, spp, spp_offset, max_depth, 0, seed, rb, re, RENDER_COUNT if count else 0)
         st = RenderStats()
         _check(LIB.lrt_render_photon(self._h, C.byref(prm), _p(sum_rgb, C.c_double),
                                      _p(sumsq_rgb, C.c_double) if sumsq else None, 0, C.byref(st)))
