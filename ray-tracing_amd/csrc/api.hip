@@ -384,6 +384,7 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
     }
     const auto t1 = std::chrono::steady_clock::now();
     if (words.empty()) { words.push_back(0); words.push_back(3u); }
+    const size_t n_nodes = words.size() / 2;
     if (refs.empty()) refs.push_back(0);
     if (tris.empty()) tris.assign(9, 0.0f);
     if (cn.empty()) cn.assign(9, 0.0f);
@@ -492,7 +493,7 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
     const auto t2 = std::chrono::steady_clock::now();
     std::memset(&sc.info, 0, sizeof(sc.info));
     sc.info.n_triangles = tris.size() / 9;
-    sc.info.n_nodes = words.size() / 2;
+    sc.info.n_nodes = n_nodes;
     sc.info.n_refs = refs.size();
     sc.info.device_bytes = sc.nodes.bytes + sc.refs.bytes + sc.tris.bytes + sc.cnormals.bytes + sc.trimat.bytes;
     sc.info.max_tree_depth = max_depth;

@@ -225,7 +225,7 @@ LRT_D f3 diff_sample(const DevScene& S, const DevLights& Ls, const Mats& M, cons
 
 // partial sums of the samples [j c, min(spp, (j + 1) c)) of every pixel:
 // thread t takes pixel t % npix and chunk j = t / npix
-__global__ __launch_bounds__(kBlock) void k_diff_render(const DevScene S, const DevLights Ls, const Mats M,
+__global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_diff_render(const DevScene S, const DevLights Ls, const Mats M,
                                                         const DiffParams D, const CameraDev cam, uint32_t chunk,
                                                         uint32_t nchunks, float* __restrict__ part) {
     __shared__ uint2 stack[kLdsStack * kBlock];
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(kBlock) void k_diff_resolve(const float* __restrict
 
 // gradient of sum(d_img * image) with respect to the parameters; threads over
 // (pixel, sample chunk) like k_diff_render
-__global__ __launch_bounds__(kBlock) void k_diff_adjoint(const DevScene S, const DevLights Ls, const Mats M,
+__global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_diff_adjoint(const DevScene S, const DevLights Ls, const Mats M,
                                                          const DiffParams D, const CameraDev cam, uint32_t chunk,
                                                          uint32_t nchunks, const float* __restrict__ d_img,
                                                          float* __restrict__ grad_tex, float* __restrict__ grad_s) {

@@ -45,6 +45,24 @@ struct DevScene {
 #endif
 // stack entries kept in LDS per lane (the rest spill to private memory), and
 // triangles whose reference and vertex loads are issued together in a leaf
+// waves per SIMD the traversal kernels are compiled to fit (their registers
+// are capped accordingly; 0 = the compiler's choice), and the same for k_shade
+#ifndef LRT_TRACE_WAVES
+#define LRT_TRACE_WAVES 8
+#endif
+#if LRT_TRACE_WAVES > 0
+#define LRT_TRACE_ATTR __attribute__((amdgpu_waves_per_eu(LRT_TRACE_WAVES)))
+#else
+#define LRT_TRACE_ATTR
+#endif
+#ifndef LRT_SHADE_WAVES
+#define LRT_SHADE_WAVES 2
+#endif
+#if LRT_SHADE_WAVES > 0
+#define LRT_SHADE_ATTR __attribute__((amdgpu_waves_per_eu(LRT_SHADE_WAVES)))
+#else
+#define LRT_SHADE_ATTR
+#endif
 constexpr int kLdsStack = LRT_LDS_STACK;
 constexpr int kLeafBatch = LRT_LEAF_BATCH;
 constexpr int kMaxStack = 48;
@@ -121,8 +139,20 @@ struct Counters {
     uint32_t nodes, refs, tris;
 };
 
+#ifndef LRT_WHILE_WHILE
+#define LRT_WHILE_WHILE 1
+#endif
+
+
 // One kd-tree, closest hit.  `closest` enters as the object's t_max and
 // leaves as the nearest accepted t.  ANY: stop at the first accepted hit.
+//
+// The loop descends through interior nodes until the lane reaches a leaf and
+// only then tests the leaf ("while-while"): lanes of a wave that reach their
+// leaves at different depths wait in the cheap descent instead of running the
+// leaf code once per node step.  The visit order is the reference's: the
+// descent neither changes mint nor closest, so the reference's per-node test
+// closest < mint can only change after a pop.
 template <bool ANY, bool COUNT>
 LRT_D bool kd_traverse(const DevScene& S, const DevObject& ob, const Ray& r, float inv[3], float tmin,
                        float& closest, int32_t& prim, float& hu, float& hv, uint2* lds, Counters& cnt) {
@@ -135,73 +165,82 @@ LRT_D bool kd_traverse(const DevScene& S, const DevObject& ob, const Ray& r, flo
     bool found = false;
     for (;;) {
         if (closest < mint) break;
-        const uint2 nd = S.nodes[node];
+        uint2 nd = S.nodes[node];
         if (COUNT) cnt.nodes++;
-        const uint32_t tag = nd.y & 3u;
-        if (tag == 3u) {
-            const uint32_t count = nd.y >> 2;
-            const uint32_t first = nd.x;
-            float limit = fminf(maxt, closest);
-            for (uint32_t k = 0; k < count; k += kLeafBatch) {
-                // issue the reference and vertex loads of a batch together,
-                // then test in reference order (ties go to the later triangle)
-                uint32_t tri[kLeafBatch];
-                TriRec rec[kLeafBatch];
+        uint32_t tag = nd.y & 3u;
+        while (tag != 3u) {
+            const int axis = (int)tag;
+            const float split = __uint_as_float(nd.x);
+            const float oa = comp(r.o, axis), da = comp(r.d, axis);
+            const float ia = axis == 0 ? inv[0] : (axis == 1 ? inv[1] : inv[2]);
+            const float t_plane = (split - oa) * ia;
+            const bool left_first = oa < split || (oa == split && da <= 0.0f);
+            const uint32_t left = node + 1, right = ob.node_base + (nd.y >> 2);
+            const uint32_t first = left_first ? left : right;
+            const uint32_t second = left_first ? right : left;
+            if (t_plane > maxt || t_plane <= 0.0f) {
+                node = first;
+            } else if (t_plane < mint) {
+                node = second;
+            } else {
+                const uint2 e = make_uint2(second, __float_as_uint(t_plane));
+                if (top < kLdsStack) lds[top * stride] = e;
+                else spill[top - kLdsStack] = e;
+                ++top;
+                node = first;
+                maxt = t_plane;
+            }
+#if LRT_WHILE_WHILE
+            nd = S.nodes[node];
+            if (COUNT) cnt.nodes++;
+            tag = nd.y & 3u;
+#else
+            break;
+#endif
+        }
+#if !LRT_WHILE_WHILE
+        if (tag != 3u) continue;
+#endif
+        const uint32_t count = nd.y >> 2;
+        const uint32_t first = nd.x;
+        float limit = fminf(maxt, closest);
+        for (uint32_t k = 0; k < count; k += kLeafBatch) {
+            // issue the reference and vertex loads of a batch together,
+            // then test in reference order (ties go to the later triangle)
+            uint32_t tri[kLeafBatch];
+            TriRec rec[kLeafBatch];
 #pragma unroll
-                for (int j = 0; j < kLeafBatch; ++j)
-                    if (k + j < count) tri[j] = S.refs[first + k + j];
+            for (int j = 0; j < kLeafBatch; ++j)
+                if (k + j < count) tri[j] = S.refs[first + k + j];
 #pragma unroll
-                for (int j = 0; j < kLeafBatch; ++j)
-                    if (k + j < count) rec[j] = load_tri(S.tris + 9 * (size_t)tri[j]);
+            for (int j = 0; j < kLeafBatch; ++j)
+                if (k + j < count) rec[j] = load_tri(S.tris + 9 * (size_t)tri[j]);
 #pragma unroll
-                for (int j = 0; j < kLeafBatch; ++j) {
-                    if (k + j < count) {
-                        if (COUNT) { cnt.refs++; cnt.tris++; }
-                        float t, u, v;
-                        if (tri_test(rec[j], r, mint, limit, t, u, v)) {
-                            closest = t;
-                            prim = (int32_t)(tri[j] - ob.tri_base);
-                            hu = u;
-                            hv = v;
-                            found = true;
-                            if (ANY) return true;
-                        }
-                        limit = fminf(maxt, closest);
+            for (int j = 0; j < kLeafBatch; ++j) {
+                if (k + j < count) {
+                    if (COUNT) { cnt.refs++; cnt.tris++; }
+                    float t, u, v;
+                    if (tri_test(rec[j], r, mint, limit, t, u, v)) {
+                        closest = t;
+                        prim = (int32_t)(tri[j] - ob.tri_base);
+                        hu = u;
+                        hv = v;
+                        found = true;
+                        if (ANY) return true;
                     }
+                    limit = fminf(maxt, closest);
                 }
             }
-            if (top == 0) break;
-            --top;
-            const uint2 e = top < kLdsStack ? lds[top * stride] : spill[top - kLdsStack];
-            node = e.x;
-            mint = __uint_as_float(e.y);
-            if (top == 0) maxt = maxt0;
-            else {
-                const uint2 below = (top - 1) < kLdsStack ? lds[(top - 1) * stride] : spill[top - 1 - kLdsStack];
-                maxt = __uint_as_float(below.y);
-            }
-            continue;
         }
-        const int axis = (int)tag;
-        const float split = __uint_as_float(nd.x);
-        const float oa = comp(r.o, axis), da = comp(r.d, axis);
-        const float ia = axis == 0 ? inv[0] : (axis == 1 ? inv[1] : inv[2]);
-        const float t_plane = (split - oa) * ia;
-        const bool left_first = oa < split || (oa == split && da <= 0.0f);
-        const uint32_t left = node + 1, right = ob.node_base + (nd.y >> 2);
-        const uint32_t first = left_first ? left : right;
-        const uint32_t second = left_first ? right : left;
-        if (t_plane > maxt || t_plane <= 0.0f) {
-            node = first;
-        } else if (t_plane < mint) {
-            node = second;
-        } else {
-            const uint2 e = make_uint2(second, __float_as_uint(t_plane));
-            if (top < kLdsStack) lds[top * stride] = e;
-            else spill[top - kLdsStack] = e;
-            ++top;
-            node = first;
-            maxt = t_plane;
+        if (top == 0) break;
+        --top;
+        const uint2 e = top < kLdsStack ? lds[top * stride] : spill[top - kLdsStack];
+        node = e.x;
+        mint = __uint_as_float(e.y);
+        if (top == 0) maxt = maxt0;
+        else {
+            const uint2 below = (top - 1) < kLdsStack ? lds[(top - 1) * stride] : spill[top - 1 - kLdsStack];
+            maxt = __uint_as_float(below.y);
         }
     }
     return found;

@@ -77,7 +77,7 @@ __global__ __launch_bounds__(kBlock) void k_eye_raygen(const CameraDev cam, uint
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_eye_extend(const DevScene S, const EyeItems It, uint32_t n, HitBuf Hb) {
+__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_eye_extend(const DevScene S, const EyeItems It, uint32_t n, HitBuf Hb) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -106,7 +106,7 @@ LRT_D void eye_push(EyeItems& O, uint32_t s, uint32_t sid, uint32_t path, uint32
 }
 
 // counters: [0] children, [1] shadow rays, [2] photon queries
-__global__ __launch_bounds__(kBlock) void k_eye_shade(const DevScene S, const DevLights Ls, const Mats M,
+__global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_eye_shade(const DevScene S, const DevLights Ls, const Mats M,
                                                       const EyeItems It, uint32_t n, const HitBuf Hb, EyeItems Out,
                                                       EyeShadow Sh, EyeQuery Q, uint32_t* counters, EyeParams prm,
                                                       unsigned long long* __restrict__ acc) {
@@ -300,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void k_eye_shade(const DevScene S, const De
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_eye_shadow(const DevScene S, const EyeShadow Sh, uint32_t n,
+__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_eye_shadow(const DevScene S, const EyeShadow Sh, uint32_t n,
                                                        unsigned long long* __restrict__ acc) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;

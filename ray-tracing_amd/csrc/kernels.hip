@@ -30,7 +30,7 @@ LRT_D uint32_t wave_append(uint32_t* counter, bool want) {
 
 // ---------------------------------------------------------------------------
 // ray queries
-__global__ __launch_bounds__(kBlock) void k_trace_closest(const DevScene S, const float* __restrict__ rays, uint32_t n,
+__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_trace_closest(const DevScene S, const float* __restrict__ rays, uint32_t n,
                                                           int32_t* __restrict__ obj_prim, float* __restrict__ t_out,
                                                           float* __restrict__ uv_out) {
     __shared__ uint2 stack[kLdsStack * kBlock];
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_count(const DevScene S, const 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_trace_occluded(const DevScene S, const float* __restrict__ rays, uint32_t n,
+__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_trace_occluded(const DevScene S, const float* __restrict__ rays, uint32_t n,
                                                            const int32_t* __restrict__ ignore, uint8_t* __restrict__ out) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(PathState P, const CameraDev 
 }
 
 // extension rays of the active queue -> hit records by queue slot
-__global__ __launch_bounds__(kBlock) void k_extend(const DevScene S, const PathState P, const uint32_t* __restrict__ queue,
+__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_extend(const DevScene S, const PathState P, const uint32_t* __restrict__ queue,
                                                    const uint32_t* __restrict__ counters, HitBuf Hb) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;
@@ -241,7 +241,7 @@ LRT_D SurfaceHit surface(const DevScene& S, const Ray& r, int obj, int prim, flo
 }
 
 // one bounce of path_integrator::radiance for every queued path
-__global__ __launch_bounds__(kBlock) void k_shade(const DevScene S, const DevLights Ls, const Mats M, PathState P,
+__global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_shade(const DevScene S, const DevLights Ls, const Mats M, PathState P,
                                                   const uint32_t* __restrict__ queue, uint32_t* __restrict__ next_queue,
                                                   uint32_t* counters, const HitBuf Hb, ShadowBuf Sb, ShadeParams prm) {
     const uint32_t n = counters[0];
@@ -444,7 +444,7 @@ __global__ __launch_bounds__(kBlock) void k_aov(const DevScene S, const Mats M, 
 }
 
 
-__global__ __launch_bounds__(kBlock) void k_shadow(const DevScene S, PathState P, const uint32_t* __restrict__ counters,
+__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_shadow(const DevScene S, PathState P, const uint32_t* __restrict__ counters,
                                                    const ShadowBuf Sb) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;
