@@ -1622,12 +1622,13 @@ static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, doub
                 ep.cap_items = (uint32_t)std::min<size_t>(sc->eye_cap[nxt], 0xffffffffu);
                 ep.cap_shadow = (uint32_t)std::min<size_t>(sc->eye_shadow_cap, 0xffffffffu);
                 ep.cap_query = (uint32_t)std::min<size_t>(sc->eye_query_cap, 0xffffffffu);
-                const uint32_t zero4[4] = {0, 0, 0, 0};
-                HIPCHECK(hipMemcpyAsync(counters, zero4, sizeof(zero4), hipMemcpyHostToDevice, sc->stream));
+                // [0..2] appends of k_eye_shade, [4] / [5] ray fetches of k_eye_extend / k_eye_shadow
+                const uint32_t zero8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+                HIPCHECK(hipMemcpyAsync(counters, zero8, sizeof(zero8), hipMemcpyHostToDevice, sc->stream));
                 const uint32_t g = (uint32_t)std::min<uint64_t>((n + kBlock - 1) / kBlock, (uint64_t)sc->cus * 16);
                 auto e1 = tc.next();
                 HIPCHECK(hipEventRecord(e1.first, sc->stream));
-                hipLaunchKernelGGL(k_eye_extend, dim3(g), dim3(kBlock), 0, sc->stream, sc->S, A, n, Hb);
+                hipLaunchKernelGGL(k_eye_extend, dim3(g), dim3(kBlock), 0, sc->stream, sc->S, A, n, Hb, counters + 4);
                 HIPCHECK(hipEventRecord(e1.second, sc->stream));
                 auto e2 = ts.next();
                 HIPCHECK(hipEventRecord(e2.first, sc->stream));
@@ -1648,7 +1649,7 @@ static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, doub
                     HIPCHECK(hipEventRecord(e3.first, sc->stream));
                     hipLaunchKernelGGL(k_eye_shadow,
                                        dim3((uint32_t)std::min<uint64_t>((cnt[1] + kBlock - 1) / kBlock, (uint64_t)sc->cus * 16)),
-                                       dim3(kBlock), 0, sc->stream, sc->S, Sh, cnt[1], acc);
+                                       dim3(kBlock), 0, sc->stream, sc->S, Sh, cnt[1], acc, counters + 5);
                     HIPCHECK(hipEventRecord(e3.second, sc->stream));
                 }
                 if (cnt[2]) {

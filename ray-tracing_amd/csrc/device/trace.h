@@ -48,7 +48,7 @@ struct DevScene {
 // waves per SIMD the traversal kernels are compiled to fit (their registers
 // are capped accordingly; 0 = the compiler's choice), and the same for k_shade
 #ifndef LRT_TRACE_WAVES
-#define LRT_TRACE_WAVES 8
+#define LRT_TRACE_WAVES 7
 #endif
 #if LRT_TRACE_WAVES > 0
 #define LRT_TRACE_ATTR __attribute__((amdgpu_waves_per_eu(LRT_TRACE_WAVES)))
@@ -303,6 +303,192 @@ LRT_D bool trace_occluded(const DevScene& S, const Ray& r, float tmin, float tma
         }
     }
     return false;
+}
+
+// ---- resumable traversal for persistent kernels ----------------------------
+// A lane's ray is traced in rounds (one descent to a leaf, the leaf's tests
+// and a pop per round) so that a kernel can hand the lane a new ray as soon
+// as its ray is finished instead of waiting for the slowest lane of the wave.
+// The visit order, windows and acceptance rules are those of kd_traverse and
+// trace_closest / trace_occluded.
+
+#ifndef LRT_REFILL
+#define LRT_REFILL 24
+#endif
+constexpr int kRefill = LRT_REFILL;  // idle lanes that make a wave take new rays
+
+struct LaneTrace {
+    Ray r;
+    float inv[3];
+    float tmin, closest;   // closest: nearest accepted t (ANY: the ray's tmax)
+    int32_t obj, prim;     // hit so far (closest hit)
+    float u, v;
+    int32_t o;             // object being traversed
+    int32_t ignore;        // emitter whose surfaces do not occlude (ANY)
+    uint32_t node;         // kd state of object o
+    float mint, maxt, maxt0;
+    int32_t top;
+    bool in_mesh, hit;
+};
+
+LRT_D void lane_trace_start(LaneTrace& T, const Ray& r, float tmin, float tmax, int ignore) {
+    T.r = r;
+    T.inv[0] = 1.0f / r.d.x;
+    T.inv[1] = 1.0f / r.d.y;
+    T.inv[2] = 1.0f / r.d.z;
+    T.tmin = tmin;
+    T.closest = tmax;
+    T.obj = -1;
+    T.prim = -1;
+    T.u = 0.0f;
+    T.v = 0.0f;
+    T.o = 0;
+    T.ignore = ignore;
+    T.in_mesh = false;
+    T.hit = false;
+}
+
+// One round; returns true when the ray is finished (T.hit / T.obj, T.prim,
+// T.closest, T.u, T.v hold the result).
+template <bool ANY>
+LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* spill) {
+    const uint32_t stride = blockDim.x;
+    if (!T.in_mesh) {
+        // the objects in order: analytic shapes at once, a mesh starts a kd traversal
+        for (; T.o < S.n_objects; ++T.o) {
+            const DevObject& ob = S.obj[T.o];
+            if (ANY && ob.emitter >= 0 && ob.emitter == T.ignore) continue;
+            if (ob.shape == kShapeMesh) {
+                T.node = ob.node_base;
+                T.mint = T.tmin;
+                T.maxt = T.closest;
+                T.maxt0 = T.closest;
+                T.top = 0;
+                T.in_mesh = true;
+                break;
+            } else if (ob.shape == kShapeRect) {
+                float t, a, b;
+                if (rect_test(ob, T.r, T.tmin, T.closest, t, a, b)) {
+                    T.hit = true;
+                    if (ANY) return true;
+                    T.closest = t; T.obj = T.o; T.prim = 0; T.u = a; T.v = b;
+                }
+            } else {
+                float t;
+                if (sphere_test(ob, T.r, T.tmin, T.closest, t)) {
+                    T.hit = true;
+                    if (ANY) return true;
+                    T.closest = t; T.obj = T.o; T.prim = 0; T.u = 0.0f; T.v = 0.0f;
+                }
+            }
+        }
+        if (!T.in_mesh) return true;
+    }
+    const DevObject& ob = S.obj[T.o];
+    const Ray& r = T.r;
+    bool done = T.closest < T.mint;
+    if (!done) {
+        uint32_t node = T.node;
+        float maxt = T.maxt;
+        const float mint = T.mint;
+        int top = T.top;
+        uint2 nd = S.nodes[node];
+        uint32_t tag = nd.y & 3u;
+        while (tag != 3u) {
+            const int axis = (int)tag;
+            const float split = __uint_as_float(nd.x);
+            const float oa = comp(r.o, axis), da = comp(r.d, axis);
+            const float ia = axis == 0 ? T.inv[0] : (axis == 1 ? T.inv[1] : T.inv[2]);
+            const float t_plane = (split - oa) * ia;
+            const bool left_first = oa < split || (oa == split && da <= 0.0f);
+            const uint32_t left = node + 1, right = ob.node_base + (nd.y >> 2);
+            const uint32_t first = left_first ? left : right;
+            const uint32_t second = left_first ? right : left;
+            if (t_plane > maxt || t_plane <= 0.0f) {
+                node = first;
+            } else if (t_plane < mint) {
+                node = second;
+            } else {
+                const uint2 e = make_uint2(second, __float_as_uint(t_plane));
+                if (top < kLdsStack) lds[top * stride] = e;
+                else spill[top - kLdsStack] = e;
+                ++top;
+                node = first;
+                maxt = t_plane;
+            }
+            nd = S.nodes[node];
+            tag = nd.y & 3u;
+        }
+        const uint32_t count = nd.y >> 2;
+        const uint32_t firstref = nd.x;
+        float limit = fminf(maxt, T.closest);
+        for (uint32_t k = 0; k < count; k += kLeafBatch) {
+            uint32_t tri[kLeafBatch];
+            TriRec rec[kLeafBatch];
+#pragma unroll
+            for (int j = 0; j < kLeafBatch; ++j)
+                if (k + j < count) tri[j] = S.refs[firstref + k + j];
+#pragma unroll
+            for (int j = 0; j < kLeafBatch; ++j)
+                if (k + j < count) rec[j] = load_tri(S.tris + 9 * (size_t)tri[j]);
+#pragma unroll
+            for (int j = 0; j < kLeafBatch; ++j) {
+                if (k + j < count) {
+                    float t, u, v;
+                    if (tri_test(rec[j], r, mint, limit, t, u, v)) {
+                        T.hit = true;
+                        if (ANY) return true;
+                        T.closest = t;
+                        T.obj = T.o;
+                        T.prim = (int32_t)(tri[j] - ob.tri_base);
+                        T.u = u;
+                        T.v = v;
+                    }
+                    limit = fminf(maxt, T.closest);
+                }
+            }
+        }
+        if (top == 0) {
+            done = true;
+        } else {
+            --top;
+            const uint2 e = top < kLdsStack ? lds[top * stride] : spill[top - kLdsStack];
+            T.node = e.x;
+            T.mint = __uint_as_float(e.y);
+            if (top == 0) T.maxt = T.maxt0;
+            else {
+                const uint2 below = (top - 1) < kLdsStack ? lds[(top - 1) * stride] : spill[top - 1 - kLdsStack];
+                T.maxt = __uint_as_float(below.y);
+            }
+            T.top = top;
+        }
+    }
+    if (done) {  // object o finished: the next round goes on with the following objects
+        T.in_mesh = false;
+        ++T.o;
+    }
+    return false;
+}
+
+// Wave-level work distribution for persistent kernels: when at least kRefill
+// lanes are idle (or all are), the wave takes that many consecutive items
+// from *next with one atomic; returns the lane's new item or UINT32_MAX.
+// `more` turns false once the items are exhausted (uniform over the wave).
+LRT_D uint32_t wave_refill(bool idle_lane, uint32_t* next, uint32_t n, bool& more) {
+    const uint32_t none = 0xffffffffu;
+    if (!more) return none;
+    const uint64_t idle = __ballot(idle_lane);
+    const int nidle = __popcll(idle);
+    if (nidle < kRefill && nidle < (int)__popcll(__ballot(true))) return none;
+    const int lane = (int)(threadIdx.x & 63u);
+    const int leader = __ffsll((long long)idle) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(next, (uint32_t)nidle);
+    base = (uint32_t)__shfl((int)base, leader);
+    if ((uint64_t)base + (uint64_t)nidle >= n) more = false;
+    if (!idle_lane) return none;
+    const uint32_t i = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+    return i < n ? i : none;
 }
 
 }  // namespace lrt

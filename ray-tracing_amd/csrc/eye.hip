@@ -77,19 +77,34 @@ __global__ __launch_bounds__(kBlock) void k_eye_raygen(const CameraDev cam, uint
     }
 }
 
-__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_eye_extend(const DevScene S, const EyeItems It, uint32_t n, HitBuf Hb) {
+// closest hits of the items; persistent (lanes refilled from *next)
+__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_eye_extend(const DevScene S, const EyeItems It, uint32_t n,
+                                                                      HitBuf Hb, uint32_t* __restrict__ next) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const Ray r{f3{It.ox[i], It.oy[i], It.oz[i]}, f3{It.dx[i], It.dy[i], It.dz[i]}};
-        HitInfo h;
-        Counters c{0, 0, 0};
-        const bool any = trace_closest<false>(S, r, 0.001f, kInfF, h, lds, c);
-        Hb.obj[i] = any ? h.obj : -1;
-        Hb.prim[i] = h.prim;
-        Hb.t[i] = h.t;
-        Hb.u[i] = h.u;
-        Hb.v[i] = h.v;
+    uint2 spill[kMaxStack - kLdsStack];
+    LaneTrace T;
+    uint32_t i = 0;
+    bool active = false, more = true;
+    for (;;) {
+        const uint32_t got = wave_refill(!active, next, n, more);
+        if (got != 0xffffffffu) {
+            i = got;
+            lane_trace_start(T, Ray{f3{It.ox[i], It.oy[i], It.oz[i]}, f3{It.dx[i], It.dy[i], It.dz[i]}}, 0.001f, kInfF, -1);
+            active = true;
+        }
+        if (!__ballot(active)) {
+            if (!more) break;
+            continue;
+        }
+        if (active && lane_trace_round<false>(S, T, lds, spill)) {
+            Hb.obj[i] = T.hit ? T.obj : -1;
+            Hb.prim[i] = T.prim;
+            Hb.t[i] = T.closest;
+            Hb.u[i] = T.u;
+            Hb.v[i] = T.v;
+            active = false;
+        }
     }
 }
 
@@ -300,14 +315,33 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_eye_shade(const DevSc
     }
 }
 
+// unoccluded shadow rays add their contributions; persistent (lanes
+// refilled from *next)
 __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_eye_shadow(const DevScene S, const EyeShadow Sh, uint32_t n,
-                                                       unsigned long long* __restrict__ acc) {
+                                                                      unsigned long long* __restrict__ acc,
+                                                                      uint32_t* __restrict__ next) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const Ray r{f3{Sh.ox[i], Sh.oy[i], Sh.oz[i]}, f3{Sh.dx[i], Sh.dy[i], Sh.dz[i]}};
-        if (!trace_occluded(S, r, 0.001f, Sh.tmax[i], Sh.ignore[i], lds))
-            eye_add(acc, Sh.sid[i], f3{Sh.cr[i], Sh.cg[i], Sh.cb[i]});
+    uint2 spill[kMaxStack - kLdsStack];
+    LaneTrace T;
+    uint32_t i = 0;
+    bool active = false, more = true;
+    for (;;) {
+        const uint32_t got = wave_refill(!active, next, n, more);
+        if (got != 0xffffffffu) {
+            i = got;
+            lane_trace_start(T, Ray{f3{Sh.ox[i], Sh.oy[i], Sh.oz[i]}, f3{Sh.dx[i], Sh.dy[i], Sh.dz[i]}}, 0.001f,
+                             Sh.tmax[i], Sh.ignore[i]);
+            active = true;
+        }
+        if (!__ballot(active)) {
+            if (!more) break;
+            continue;
+        }
+        if (active && lane_trace_round<true>(S, T, lds, spill)) {
+            if (!T.hit) eye_add(acc, Sh.sid[i], f3{Sh.cr[i], Sh.cg[i], Sh.cb[i]});
+            active = false;
+        }
     }
 }
 

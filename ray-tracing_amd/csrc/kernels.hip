@@ -139,7 +139,9 @@ __global__ __launch_bounds__(kBlock) void k_raygen(PathState P, const CameraDev 
         counters[0] = n;  // active queue length
         counters[1] = 0;  // next queue
         counters[2] = 0;  // shadow queue
-        counters[3] = 0;  // photon queries
+        counters[3] = 0;
+        counters[4] = 0;  // next extension ray to fetch (k_extend)
+        counters[5] = 0;  // next shadow ray to fetch (k_shadow)
     }
     if (i >= n) return;
     const uint32_t ip = i % P.npix, slot = i / P.npix;
@@ -159,23 +161,39 @@ __global__ __launch_bounds__(kBlock) void k_raygen(PathState P, const CameraDev 
     queue[i] = i;
 }
 
-// extension rays of the active queue -> hit records by queue slot
-__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_extend(const DevScene S, const PathState P, const uint32_t* __restrict__ queue,
-                                                   const uint32_t* __restrict__ counters, HitBuf Hb) {
+// extension rays of the active queue -> hit records by queue slot.
+// Persistent: every lane takes a new ray as soon as its ray is finished (the
+// wave fetches rays for its idle lanes from counters[4]).
+__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_extend(const DevScene S, const PathState P,
+                                                                  const uint32_t* __restrict__ queue,
+                                                                  uint32_t* __restrict__ counters, HitBuf Hb) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;
+    uint2 spill[kMaxStack - kLdsStack];
     const uint32_t n = counters[0];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t p = queue[i];
-        const Ray r{f3{P.ox[p], P.oy[p], P.oz[p]}, f3{P.dx[p], P.dy[p], P.dz[p]}};
-        HitInfo h;
-        Counters c{0, 0, 0};
-        const bool any = trace_closest<false>(S, r, 0.001f, kInfF, h, lds, c);
-        Hb.obj[i] = any ? h.obj : -1;
-        Hb.prim[i] = h.prim;
-        Hb.t[i] = h.t;
-        Hb.u[i] = h.u;
-        Hb.v[i] = h.v;
+    LaneTrace T;
+    uint32_t i = 0;
+    bool active = false, more = true;
+    for (;;) {
+        const uint32_t got = wave_refill(!active, &counters[4], n, more);
+        if (got != 0xffffffffu) {
+            i = got;
+            const uint32_t p = queue[i];
+            lane_trace_start(T, Ray{f3{P.ox[p], P.oy[p], P.oz[p]}, f3{P.dx[p], P.dy[p], P.dz[p]}}, 0.001f, kInfF, -1);
+            active = true;
+        }
+        if (!__ballot(active)) {
+            if (!more) break;
+            continue;
+        }
+        if (active && lane_trace_round<false>(S, T, lds, spill)) {
+            Hb.obj[i] = T.hit ? T.obj : -1;
+            Hb.prim[i] = T.prim;
+            Hb.t[i] = T.closest;
+            Hb.u[i] = T.u;
+            Hb.v[i] = T.v;
+            active = false;
+        }
     }
 }
 
@@ -444,17 +462,36 @@ __global__ __launch_bounds__(kBlock) void k_aov(const DevScene S, const Mats M, 
 }
 
 
-__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_shadow(const DevScene S, PathState P, const uint32_t* __restrict__ counters,
-                                                   const ShadowBuf Sb) {
+// shadow rays: unoccluded ones add their contribution to the path radiance.
+// Persistent like k_extend (rays fetched from counters[5]).
+__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_shadow(const DevScene S, PathState P,
+                                                                  uint32_t* __restrict__ counters, const ShadowBuf Sb) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;
+    uint2 spill[kMaxStack - kLdsStack];
     const uint32_t n = counters[2];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const Ray r{f3{Sb.ox[i], Sb.oy[i], Sb.oz[i]}, f3{Sb.dx[i], Sb.dy[i], Sb.dz[i]}};
-        if (!trace_occluded(S, r, 0.001f, Sb.tmax[i], Sb.ignore[i], lds)) {
-            const uint32_t p = Sb.path[i];
-            const f3 L = f3{P.lr[p], P.lg[p], P.lb[p]} + f3{Sb.cr[i], Sb.cg[i], Sb.cb[i]};
-            P.lr[p] = L.x; P.lg[p] = L.y; P.lb[p] = L.z;
+    LaneTrace T;
+    uint32_t i = 0;
+    bool active = false, more = true;
+    for (;;) {
+        const uint32_t got = wave_refill(!active, &counters[5], n, more);
+        if (got != 0xffffffffu) {
+            i = got;
+            lane_trace_start(T, Ray{f3{Sb.ox[i], Sb.oy[i], Sb.oz[i]}, f3{Sb.dx[i], Sb.dy[i], Sb.dz[i]}}, 0.001f,
+                             Sb.tmax[i], Sb.ignore[i]);
+            active = true;
+        }
+        if (!__ballot(active)) {
+            if (!more) break;
+            continue;
+        }
+        if (active && lane_trace_round<true>(S, T, lds, spill)) {
+            if (!T.hit) {
+                const uint32_t p = Sb.path[i];
+                const f3 L = f3{P.lr[p], P.lg[p], P.lb[p]} + f3{Sb.cr[i], Sb.cg[i], Sb.cb[i]};
+                P.lr[p] = L.x; P.lg[p] = L.y; P.lb[p] = L.z;
+            }
+            active = false;
         }
     }
 }
@@ -468,6 +505,8 @@ __global__ void k_advance(uint32_t* counters, unsigned long long* stats) {
         counters[1] = 0;
         counters[2] = 0;
         counters[3] = 0;
+        counters[4] = 0;
+        counters[5] = 0;
     }
 }
 
