@@ -890,7 +890,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
             sc->pstate.alloc((size_t)n * 14 * 4 + 14 * 256);
             sc->hits.alloc((size_t)n * 5 * 4 + 5 * 256);
             sc->shadow.alloc((size_t)n * 12 * 4 + 12 * 256);
-            sc->queues.alloc((size_t)n * 2 * 4 + 2 * 256);
+            sc->queues.alloc((size_t)n * 3 * 4 + 3 * 256);
             sc->cap = n;
         }
         sc->counters.alloc(256);
@@ -917,6 +917,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         c = sc->queues.as<char>();
         uint32_t* qa = alloc_carve<uint32_t>(c, n);
         uint32_t* qb = alloc_carve<uint32_t>(c, n);
+        uint32_t* order = alloc_carve<uint32_t>(c, n);
         uint32_t* counters = sc->counters.as<uint32_t>();
         unsigned long long* dstats = sc->stats.as<unsigned long long>();
         // accumulation target
@@ -942,6 +943,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         const uint32_t g_all = (n + kBlock - 1) / kBlock;
         const uint32_t g_trace = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc->cus * 20);
         const uint32_t g_shade = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc->cus * 8);
+        const uint32_t g_bins = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc->cus * 4);
         Timer& tc = sc->timers[LRT_KERNEL_CLOSEST];
         Timer& to = sc->timers[LRT_KERNEL_OCCLUDED];
         Timer& ts = sc->timers[LRT_KERNEL_SHADE];
@@ -964,8 +966,10 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
                 ShadeParams sp{depth, prm->max_depth, prm->rr_depth, smp, key};
                 auto e2 = ts.next();
                 HIPCHECK(hipEventRecord(e2.first, sc->stream));
+                hipLaunchKernelGGL(k_shade_bins, dim3(g_bins), dim3(kBlock), 0, sc->stream, sc->S, Hb, counters);
+                hipLaunchKernelGGL(k_shade_order, dim3(g_bins), dim3(kBlock), 0, sc->stream, sc->S, Hb, counters, order);
                 hipLaunchKernelGGL(k_shade, dim3(g_shade), dim3(kBlock), 0, sc->stream, sc->S, sc->Ls, sc->M, P, q, qn,
-                                   counters, Hb, Sb, sp);
+                                   counters, Hb, order, Sb, sp);
                 HIPCHECK(hipEventRecord(e2.second, sc->stream));
                 auto e3 = to.next();
                 HIPCHECK(hipEventRecord(e3.first, sc->stream));

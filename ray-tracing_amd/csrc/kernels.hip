@@ -142,6 +142,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(PathState P, const CameraDev 
         counters[3] = 0;
         counters[4] = 0;  // next extension ray to fetch (k_extend)
         counters[5] = 0;  // next shadow ray to fetch (k_shadow)
+        for (int b = 16; b < 48; ++b) counters[b] = 0;  // shading bins: counts, cursors
     }
     if (i >= n) return;
     const uint32_t ip = i % P.npix, slot = i / P.npix;
@@ -258,21 +259,76 @@ LRT_D SurfaceHit surface(const DevScene& S, const Ray& r, int obj, int prim, flo
     return s;
 }
 
+// Shading order: hit slots grouped by what is hit (escapes, emitter shapes,
+// then one bin per material), so that the lanes of a shading wave run the
+// same BSDF code.  The results do not depend on the order: every path's state
+// is its own and the pixel sums are taken per path afterwards.
+constexpr int kShadeBins = 16;
+
+LRT_D uint32_t shade_bin(const DevScene& S, const HitBuf& Hb, uint32_t i) {
+    const int obj = Hb.obj[i];
+    if (obj < 0) return 0u;
+    const DevObject& ob = S.obj[obj];
+    int m = ob.material;
+    if (ob.shape == kShapeMesh && ob.per_tri_material) m = (int)S.tri_mat[ob.tri_base + (uint32_t)Hb.prim[i]];
+    else if (ob.shape == kShapeRect) m = -1;
+    return m < 0 ? 1u : (uint32_t)min(m + 2, kShadeBins - 1);
+}
+
+// counts per bin into counters[16 + b]
+__global__ __launch_bounds__(kBlock) void k_shade_bins(const DevScene S, const HitBuf Hb, uint32_t* __restrict__ counters) {
+    __shared__ uint32_t cnt[kShadeBins];
+    if (threadIdx.x < kShadeBins) cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t n = counters[0];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        atomicAdd(&cnt[shade_bin(S, Hb, i)], 1u);
+    __syncthreads();
+    if (threadIdx.x < kShadeBins && cnt[threadIdx.x]) atomicAdd(&counters[16 + threadIdx.x], cnt[threadIdx.x]);
+}
+
+// order[]: slots bin by bin.  Each block takes a contiguous chunk of slots and
+// reserves its range in every bin with one atomic (cursors counters[32 + b]).
+__global__ __launch_bounds__(kBlock) void k_shade_order(const DevScene S, const HitBuf Hb, uint32_t* __restrict__ counters,
+                                                        uint32_t* __restrict__ order) {
+    __shared__ uint32_t cnt[kShadeBins], base[kShadeBins];
+    if (threadIdx.x < kShadeBins) cnt[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t n = counters[0];
+    const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t b0 = min(n, blockIdx.x * per), b1 = min(n, b0 + per);
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&cnt[shade_bin(S, Hb, i)], 1u);
+    __syncthreads();
+    if (threadIdx.x < kShadeBins) {
+        uint32_t start = 0;
+        for (int b = 0; b < (int)threadIdx.x; ++b) start += counters[16 + b];
+        base[threadIdx.x] = cnt[threadIdx.x] ? start + atomicAdd(&counters[32 + threadIdx.x], cnt[threadIdx.x]) : 0u;
+        cnt[threadIdx.x] = 0u;
+    }
+    __syncthreads();
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+        const uint32_t b = shade_bin(S, Hb, i);
+        order[base[b] + atomicAdd(&cnt[b], 1u)] = i;
+    }
+}
+
 // one bounce of path_integrator::radiance for every queued path
 __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_shade(const DevScene S, const DevLights Ls, const Mats M, PathState P,
                                                   const uint32_t* __restrict__ queue, uint32_t* __restrict__ next_queue,
-                                                  uint32_t* counters, const HitBuf Hb, ShadowBuf Sb, ShadeParams prm) {
+                                                  uint32_t* counters, const HitBuf Hb, const uint32_t* __restrict__ order,
+                                                  ShadowBuf Sb, ShadeParams prm) {
     const uint32_t n = counters[0];
     const uint32_t stride = gridDim.x * blockDim.x;
     // grid-stride with a wave-uniform trip count so the queue appends see full waves
     const uint32_t n_round = (n + stride - 1) / stride * stride;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_round; j += stride) {
         bool cont = false, shadow = false;
         uint32_t p = 0;
         f3 so, sd, contrib;
         float tmax = 0.0f;
         int ignore = -1;
-        if (i < n) {
+        if (j < n) {
+            const uint32_t i = order[j];  // hit slots grouped by material
             p = queue[i];
             const Ray r{f3{P.ox[p], P.oy[p], P.oz[p]}, f3{P.dx[p], P.dy[p], P.dz[p]}};
             f3 T{P.tr[p], P.tg[p], P.tb[p]};
@@ -508,6 +564,7 @@ __global__ void k_advance(uint32_t* counters, unsigned long long* stats) {
         counters[4] = 0;
         counters[5] = 0;
     }
+    if (blockIdx.x == 0 && threadIdx.x >= 16 && threadIdx.x < 48) counters[threadIdx.x] = 0;  // shading bins
 }
 
 // per pixel, the pass's samples in sample order
