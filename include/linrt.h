@@ -243,9 +243,12 @@ typedef struct lrt_render_params {
     uint32_t rr_depth;     /* path_integrator.h:19  (reference: 5) */
     uint64_t seed;         /* Philox-4x32-10 key */
     uint32_t row_begin, row_end; /* rows [begin, end) of the image; 0,0 = all */
-    uint32_t flags;        /* LRT_RENDER_COUNT: also count traversal work (stats) */
+    uint32_t flags;        /* LRT_RENDER_COUNT: also count traversal work (stats);
+                              LRT_RENDER_REFERENCE_SAMPLER: draw the samples from the
+                              reference's own Sobol sampler (sampler.h, sobol.h) instead of
+                              Philox (seed unused), so each sample follows the reference's path */
 } lrt_render_params;
-enum { LRT_RENDER_COUNT = 1 };
+enum { LRT_RENDER_COUNT = 1, LRT_RENDER_REFERENCE_SAMPLER = 2 };
 
 typedef struct lrt_render_stats {
     uint64_t samples, extension_rays, shadow_rays, segments;

@@ -267,6 +267,7 @@ struct lrt_scene {
     // photon mapping
     PhotonState pm[2];   // global, caustic
     DevBuf eye_items[2], eye_hits, eye_shadow, eye_query, eye_acc, eye_sort, eye_hist;
+    DevBuf sobol;  // the reference sampler's direction numbers (uploaded on first use)
     size_t eye_cap[2] = {0, 0}, eye_hit_cap = 0, eye_shadow_cap = 0, eye_query_cap = 0;
     EmitCdf emit;
     float bounding_radius = 0.0f;
@@ -940,6 +941,15 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         sc->totals.alloc(256);
         HIPCHECK(hipMemcpyAsync(sc->totals.p, zero, sizeof(zero), hipMemcpyHostToDevice, sc->stream));
         const uint2 key = make_uint2((uint32_t)prm->seed, (uint32_t)(prm->seed >> 32));
+        const uint32_t* sobol = nullptr;
+        if (prm->flags & LRT_RENDER_REFERENCE_SAMPLER) {
+            if (!sc->sobol.p) {
+                uint32_t dirs[16 * 32];
+                sobol_directions(dirs);
+                upload(sc->sobol, dirs, sizeof(dirs), sc->stream);
+            }
+            sobol = sc->sobol.as<uint32_t>();
+        }
         const uint32_t g_all = (n + kBlock - 1) / kBlock;
         const uint32_t g_trace = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc->cus * 20);
         const uint32_t g_shade = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc->cus * 8);
@@ -952,7 +962,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
             const uint32_t ns = std::min(slots, prm->spp - s);  // samples in this pass
             const uint32_t np = npix * ns;
             hipLaunchKernelGGL(k_raygen, dim3((np + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, P, sc->cam, W, H, rb,
-                               np, smp, key, qa, counters);
+                               np, smp, key, sobol, qa, counters);
             uint32_t* q = qa;
             uint32_t* qn = qb;
             for (uint32_t depth = 0; depth < prm->max_depth; ++depth) {
@@ -963,7 +973,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
                 HIPCHECK(hipEventRecord(e1.first, sc->stream));
                 hipLaunchKernelGGL(k_extend, dim3(g_trace), dim3(kBlock), 0, sc->stream, sc->S, P, q, counters, Hb);
                 HIPCHECK(hipEventRecord(e1.second, sc->stream));
-                ShadeParams sp{depth, prm->max_depth, prm->rr_depth, smp, key};
+                ShadeParams sp{depth, prm->max_depth, prm->rr_depth, smp, key, sobol};
                 auto e2 = ts.next();
                 HIPCHECK(hipEventRecord(e2.first, sc->stream));
                 hipLaunchKernelGGL(k_shade_bins, dim3(g_bins), dim3(kBlock), 0, sc->stream, sc->S, Hb, counters);

@@ -64,6 +64,29 @@ LRT_D uint4 philox(uint4 c, uint2 k) {
     return c;
 }
 LRT_D float u24(uint32_t x) { return (float)(x >> 8) * (1.0f / 16777216.0f); }
+
+// The reference's sampler (sampler.h:12-32, sobol.h:12-41): dimension d of
+// sample `index` is the Gray-code Sobol value of that dimension (a hash
+// beyond the table's 16 dimensions) XOR a scramble of (pixel, d), scaled by
+// 2^-32.  dirs: the 16 x 32 direction numbers; pixel_seed = mix_bits(pixel).
+LRT_D uint32_t mix_bits(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+LRT_D float ref_sample(const uint32_t* __restrict__ dirs, uint32_t pixel_seed, uint32_t index, uint32_t d) {
+    uint32_t bits = 0;
+    if (d >= 16u) {
+        bits = mix_bits(index + d * 0x9e3779b9u);
+    } else {
+        for (uint32_t g = index ^ (index >> 1); g; g &= g - 1u) bits ^= dirs[d * 32u + (uint32_t)__builtin_ctz(g)];
+    }
+    bits ^= mix_bits(pixel_seed + d * 0x9e3779b9u);
+    return (float)bits * (1.0f / 4294967296.0f);
+}
 LRT_D uint32_t pick(uint4 v, uint32_t lane) { return lane == 0 ? v.x : (lane == 1 ? v.y : (lane == 2 ? v.z : v.w)); }
 
 }  // namespace lrt

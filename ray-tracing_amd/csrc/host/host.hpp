@@ -70,6 +70,10 @@ void photon_kd_build(const float* xyz, uint32_t n, PhotonKd& out, int threads = 
 // Philox-4x32-10 on the host
 void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+// direction numbers of the reference sampler's 16-dimensional Gray-code Sobol
+// sequence (sobol.h:43-106): out[d * 32 + b] for dimension d, bit b
+void sobol_directions(uint32_t out[16 * 32]);
+
 void set_error(const std::string& msg);
 
 }  // namespace lrt

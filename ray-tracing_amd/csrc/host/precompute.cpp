@@ -283,4 +283,30 @@ void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+// Dimension 1 is van der Corput; dimensions 2..16 follow the Joe-Kuo
+// recurrence m_i = 2 a_1 m_{i-1} ^ 4 a_2 m_{i-2} ^ ... ^ 2^s m_{i-s} ^ m_{i-s}
+// from their primitive polynomials (degree s, coefficients a, initial m),
+// the parameters the reference's table is built from (sobol.h:43-106).
+void sobol_directions(uint32_t out[16 * 32]) {
+    static const uint32_t prim[15][8] = {
+        {1, 0, 1}, {2, 1, 1, 3}, {3, 1, 1, 3, 1}, {3, 2, 1, 1, 1}, {4, 1, 1, 1, 3, 3},
+        {4, 4, 1, 3, 5, 13}, {5, 2, 1, 1, 5, 5, 17}, {5, 4, 1, 1, 5, 5, 5}, {5, 7, 1, 1, 7, 11, 19},
+        {5, 11, 1, 1, 5, 1, 1}, {5, 13, 1, 1, 1, 3, 11}, {5, 14, 1, 3, 5, 5, 13},
+        {6, 1, 1, 3, 3, 9, 7, 49}, {6, 13, 1, 1, 1, 15, 21, 21}, {6, 16, 1, 3, 1, 13, 27, 49}};
+    for (int b = 0; b < 32; ++b) out[b] = 1u << (31 - b);
+    for (int d = 0; d < 15; ++d) {
+        const uint32_t deg = prim[d][0], a = prim[d][1];
+        uint32_t m[33] = {};
+        for (uint32_t i = 1; i <= deg; ++i) m[i] = prim[d][1 + i];
+        for (int i = (int)deg + 1; i <= 32; ++i) {
+            uint32_t x = (m[i - deg] << deg) ^ m[i - deg];
+            for (int k = 1; k < (int)deg; ++k)
+                if ((a >> (deg - 1 - k)) & 1u) x ^= m[i - k] << k;
+            if (i < 32) x &= (1u << i) - 1u;
+            m[i] = x;
+        }
+        for (int i = 1; i <= 32; ++i) out[(d + 1) * 32 + (i - 1)] = m[i] << (32 - i);
+    }
+}
+
 }  // namespace lrt

@@ -80,6 +80,7 @@ struct EyeParams {
 struct ShadeParams {
     uint32_t depth, max_depth, rr_depth, sample;
     uint2 key;
+    const uint32_t* sobol;  // non-null: the reference's Sobol sampler (direction numbers), else Philox
 };
 
 }  // namespace lrt

@@ -132,8 +132,8 @@ LRT_D float dim_u(uint32_t pix, uint32_t smp, uint32_t d, uint2 key) {
 }
 
 __global__ __launch_bounds__(kBlock) void k_raygen(PathState P, const CameraDev cam, uint32_t W, uint32_t H, uint32_t row0,
-                                                   uint32_t n, uint32_t smp, uint2 key, uint32_t* queue,
-                                                   uint32_t* counters) {
+                                                   uint32_t n, uint32_t smp, uint2 key, const uint32_t* __restrict__ sobol,
+                                                   uint32_t* queue, uint32_t* counters) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
         counters[0] = n;  // active queue length
@@ -148,8 +148,16 @@ __global__ __launch_bounds__(kBlock) void k_raygen(PathState P, const CameraDev 
     const uint32_t ip = i % P.npix, slot = i / P.npix;
     const uint32_t x = ip % W, y = row0 + ip / W;
     const uint32_t pix = y * W + x;
-    const uint4 v = philox(make_uint4(pix, smp + slot, 0u, 0u), key);
-    const float jx = u24(v.x), jy = u24(v.y);
+    float jx, jy;
+    if (sobol) {  // next_2d of the reference: (dimension 1, dimension 0), see k_shade
+        const uint32_t ps = mix_bits(pix);
+        jx = ref_sample(sobol, ps, smp + slot, 1u);
+        jy = ref_sample(sobol, ps, smp + slot, 0u);
+    } else {
+        const uint4 v = philox(make_uint4(pix, smp + slot, 0u, 0u), key);
+        jx = u24(v.x);
+        jy = u24(v.y);
+    }
     const float u = ((float)x + jx) / (float)(W - 1);
     const float vv = ((float)y + jy) / (float)(H - 1);
     const f3 d = normalize(ld3(cam.llc) + u * ld3(cam.horizontal) + vv * ld3(cam.vertical) - ld3(cam.origin));
@@ -368,13 +376,26 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_shade(const DevScene 
                     // six sampler dimensions per bounce (path_integrator.h:80-83)
                     const uint32_t pix = P.pixel0 + p % P.npix, smp = prm.sample + p / P.npix;
                     const uint32_t d0 = 2u + 6u * prm.depth;
-                    const uint4 A = philox(make_uint4(pix, smp, d0 >> 2, 0u), prm.key);
-                    const uint4 B = philox(make_uint4(pix, smp, (d0 >> 2) + 1u, 0u), prm.key);
-                    float u6[6];
+                    float u6[6];  // light uv, lobe, BSDF uv, roulette
+                    if (prm.sobol) {
+                        // sampler::next_2d is vec2(next_1d(), next_1d()) (sampler.h:27-29),
+                        // whose arguments the compiled reference evaluates right to
+                        // left: a 2D sample is (dimension d + 1, dimension d)
+                        const uint32_t ps = mix_bits(pix);
+                        u6[0] = ref_sample(prm.sobol, ps, smp, d0 + 1u);
+                        u6[1] = ref_sample(prm.sobol, ps, smp, d0);
+                        u6[2] = ref_sample(prm.sobol, ps, smp, d0 + 2u);
+                        u6[3] = ref_sample(prm.sobol, ps, smp, d0 + 4u);
+                        u6[4] = ref_sample(prm.sobol, ps, smp, d0 + 3u);
+                        u6[5] = ref_sample(prm.sobol, ps, smp, d0 + 5u);
+                    } else {
+                        const uint4 A = philox(make_uint4(pix, smp, d0 >> 2, 0u), prm.key);
+                        const uint4 B = philox(make_uint4(pix, smp, (d0 >> 2) + 1u, 0u), prm.key);
 #pragma unroll
-                    for (int k = 0; k < 6; ++k) {
-                        const uint32_t d = d0 + (uint32_t)k;
-                        u6[k] = u24((d >> 2) == (d0 >> 2) ? pick(A, d & 3u) : pick(B, d & 3u));
+                        for (int k = 0; k < 6; ++k) {
+                            const uint32_t d = d0 + (uint32_t)k;
+                            u6[k] = u24((d >> 2) == (d0 >> 2) ? pick(A, d & 3u) : pick(B, d & 3u));
+                        }
                     }
                     ctx.lobe = u6[2];
                     const uint32_t fl = bsdf_flags(M, sh.material);

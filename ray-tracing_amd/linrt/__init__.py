@@ -92,6 +92,7 @@ class RenderParams(C.Structure):
                 ("row_begin", C.c_uint32), ("row_end", C.c_uint32), ("flags", C.c_uint32)]
 
 RENDER_COUNT = 1
+RENDER_REFERENCE_SAMPLER = 2
 
 
 class RenderStats(C.Structure):
@@ -373,15 +374,17 @@ class Scene:
         return ms.value, nl.value, it.value
 
     def render(self, width, height, spp, max_depth, rr_depth, seed=1, spp_offset=0, rows=None, sumsq=True,
-               sum_rgb=None, sumsq_rgb=None):
+               sum_rgb=None, sumsq_rgb=None, reference_sampler=False):
         """Adds spp samples per pixel into (sum, sumsq) float64 buffers [H, W, 3]
-        (row 0 = bottom, like src/main.cpp:122) and returns them with stats."""
+        (row 0 = bottom, like src/main.cpp:122) and returns them with stats.
+        reference_sampler: the reference's Sobol sampler instead of Philox."""
         if sum_rgb is None:
             sum_rgb = np.zeros((height, width, 3), np.float64)
         if sumsq and sumsq_rgb is None:
             sumsq_rgb = np.zeros((height, width, 3), np.float64)
         rb, re = rows if rows is not None else (0, 0)
-        prm = RenderParams(width, height, spp, spp_offset, max_depth, rr_depth, seed, rb, re, 0)
+        prm = RenderParams(width, height, spp, spp_offset, max_depth, rr_depth, seed, rb, re,
+                           RENDER_REFERENCE_SAMPLER if reference_sampler else 0)
         st = RenderStats()
         _check(LIB.lrt_render(self._h, C.byref(prm), _p(sum_rgb, C.c_double),
                               _p(sumsq_rgb, C.c_double) if sumsq else None, 0, C.byref(st)))
