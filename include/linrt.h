@@ -331,8 +331,31 @@ typedef struct lrt_gen_params {
 } lrt_gen_params;
 int lrt_generate_scene(const lrt_gen_params* params, const float* tables, uint32_t n_tables,
                        const lrt_light_desc* sky, lrt_generated** out);
+/* the scene of src/main.cpp:55-87 (the README renders) around the mesh of an
+ * OBJ file (load_obj, obj_loader.h:44-141): the mesh with materials[0] (the
+ * others are materials it nests), the ground sphere (material n_materials),
+ * the rectangle light and the camera of main.cpp; tables: the
+ * rough-transmittance slices the materials need.  LRT_ERR_ARG when load_obj
+ * fails. */
+int lrt_generate_obj_scene(const char* obj_path, const lrt_material* materials, uint32_t n_materials,
+                           const float* tables, uint32_t n_tables, float aspect, lrt_generated** out);
 const lrt_scene_desc* lrt_generated_desc(const lrt_generated* g);
 int lrt_generated_free(lrt_generated* g);
+
+/* ---- OBJ meshes (obj_loader.h:10-141) ------------------------------------
+ * lrt_load_obj reads "v", "vn" and "f" records (indices 1-based or negative,
+ * corners p, p/t, p/t/n, p//n), fans polygons around their first corner,
+ * drops zero-area triangles, gives each corner the file's normal or the face
+ * normal, and centres and scales the mesh to a unit longest bounding-box
+ * side, as the reference's load_obj.  Where that returns nullptr (file not
+ * readable, index out of range, a face of fewer than 3 corners, no triangle)
+ * or would throw (an unparsable index) the call returns LRT_ERR_ARG.
+ * lrt_mesh_view's arrays stay valid until lrt_mesh_free. */
+typedef struct lrt_mesh lrt_mesh;
+int lrt_load_obj(const char* path, lrt_mesh** out);
+int lrt_mesh_view(const lrt_mesh* mesh, uint32_t* n_vertices, uint32_t* n_triangles, const float** positions,
+                  const uint32_t** indices, const float** corner_normals);
+int lrt_mesh_free(lrt_mesh* mesh);
 
 /* ---- Philox-4x32-10 (the counter-based sampler of every kernel) -------- */
 int lrt_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -23,6 +23,7 @@
 #include "integrator/photonmapper/photon_integrator.h"
 #include "integrator/photonmapper/photon_map.h"
 #include "integrator/photonmapper/photon_tracer.h"
+#include "io/obj_loader.h"
 #include "material/coating_bsdf.h"
 #include "material/dielectric_bsdf.h"
 #include "material/diffuse_bsdf.h"
@@ -126,6 +127,28 @@ void parallel_for(uint32_t n, int threads, F&& f) {
 }  // namespace
 
 extern "C" {
+
+// LinRT::load_obj (obj_loader.h:44-141): returns 0 and the counts (and the
+// arrays where the pointers are not null), or 1 where it returns nullptr
+int ref_load_obj(const char* path, uint32_t* n_vertices, uint32_t* n_corners, float* positions, uint32_t* indices,
+                 float* corner_normals) {
+    const auto m = LinRT::load_obj(path);
+    if (!m) return 1;
+    *n_vertices = (uint32_t)m->positions.size();
+    *n_corners = (uint32_t)m->indices.size();
+    for (size_t i = 0; positions && i < m->positions.size(); ++i) {
+        positions[3 * i] = m->positions[i].x;
+        positions[3 * i + 1] = m->positions[i].y;
+        positions[3 * i + 2] = m->positions[i].z;
+    }
+    for (size_t i = 0; indices && i < m->indices.size(); ++i) indices[i] = m->indices[i];
+    for (size_t i = 0; corner_normals && i < m->corner_normals.size(); ++i) {
+        corner_normals[3 * i] = m->corner_normals[i].x;
+        corner_normals[3 * i + 1] = m->corner_normals[i].y;
+        corner_normals[3 * i + 2] = m->corner_normals[i].z;
+    }
+    return 0;
+}
 
 int ref_set_root(const char* root) {
     g_root = root;

@@ -34,6 +34,8 @@ namespace lrt {
 struct Generated;
 Generated* generate(const lrt_gen_params& p, const float* tables, uint32_t n_tables, const lrt_light_desc* sky);
 void free_generated(Generated* g);
+Generated* generate_obj_scene(const char* path, const lrt_material* materials, uint32_t n_materials,
+                              const float* tables, uint32_t n_tables, float aspect);
 const lrt_scene_desc* generated_desc(const Generated* g);
 
 namespace {
@@ -1748,9 +1750,56 @@ int lrt_generate_scene(const lrt_gen_params* params, const float* tables, uint32
         *out = (lrt_generated*)generate(*params, tables, n_tables, sky);
     });
 }
+int lrt_generate_obj_scene(const char* obj_path, const lrt_material* materials, uint32_t n_materials,
+                           const float* tables, uint32_t n_tables, float aspect, lrt_generated** out) {
+    return guarded([&] {
+        require(obj_path && materials && out, "null pointer");
+        require(n_materials >= 1, "the mesh needs a material");
+        *out = nullptr;
+        try {
+            *out = (lrt_generated*)generate_obj_scene(obj_path, materials, n_materials, tables, n_tables, aspect);
+        } catch (const std::invalid_argument& e) {
+            throw Error(LRT_ERR_ARG, e.what());
+        }
+    });
+}
 const lrt_scene_desc* lrt_generated_desc(const lrt_generated* g) { return g ? generated_desc((const Generated*)g) : nullptr; }
 int lrt_generated_free(lrt_generated* g) {
     free_generated((Generated*)g);
+    return LRT_OK;
+}
+
+// ---- OBJ meshes ----------------------------------------------------------
+struct lrt_mesh {
+    std::vector<float> positions, corner_normals;
+    std::vector<uint32_t> indices;
+};
+
+int lrt_load_obj(const char* path, lrt_mesh** out) {
+    return guarded([&] {
+        require(path && out, "null pointer");
+        *out = nullptr;
+        auto m = std::make_unique<lrt_mesh>();
+        std::string err;
+        if (!load_obj(path, m->positions, m->indices, m->corner_normals, err)) throw Error(LRT_ERR_ARG, "load_obj: " + err);
+        *out = m.release();
+    });
+}
+
+int lrt_mesh_view(const lrt_mesh* m, uint32_t* n_vertices, uint32_t* n_triangles, const float** positions,
+                  const uint32_t** indices, const float** corner_normals) {
+    return guarded([&] {
+        require(m && n_vertices && n_triangles, "null pointer");
+        *n_vertices = (uint32_t)(m->positions.size() / 3);
+        *n_triangles = (uint32_t)(m->indices.size() / 3);
+        if (positions) *positions = m->positions.data();
+        if (indices) *indices = m->indices.data();
+        if (corner_normals) *corner_normals = m->corner_normals.data();
+    });
+}
+
+int lrt_mesh_free(lrt_mesh* m) {
+    delete m;
     return LRT_OK;
 }
 

@@ -70,6 +70,13 @@ void photon_kd_build(const float* xyz, uint32_t n, PhotonKd& out, int threads = 
 // Philox-4x32-10 on the host
 void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 
+// load_obj (obj_loader.h:10-141): positions (3 per vertex, centred and scaled
+// to a unit longest side), indices (3 per triangle), corner normals (9 per
+// triangle); false with a message where the reference returns nullptr or
+// would throw
+bool load_obj(const char* path, std::vector<float>& positions, std::vector<uint32_t>& indices,
+              std::vector<float>& corner_normals, std::string& error);
+
 // direction numbers of the reference sampler's 16-dimensional Gray-code Sobol
 // sequence (sobol.h:43-106): out[d * 32 + b] for dimension d, bit b
 void sobol_directions(uint32_t out[16 * 32]);

@@ -436,6 +436,79 @@ Generated* generate(const lrt_gen_params& p, const float* tables, uint32_t n_tab
     return g.release();
 }
 
+// The scene of src/main.cpp:55-87 around a mesh loaded from an OBJ file:
+// the mesh with materials[0] (the other given materials are those it nests),
+// a ground sphere of radius 100 at (0, -100.5, 0) with diffuse albedo
+// (0.8, 0.8, 0) (the material after the given ones), a 1.2 x 1.2 rectangle
+// light of radiance 12 at y = 1.5 facing down, and the pinhole camera at
+// (0, 0.5, 3) looking at the origin, 60 degrees.
+Generated* generate_obj_scene(const char* path, const lrt_material* materials, uint32_t n_materials,
+                              const float* tables, uint32_t n_tables, float aspect) {
+    auto g = std::make_unique<Generated>();
+    std::memset(&g->desc, 0, sizeof(g->desc));
+    if (tables && n_tables) g->tables.assign(tables, tables + (size_t)n_tables * 101);
+    g->meshes.resize(1);
+    Mesh& m = g->meshes[0];
+    std::string err;
+    if (!load_obj(path, m.pos, m.idx, m.cn, err)) throw std::invalid_argument("load_obj: " + err);
+    g->materials.assign(materials, materials + n_materials);
+    lrt_material ground = blank();
+    ground.type = LRT_BSDF_DIFFUSE;
+    ground.diffuse[0] = 0.8f;
+    ground.diffuse[1] = 0.8f;
+    ground.diffuse[2] = 0.0f;
+    g->materials.push_back(ground);
+    lrt_object_desc o;
+    std::memset(&o, 0, sizeof(o));
+    o.shape = LRT_SHAPE_MESH;
+    o.material = 0;
+    o.emitter = -1;
+    o.n_vertices = (uint32_t)(m.pos.size() / 3);
+    o.n_triangles = (uint32_t)(m.idx.size() / 3);
+    o.positions = m.pos.data();
+    o.indices = m.idx.data();
+    o.corner_normals = m.cn.data();
+    g->objects.push_back(o);
+    std::memset(&o, 0, sizeof(o));
+    o.shape = LRT_SHAPE_SPHERE;
+    o.material = (int32_t)n_materials;
+    o.emitter = -1;
+    o.center[1] = -100.5f;
+    o.radius = 100.0f;
+    g->objects.push_back(o);
+    std::memset(&o, 0, sizeof(o));
+    o.shape = LRT_SHAPE_RECTANGLE;
+    o.material = -1;
+    o.emitter = 0;
+    const float corner[3] = {-0.6f, 1.5f, -0.6f}, eu[3] = {1.2f, 0.0f, 0.0f}, ev[3] = {0.0f, 0.0f, 1.2f};
+    std::memcpy(o.corner, corner, 12);
+    std::memcpy(o.edge_u, eu, 12);
+    std::memcpy(o.edge_v, ev, 12);
+    g->objects.push_back(o);
+    lrt_light_desc ld;
+    std::memset(&ld, 0, sizeof(ld));
+    ld.kind = LRT_LIGHT_AREA;
+    ld.object = 2;
+    ld.radiance[0] = ld.radiance[1] = ld.radiance[2] = 12.0f;
+    g->lights.push_back(ld);
+    lrt_camera_desc& cam = g->desc.camera;
+    const float org[3] = {0.0f, 0.5f, 3.0f}, la[3] = {0.0f, 0.0f, 0.0f}, up[3] = {0.0f, 1.0f, 0.0f};
+    std::memcpy(cam.origin, org, 12);
+    std::memcpy(cam.look_at, la, 12);
+    std::memcpy(cam.up, up, 12);
+    cam.fov_deg = 60.0f;
+    cam.aspect = aspect > 0.0f ? aspect : 1.0f;
+    g->desc.n_objects = (uint32_t)g->objects.size();
+    g->desc.objects = g->objects.data();
+    g->desc.n_materials = (uint32_t)g->materials.size();
+    g->desc.materials = g->materials.data();
+    g->desc.n_lights = (uint32_t)g->lights.size();
+    g->desc.lights = g->lights.data();
+    g->desc.n_tables = (uint32_t)(g->tables.size() / 101);
+    g->desc.tables = g->tables.empty() ? nullptr : g->tables.data();
+    return g.release();
+}
+
 void free_generated(Generated* g) { delete g; }
 const lrt_scene_desc* generated_desc(const Generated* g) { return &g->desc; }
 

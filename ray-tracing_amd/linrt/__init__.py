@@ -154,6 +154,7 @@ EXPORTED = [
     "lrt_denoiser_create", "lrt_denoiser_release", "lrt_denoiser_param_count", "lrt_denoiser_params",
     "lrt_denoiser_forward", "lrt_denoiser_loss_grad", "lrt_denoiser_adam_step", "lrt_conv3x3",
     "lrt_diff_param_count", "lrt_diff_render", "lrt_diff_adjoint", "lrt_diff_render_held",
+    "lrt_load_obj", "lrt_mesh_view", "lrt_mesh_free", "lrt_generate_obj_scene",
 ]
 
 
@@ -265,6 +266,78 @@ class Generated:
         pos = np.ctypeslib.as_array(o.positions, (o.n_vertices * 3,)).reshape(-1, 3).copy()
         idx = np.ctypeslib.as_array(o.indices, (o.n_triangles * 3,)).reshape(-1, 3).copy()
         return pos, idx
+
+
+class ObjScene(Generated):
+    """The scene of src/main.cpp:55-87 around the mesh of an OBJ file
+    (lrt_generate_obj_scene): materials[0] on the mesh, the others nested by
+    it; the ground sphere takes the next material index."""
+
+    def __init__(self, path, materials, tables=None, aspect=1.0):
+        self._tables = None if tables is None else np.ascontiguousarray(tables, np.float32)
+        self._mats = (Material * len(materials))(*materials)
+        h = C.c_void_p()
+        nt = 0 if self._tables is None else self._tables.size // 101
+        self._h = None
+        _check(LIB.lrt_generate_obj_scene(str(path).encode(), self._mats, C.c_uint32(len(materials)),
+                                          _p(self._tables), C.c_uint32(nt), C.c_float(aspect), C.byref(h)))
+        self._h = h
+        self.desc_ptr = LIB.lrt_generated_desc(h)
+        self.desc = self.desc_ptr.contents
+
+
+def load_obj(path):
+    """load_obj (obj_loader.h:44-141) through lrt_load_obj: positions [V, 3],
+    indices [T, 3], corner normals [T, 3, 3]."""
+    h = C.c_void_p()
+    _check(LIB.lrt_load_obj(str(path).encode(), C.byref(h)))
+    try:
+        nv, nt = C.c_uint32(), C.c_uint32()
+        pp, ip, npp = C.POINTER(C.c_float)(), C.POINTER(C.c_uint32)(), C.POINTER(C.c_float)()
+        _check(LIB.lrt_mesh_view(h, C.byref(nv), C.byref(nt), C.byref(pp), C.byref(ip), C.byref(npp)))
+        pos = np.ctypeslib.as_array(pp, (nv.value * 3,)).reshape(-1, 3).copy()
+        idx = np.ctypeslib.as_array(ip, (nt.value * 3,)).reshape(-1, 3).copy()
+        cn = np.ctypeslib.as_array(npp, (nt.value * 9,)).reshape(-1, 3, 3).copy()
+    finally:
+        LIB.lrt_mesh_free(h)
+    return pos, idx, cn
+
+
+# the bunny materials of src/main.cpp:38-47, by the names of the README renders
+README_MATERIALS = ("diffuse", "rough-conductor", "plastic", "rough-plastic", "coating", "rough-coating",
+                    "rough-dielectric")
+
+
+def readme_materials(name):
+    """(materials, tables) of one README render: the mesh material first, the
+    materials it nests after it; tables from data/tables_readme.json."""
+    tables, _ = load_tables("tables_readme.json")
+    f32 = np.float32
+    eta_p = float(f32(1.49) / f32(1.000277))
+    plastic = material(BSDF_PLASTIC, int_ior=1.49, ext_ior=1.000277, diffuse=(0.8, 0.8, 0.5),
+                       fdr_int=fresnel_diffuse_reflectance(float(f32(1.0) / f32(eta_p))))
+    rough_glass = material(BSDF_ROUGH_DIELECTRIC, int_ior=1.5, ext_ior=1.0, alpha=0.05)
+    if name == "diffuse":
+        mats = [material(BSDF_DIFFUSE, diffuse=(0.3, 0.6, 0.6))]
+    elif name == "rough-conductor":
+        mats = [material(BSDF_CONDUCTOR, alpha=0.3, cond_eta=(0.200438, 0.924033, 1.102212),
+                         cond_k=(3.912948, 2.452847, 2.142188))]
+    elif name == "plastic":
+        mats = [plastic]
+    elif name == "rough-plastic":
+        mats = [material(BSDF_ROUGH_PLASTIC, int_ior=1.49, ext_ior=1.000277, alpha=0.1, diffuse=(0.8, 0.1, 0.05),
+                         table_ext=0, table_int=1)]
+    elif name == "coating":
+        mats = [material(BSDF_COATING, int_ior=1.7, ext_ior=1.000277, nested=1, specular_reflectance=(0.3, 0.6, 0.6)),
+                plastic]
+    elif name == "rough-coating":
+        mats = [material(BSDF_ROUGH_COATING, int_ior=1.7, ext_ior=1.000277, alpha=0.3, nested=1, table_ext=2),
+                rough_glass]
+    elif name == "rough-dielectric":
+        mats = [rough_glass]
+    else:
+        raise ValueError(f"unknown material {name!r}: one of {README_MATERIALS}")
+    return mats, tables
 
 
 def box_icosphere(seed=1, subdivisions=5, aspect=1.0):

@@ -150,3 +150,43 @@ def test_reference_sampler_per_sample_vs_reference():
     # and the Philox default gives another estimate of the same image
     sm, sq, _ = sc.render(16, 16, 64, 50, 5)
     assert abs(sm.mean() / 64 - want.mean()) < 0.1 * want.mean()
+
+
+def _write_obj(path, pos, idx):
+    with open(path, "w") as f:
+        for p in pos:
+            f.write(f"v {p[0]:.9g} {p[1]:.9g} {p[2]:.9g}\n")
+        for t in idx:
+            f.write(f"f {t[0] + 1} {t[1] + 1} {t[2] + 1}\n")
+
+
+@pytest.mark.parametrize("name", linrt.README_MATERIALS)
+def test_readme_scene_vs_oracle(tmp_path, name):
+    """The README scene of src/main.cpp (lrt_generate_obj_scene) around an
+    icosphere written as OBJ, in each README material: the GPU path tracer
+    against the oracle (Philox streams: z-test and closeness), and with the
+    reference sampler every GPU sample against the oracle's reference-sampler
+    sample (which the CPU tests pin to the reference itself on the bunny)."""
+    pos, idx = linrt.box_icosphere(seed=1, subdivisions=3).mesh(1)
+    obj = tmp_path / "ico.obj"
+    _write_obj(obj, pos, idx)
+    mats, tables = linrt.readme_materials(name)
+    g = linrt.ObjScene(obj, mats, tables)
+    sc = linrt.Scene(g.desc_ptr)
+    W = H = 32
+    spp = 16
+    s_g, q_g, st = sc.render(W, H, spp, 16, 5, seed=5)
+    o = oracle.scene(g.desc_ptr).render(W, H, spp, 16, 5, seed=5, mode=0)
+    z = z_scores(s_g, q_g, spp, o["sum"], o["sumsq"], spp)
+    close = np.isclose(s_g, o["sum"], rtol=1e-4, atol=1e-6).mean()
+    print(f"README scene {name}: equal to 1e-4 {close:.4f}")
+    check_z(z, f"readme {name}")
+    assert close > 0.95
+    ps = oracle.scene(g.desc_ptr).render(16, 16, 2, 50, 5, mode=1, per_sample=True)["per_sample"]
+    got = np.zeros_like(ps)
+    for s in range(2):
+        sm, _, _ = sc.render(16, 16, 1, 50, 5, spp_offset=s, reference_sampler=True)
+        got[:, :, s, :] = sm.astype(np.float32)
+    within = (np.abs(got - ps) <= 1e-4 * np.abs(ps) + 1e-6).mean()
+    print(f"README scene {name}: reference-sampler samples within 1e-4 {within:.4f}, exact {(got == ps).mean():.4f}")
+    assert within >= 0.97
