@@ -357,6 +357,22 @@ int lrt_mesh_view(const lrt_mesh* mesh, uint32_t* n_vertices, uint32_t* n_triang
                   const uint32_t** indices, const float** corner_normals);
 int lrt_mesh_free(lrt_mesh* mesh);
 
+/* ---- image files (image_write.cpp:10-100) ---------------------------------
+ * rgb: width x height float RGB, row 0 first.  8-bit formats map each channel
+ * as the reference's to_byte (NaN -> 0, pow(v, 1/2.2), clamp to [0, 0.999],
+ * x 256, truncated); PNG rows are flipped when asked (src/main.cpp:151 saves
+ * the bottom-up frame flipped) and hold the same pixels as the reference's
+ * (stored deflate blocks instead of stb's compression); PPM (its plain P3)
+ * and HDR (Radiance RGBE) keep the rows in memory order, as the reference's
+ * writer does; JPEG (lossy) is not written.  Lossless additions: PFM and
+ * raw little-endian float32 with a JSON sidecar (path + ".json"), rows
+ * bottom first (flipped first when asked). */
+enum { LRT_IMAGE_PNG = 0, LRT_IMAGE_JPEG = 1, LRT_IMAGE_PPM = 2, LRT_IMAGE_HDR = 3, LRT_IMAGE_PFM = 4,
+       LRT_IMAGE_RAW = 5 };
+int lrt_image_write(const char* path, const float* rgb, uint32_t width, uint32_t height, int format,
+                    int flip_vertically);
+int lrt_image_to_bytes(const float* rgb, uint32_t width, uint32_t height, int flip_vertically, uint8_t* out);
+
 /* ---- Philox-4x32-10 (the counter-based sampler of every kernel) -------- */
 int lrt_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 

@@ -23,6 +23,7 @@
 #include "integrator/photonmapper/photon_integrator.h"
 #include "integrator/photonmapper/photon_map.h"
 #include "integrator/photonmapper/photon_tracer.h"
+#include "io/image_write.h"
 #include "io/obj_loader.h"
 #include "material/coating_bsdf.h"
 #include "material/dielectric_bsdf.h"
@@ -148,6 +149,15 @@ int ref_load_obj(const char* path, uint32_t* n_vertices, uint32_t* n_corners, fl
         corner_normals[3 * i + 2] = m->corner_normals[i].z;
     }
     return 0;
+}
+
+// ImageWriter::save (image_write.cpp:22-100); format: 0 PNG, 1 JPEG, 2 PPM, 3 HDR
+int ref_image_save(const char* path, int w, int h, const float* rgb, int format, int flip) {
+    std::vector<LinRT::color> px((size_t)w * h);
+    for (size_t i = 0; i < px.size(); ++i) px[i] = LinRT::color(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+    const LinRT::ImageFormat f[4] = {LinRT::ImageFormat::PNG, LinRT::ImageFormat::JPEG, LinRT::ImageFormat::PPM,
+                                     LinRT::ImageFormat::HDR};
+    return LinRT::ImageWriter::save(path, w, h, px, f[format], flip != 0) ? 0 : 1;
 }
 
 int ref_set_root(const char* root) {

@@ -2,15 +2,17 @@
 // (scene -> integrator -> timed frame loop -> image), rendering the seeded
 // box-icosphere scene on the GPU through linrt::gpu_path_integrator (or,
 // with --photon, linrt::gpu_photon_integrator: the reference's other
-// integrator, selected there by #ifdef, src/main.cpp:92-101) and writing a
-// little-endian PFM (FP32 RGB, bottom row first).
+// integrator, selected there by #ifdef, src/main.cpp:92-101) and writing the
+// image with lrt_image_write: OUT.png as the reference saves output.png
+// (8-bit, rows flipped, src/main.cpp:151), or .pfm / .hdr / .ppm / .raw.
 //
-//   linrt_render [--photon] OUT.pfm [width height spp max_depth rr_depth seed]
+//   linrt_render [--photon] OUT.{png,pfm,hdr,ppm,raw} [width height spp max_depth rr_depth seed]
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <string>
 
 #include "linrt.hpp"
 
@@ -54,11 +56,14 @@ int main(int argc, char** argv) {
         std::printf("%u x %u x %u spp: %.3f s, %.2f Msamples/s, %llu extension + %llu shadow rays\n", W, H, spp, s,
                     (double)st.samples / s / 1e6, (unsigned long long)st.extension_rays,
                     (unsigned long long)st.shadow_rays);
-        FILE* f = std::fopen(argv[1], "wb");
-        if (!f) { std::perror(argv[1]); return 1; }
-        std::fprintf(f, "PF\n%u %u\n-1.0\n", W, H);
-        std::fwrite(pixels.data(), sizeof(linrt::color), pixels.size(), f);
-        std::fclose(f);
+        const std::string out = argv[1];
+        const std::string ext = out.substr(out.find_last_of('.') + 1);
+        int fmt = LRT_IMAGE_PFM;
+        if (ext == "png") fmt = LRT_IMAGE_PNG;
+        else if (ext == "hdr") fmt = LRT_IMAGE_HDR;
+        else if (ext == "ppm") fmt = LRT_IMAGE_PPM;
+        else if (ext == "raw") fmt = LRT_IMAGE_RAW;
+        linrt::check(lrt_image_write(out.c_str(), &pixels[0].r, W, H, fmt, fmt == LRT_IMAGE_PNG ? 1 : 0));
     } catch (const linrt::error& e) {
         std::cerr << "linrt error " << e.code << ": " << e.what() << "\n";
         return 1;

@@ -1769,6 +1769,26 @@ int lrt_generated_free(lrt_generated* g) {
     return LRT_OK;
 }
 
+// ---- image files ---------------------------------------------------------
+int lrt_image_write(const char* path, const float* rgb, uint32_t width, uint32_t height, int format,
+                    int flip_vertically) {
+    return guarded([&] {
+        require(path && rgb, "null pointer");
+        require(width >= 1 && height >= 1, "empty image");
+        if (format == LRT_IMAGE_JPEG) throw Error(LRT_ERR_UNSUPPORTED, "JPEG is not written (lossy)");
+        require(format >= LRT_IMAGE_PNG && format <= LRT_IMAGE_RAW, "unknown image format");
+        std::string err;
+        if (!write_image(path, rgb, width, height, format, flip_vertically != 0, err)) throw Error(LRT_ERR_ARG, err);
+    });
+}
+
+int lrt_image_to_bytes(const float* rgb, uint32_t width, uint32_t height, int flip_vertically, uint8_t* out) {
+    return guarded([&] {
+        require(rgb && out, "null pointer");
+        image_to_bytes(rgb, width, height, flip_vertically != 0, out);
+    });
+}
+
 // ---- OBJ meshes ----------------------------------------------------------
 struct lrt_mesh {
     std::vector<float> positions, corner_normals;

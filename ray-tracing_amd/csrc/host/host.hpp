@@ -77,6 +77,12 @@ void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 bool load_obj(const char* path, std::vector<float>& positions, std::vector<uint32_t>& indices,
               std::vector<float>& corner_normals, std::string& error);
 
+// image files (image_write.cpp:10-100): the reference's 8-bit mapping of a
+// channel, a whole frame of it, and the writers of format LRT_IMAGE_*
+uint8_t image_to_byte(float value);
+void image_to_bytes(const float* rgb, uint32_t w, uint32_t h, bool flip, uint8_t* out);
+bool write_image(const char* path, const float* rgb, uint32_t w, uint32_t h, int format, bool flip, std::string& err);
+
 // direction numbers of the reference sampler's 16-dimensional Gray-code Sobol
 // sequence (sobol.h:43-106): out[d * 32 + b] for dimension d, bit b
 void sobol_directions(uint32_t out[16 * 32]);

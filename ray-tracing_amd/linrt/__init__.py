@@ -154,7 +154,8 @@ EXPORTED = [
     "lrt_denoiser_create", "lrt_denoiser_release", "lrt_denoiser_param_count", "lrt_denoiser_params",
     "lrt_denoiser_forward", "lrt_denoiser_loss_grad", "lrt_denoiser_adam_step", "lrt_conv3x3",
     "lrt_diff_param_count", "lrt_diff_render", "lrt_diff_adjoint", "lrt_diff_render_held",
-    "lrt_load_obj", "lrt_mesh_view", "lrt_mesh_free", "lrt_generate_obj_scene",
+    "lrt_load_obj", "lrt_mesh_view", "lrt_mesh_free", "lrt_generate_obj_scene", "lrt_image_write",
+    "lrt_image_to_bytes",
 ]
 
 
@@ -301,6 +302,28 @@ def load_obj(path):
     finally:
         LIB.lrt_mesh_free(h)
     return pos, idx, cn
+
+
+IMAGE_FORMATS = {"png": 0, "jpeg": 1, "ppm": 2, "hdr": 3, "pfm": 4, "raw": 5}
+
+
+def write_image(path, rgb, fmt=None, flip=False):
+    """lrt_image_write: rgb float [H, W, 3] (row 0 first); fmt from the file
+    suffix by default (png, ppm, hdr, pfm, raw).  The reference's main.cpp
+    saves its bottom-up frame as PNG with flip=True."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    fmt = fmt or str(path).rsplit(".", 1)[-1].lower()
+    _check(LIB.lrt_image_write(str(path).encode(), _p(rgb), C.c_uint32(rgb.shape[1]), C.c_uint32(rgb.shape[0]),
+                               C.c_int(IMAGE_FORMATS[fmt]), C.c_int(1 if flip else 0)))
+
+
+def image_to_bytes(rgb, flip=False):
+    """The reference's 8-bit mapping of a float frame (image_write.cpp:10-20)."""
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    out = np.zeros(rgb.shape, np.uint8)
+    _check(LIB.lrt_image_to_bytes(_p(rgb), C.c_uint32(rgb.shape[1]), C.c_uint32(rgb.shape[0]),
+                                  C.c_int(1 if flip else 0), _p(out, C.c_uint8)))
+    return out
 
 
 # the bunny materials of src/main.cpp:38-47, by the names of the README renders
