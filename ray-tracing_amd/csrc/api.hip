@@ -686,7 +686,11 @@ int lrt_trace_closest(lrt_scene* sc, const float* rays, uint32_t n, int32_t* hit
         Timer& tm = sc->timers[LRT_KERNEL_CLOSEST];
         auto ev = tm.next();
         HIPCHECK(hipEventRecord(ev.first, sc->stream));
-        hipLaunchKernelGGL(k_trace_closest, dim3(grid_for(*sc, n, 20)), dim3(kBlock), 0, sc->stream, sc->S, d_rays, n, d_op, d_t, d_uv);
+        sc->counters.alloc(256);
+        uint32_t* next = sc->counters.as<uint32_t>() + 8;
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, sc->stream, next, 1u, 0u);
+        hipLaunchKernelGGL(k_trace_closest, dim3(grid_for(*sc, n, 8)), dim3(kBlock), 0, sc->stream, sc->S, d_rays, n, d_op, d_t,
+                           d_uv, next);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipEventRecord(ev.second, sc->stream));
         tm.items += n;
@@ -717,7 +721,11 @@ int lrt_trace_occluded(lrt_scene* sc, const float* rays, uint32_t n, const int32
         Timer& tm = sc->timers[LRT_KERNEL_OCCLUDED];
         auto ev = tm.next();
         HIPCHECK(hipEventRecord(ev.first, sc->stream));
-        hipLaunchKernelGGL(k_trace_occluded, dim3(grid_for(*sc, n, 20)), dim3(kBlock), 0, sc->stream, sc->S, d_rays, n, d_ign, d_out);
+        sc->counters.alloc(256);
+        uint32_t* next = sc->counters.as<uint32_t>() + 8;
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, sc->stream, next, 1u, 0u);
+        hipLaunchKernelGGL(k_trace_occluded, dim3(grid_for(*sc, n, 8)), dim3(kBlock), 0, sc->stream, sc->S, d_rays, n, d_ign,
+                           d_out, next);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipEventRecord(ev.second, sc->stream));
         tm.items += n;

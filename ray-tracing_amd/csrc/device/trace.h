@@ -350,6 +350,29 @@ LRT_D void lane_trace_start(LaneTrace& T, const Ray& r, float tmin, float tmax, 
 
 // One round; returns true when the ray is finished (T.hit / T.obj, T.prim,
 // T.closest, T.u, T.v hold the result).
+// one interior node of kd_traverse: the child to visit next; the far child
+// is pushed with its entry distance when the split plane lies in [mint, maxt]
+LRT_D uint32_t kd_step(const Ray& r, const float inv[3], uint2 nd, uint32_t node, uint32_t node_base, float mint,
+                       float& maxt, int& top, uint2* lds, uint2* spill) {
+    const int axis = (int)(nd.y & 3u);
+    const float split = __uint_as_float(nd.x);
+    const float oa = comp(r.o, axis), da = comp(r.d, axis);
+    const float ia = axis == 0 ? inv[0] : (axis == 1 ? inv[1] : inv[2]);
+    const float t_plane = (split - oa) * ia;
+    const bool left_first = oa < split || (oa == split && da <= 0.0f);
+    const uint32_t left = node + 1, right = node_base + (nd.y >> 2);
+    const uint32_t first = left_first ? left : right;
+    const uint32_t second = left_first ? right : left;
+    if (t_plane > maxt || t_plane <= 0.0f) return first;
+    if (t_plane < mint) return second;
+    const uint2 e = make_uint2(second, __float_as_uint(t_plane));
+    if (top < kLdsStack) lds[top * (int)blockDim.x] = e;
+    else spill[top - kLdsStack] = e;
+    ++top;
+    maxt = t_plane;
+    return first;
+}
+
 template <bool ANY>
 LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* spill) {
     const uint32_t stride = blockDim.x;
@@ -393,31 +416,9 @@ LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* 
         const float mint = T.mint;
         int top = T.top;
         uint2 nd = S.nodes[node];
-        uint32_t tag = nd.y & 3u;
-        while (tag != 3u) {
-            const int axis = (int)tag;
-            const float split = __uint_as_float(nd.x);
-            const float oa = comp(r.o, axis), da = comp(r.d, axis);
-            const float ia = axis == 0 ? T.inv[0] : (axis == 1 ? T.inv[1] : T.inv[2]);
-            const float t_plane = (split - oa) * ia;
-            const bool left_first = oa < split || (oa == split && da <= 0.0f);
-            const uint32_t left = node + 1, right = ob.node_base + (nd.y >> 2);
-            const uint32_t first = left_first ? left : right;
-            const uint32_t second = left_first ? right : left;
-            if (t_plane > maxt || t_plane <= 0.0f) {
-                node = first;
-            } else if (t_plane < mint) {
-                node = second;
-            } else {
-                const uint2 e = make_uint2(second, __float_as_uint(t_plane));
-                if (top < kLdsStack) lds[top * stride] = e;
-                else spill[top - kLdsStack] = e;
-                ++top;
-                node = first;
-                maxt = t_plane;
-            }
+        while ((nd.y & 3u) != 3u) {
+            node = kd_step(r, T.inv, nd, node, ob.node_base, mint, maxt, top, lds, spill);
             nd = S.nodes[node];
-            tag = nd.y & 3u;
         }
         const uint32_t count = nd.y >> 2;
         const uint32_t firstref = nd.x;

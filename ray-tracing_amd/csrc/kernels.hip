@@ -30,23 +30,40 @@ LRT_D uint32_t wave_append(uint32_t* counter, bool want) {
 
 // ---------------------------------------------------------------------------
 // ray queries
-__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_trace_closest(const DevScene S, const float* __restrict__ rays, uint32_t n,
-                                                          int32_t* __restrict__ obj_prim, float* __restrict__ t_out,
-                                                          float* __restrict__ uv_out) {
+// rays -> closest hits (SoA rays: ox, oy, oz, dx, dy, dz, tmin, tmax);
+// persistent like k_extend, rays fetched from *next (zero at launch)
+__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_trace_closest(const DevScene S, const float* __restrict__ rays,
+                                                                         uint32_t n, int32_t* __restrict__ obj_prim,
+                                                                         float* __restrict__ t_out, float* __restrict__ uv_out,
+                                                                         uint32_t* __restrict__ next) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const Ray r{f3{rays[i], rays[n + i], rays[2 * (size_t)n + i]},
-                    f3{rays[3 * (size_t)n + i], rays[4 * (size_t)n + i], rays[5 * (size_t)n + i]}};
-        HitInfo h;
-        Counters c{0, 0, 0};
-        const bool any = trace_closest<false>(S, r, rays[6 * (size_t)n + i], rays[7 * (size_t)n + i], h, lds, c);
-        obj_prim[2 * (size_t)i] = any ? h.obj : -1;
-        obj_prim[2 * (size_t)i + 1] = any ? h.prim : -1;
-        t_out[i] = any ? h.t : 0.0f;
-        if (uv_out) {
-            uv_out[2 * (size_t)i] = any ? h.u : 0.0f;
-            uv_out[2 * (size_t)i + 1] = any ? h.v : 0.0f;
+    uint2 spill[kMaxStack - kLdsStack];
+    LaneTrace T;
+    uint32_t i = 0;
+    bool active = false, more = true;
+    for (;;) {
+        const uint32_t got = wave_refill(!active, next, n, more);
+        if (got != 0xffffffffu) {
+            i = got;
+            lane_trace_start(T, Ray{f3{rays[i], rays[n + i], rays[2 * (size_t)n + i]},
+                                    f3{rays[3 * (size_t)n + i], rays[4 * (size_t)n + i], rays[5 * (size_t)n + i]}},
+                             rays[6 * (size_t)n + i], rays[7 * (size_t)n + i], -1);
+            active = true;
+        }
+        if (!__ballot(active)) {
+            if (!more) break;
+            continue;
+        }
+        if (active && lane_trace_round<false>(S, T, lds, spill)) {
+            obj_prim[2 * (size_t)i] = T.hit ? T.obj : -1;
+            obj_prim[2 * (size_t)i + 1] = T.hit ? T.prim : -1;
+            t_out[i] = T.hit ? T.closest : 0.0f;
+            if (uv_out) {
+                uv_out[2 * (size_t)i] = T.hit ? T.u : 0.0f;
+                uv_out[2 * (size_t)i + 1] = T.hit ? T.v : 0.0f;
+            }
+            active = false;
         }
     }
 }
@@ -68,14 +85,35 @@ __global__ __launch_bounds__(kBlock) void k_trace_count(const DevScene S, const 
     }
 }
 
-__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_trace_occluded(const DevScene S, const float* __restrict__ rays, uint32_t n,
-                                                           const int32_t* __restrict__ ignore, uint8_t* __restrict__ out) {
+// rays -> occluded flags (a surface of an emitter other than ignore[i] in
+// [tmin, tmax]); persistent like k_shadow, rays fetched from *next
+__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_trace_occluded(const DevScene S, const float* __restrict__ rays,
+                                                                          uint32_t n, const int32_t* __restrict__ ignore,
+                                                                          uint8_t* __restrict__ out,
+                                                                          uint32_t* __restrict__ next) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const Ray r{f3{rays[i], rays[n + i], rays[2 * (size_t)n + i]},
-                    f3{rays[3 * (size_t)n + i], rays[4 * (size_t)n + i], rays[5 * (size_t)n + i]}};
-        out[i] = trace_occluded(S, r, rays[6 * (size_t)n + i], rays[7 * (size_t)n + i], ignore ? ignore[i] : -1, lds) ? 1 : 0;
+    uint2 spill[kMaxStack - kLdsStack];
+    LaneTrace T;
+    uint32_t i = 0;
+    bool active = false, more = true;
+    for (;;) {
+        const uint32_t got = wave_refill(!active, next, n, more);
+        if (got != 0xffffffffu) {
+            i = got;
+            lane_trace_start(T, Ray{f3{rays[i], rays[n + i], rays[2 * (size_t)n + i]},
+                                    f3{rays[3 * (size_t)n + i], rays[4 * (size_t)n + i], rays[5 * (size_t)n + i]}},
+                             rays[6 * (size_t)n + i], rays[7 * (size_t)n + i], ignore ? ignore[i] : -1);
+            active = true;
+        }
+        if (!__ballot(active)) {
+            if (!more) break;
+            continue;
+        }
+        if (active && lane_trace_round<true>(S, T, lds, spill)) {
+            out[i] = T.hit ? 1 : 0;
+            active = false;
+        }
     }
 }
 
