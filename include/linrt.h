@@ -248,7 +248,11 @@ typedef struct lrt_render_params {
                               reference's own Sobol sampler (sampler.h, sobol.h) instead of
                               Philox (seed unused), so each sample follows the reference's path */
 } lrt_render_params;
-enum { LRT_RENDER_COUNT = 1, LRT_RENDER_REFERENCE_SAMPLER = 2 };
+enum { LRT_RENDER_COUNT = 1, LRT_RENDER_REFERENCE_SAMPLER = 2,
+       /* lrt_diff_render / lrt_diff_adjoint only: the per-sample kernels (one thread
+          traces a whole sample) instead of the wavefront; the same estimator, an
+          independent implementation kept for checking */
+       LRT_DIFF_PER_SAMPLE = 4 };
 
 typedef struct lrt_render_stats {
     uint64_t samples, extension_rays, shadow_rays, segments;

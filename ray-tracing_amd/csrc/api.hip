@@ -13,6 +13,7 @@
 #include "host/host.hpp"
 #include "kernels.hip"
 #include "eye.hip"
+#include "diff_wave.hip"
 #include "device/nn.h"
 
 // paths per render pass (2^26: ~8.6 GiB of path state, hit and shadow
@@ -275,7 +276,7 @@ struct lrt_scene {
     float bounding_radius = 0.0f;
     float box_lo[3] = {0.0f, 0.0f, 0.0f}, box_hi[3] = {0.0f, 0.0f, 0.0f};  // scene bounds
     bool has_delta = false;
-    DevBuf diff_mats, diff_tex, diff_theta, diff_img, diff_dimg, diff_grad, diff_part;
+    DevBuf diff_mats, diff_tex, diff_theta, diff_img, diff_dimg, diff_grad, diff_part, diff_paths, diff_shadow;
     std::vector<lrt_material> host_mats;
     int max_spec_components = 0;
     Timer timers[LRT_KERNEL_COUNT_];
@@ -1090,6 +1091,116 @@ static void diff_launch_render(lrt_scene& sc, const Mats& M, const DiffParams& D
     HIPCHECK(hipGetLastError());
 }
 
+// The wavefront forward (img != null: mean radiance per pixel, W x H x 3,
+// device) or adjoint (dimg != null: gradient of sum(dimg * image) added into
+// grad, device) of the differentiable renderer, over passes of up to
+// LRT_PASS_PATHS paths like lrt_render.
+static void diff_wave_run(lrt_scene& sc, const Mats& M, const DiffParams& D, const lrt_render_params& prm, float* img,
+                          const float* dimg, float* grad, uint32_t n_params) {
+    const uint32_t W = prm.width, H = prm.height, npix = W * H;
+    const uint32_t slots = std::max<uint32_t>(1, std::min<uint32_t>(prm.spp, pass_paths() / npix));
+    const uint32_t n = npix * slots;
+    if (sc.cap < n) {
+        sc.pstate.alloc((size_t)n * 14 * 4 + 14 * 256);
+        sc.hits.alloc((size_t)n * 5 * 4 + 5 * 256);
+        sc.shadow.alloc((size_t)n * 12 * 4 + 12 * 256);
+        sc.queues.alloc((size_t)n * 3 * 4 + 3 * 256);
+        sc.cap = n;
+    }
+    sc.counters.alloc(256);
+    sc.stats.alloc(256);
+    PathState P;
+    char* c = sc.pstate.as<char>();
+    float** pf[14] = {&P.ox, &P.oy, &P.oz, &P.dx, &P.dy, &P.dz, &P.tr, &P.tg, &P.tb, &P.lr, &P.lg, &P.lb, &P.eta, &P.pdf};
+    for (float** f : pf) *f = alloc_carve<float>(c, n);
+    P.pixel0 = 0;
+    P.npix = npix;
+    HitBuf Hb;
+    c = sc.hits.as<char>();
+    Hb.obj = alloc_carve<int32_t>(c, n);
+    Hb.prim = alloc_carve<int32_t>(c, n);
+    Hb.t = alloc_carve<float>(c, n);
+    Hb.u = alloc_carve<float>(c, n);
+    Hb.v = alloc_carve<float>(c, n);
+    ShadowBuf Sb;
+    c = sc.shadow.as<char>();
+    float** sf[10] = {&Sb.ox, &Sb.oy, &Sb.oz, &Sb.dx, &Sb.dy, &Sb.dz, &Sb.tmax, &Sb.cr, &Sb.cg, &Sb.cb};
+    for (float** f : sf) *f = alloc_carve<float>(c, n);
+    Sb.ignore = alloc_carve<int32_t>(c, n);
+    Sb.path = alloc_carve<uint32_t>(c, n);
+    c = sc.queues.as<char>();
+    uint32_t* qa = alloc_carve<uint32_t>(c, n);
+    uint32_t* qb = alloc_carve<uint32_t>(c, n);
+    uint32_t* order = alloc_carve<uint32_t>(c, n);
+    DiffPaths DP{};
+    DiffShadow DS{};
+    const bool adj = dimg != nullptr;
+    if (adj) {
+        sc.diff_paths.alloc((size_t)n * 14 * 4 + 14 * 256);
+        c = sc.diff_paths.as<char>();
+        float** df[12] = {&DP.ltr, &DP.ltg, &DP.ltb, &DP.rr, &DP.rg, &DP.rb, &DP.far, &DP.fag, &DP.fab, &DP.fer, &DP.feg, &DP.feb};
+        for (float** f : df) *f = alloc_carve<float>(c, n);
+        DP.ptex = alloc_carve<int32_t>(c, n);
+        DP.pflag = alloc_carve<int32_t>(c, n);
+        sc.diff_shadow.alloc((size_t)n * 7 * 4 + 7 * 256);
+        c = sc.diff_shadow.as<char>();
+        DS.tex = alloc_carve<int32_t>(c, n);
+        float** sg[5] = {&DS.gr, &DS.gg, &DS.gb, &DS.ga, &DS.ge};
+        for (float** f : sg) *f = alloc_carve<float>(c, n);
+        DS.slot = alloc_carve<int32_t>(c, n);
+    }
+    uint32_t* counters = sc.counters.as<uint32_t>();
+    unsigned long long* dstats = sc.stats.as<unsigned long long>();
+    const uint32_t g_all = (n + kBlock - 1) / kBlock;
+    const uint32_t g_trace = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc.cus * 20);
+    const uint32_t g_shade = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc.cus * 8);
+    const uint32_t g_bins = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc.cus * 4);
+    if (img) hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(sc, (uint64_t)npix * 3, 4)), dim3(kBlock), 0, sc.stream, img,
+                                (long long)npix * 3);
+    float* grad_s = adj ? grad + (n_params - 3) : nullptr;
+    const float inv_spp = 1.0f / (float)prm.spp;
+    auto pass = [&](uint32_t smp, uint32_t np, bool replay) {
+        hipLaunchKernelGGL(k_raygen, dim3((np + kBlock - 1) / kBlock), dim3(kBlock), 0, sc.stream, P, sc.cam, W, H, 0u, np,
+                           smp, D.key, (const uint32_t*)nullptr, qa, counters);
+        uint32_t* q = qa;
+        uint32_t* qn = qb;
+        for (uint32_t depth = 0; depth < prm.max_depth; ++depth) {
+            hipLaunchKernelGGL(k_extend, dim3(g_trace), dim3(kBlock), 0, sc.stream, sc.S, P, q, counters, Hb);
+            hipLaunchKernelGGL(k_shade_bins, dim3(g_bins), dim3(kBlock), 0, sc.stream, sc.S, Hb, counters);
+            hipLaunchKernelGGL(k_shade_order, dim3(g_bins), dim3(kBlock), 0, sc.stream, sc.S, Hb, counters, order);
+            const DiffWave dw{depth, prm.max_depth, prm.rr_depth, smp, D.key, replay ? dimg : nullptr, inv_spp};
+            if (replay) {
+                hipLaunchKernelGGL(k_diff_shade<true>, dim3(g_shade), dim3(kBlock), 0, sc.stream, sc.S, sc.Ls, M, D, P, q, qn,
+                                   counters, Hb, order, Sb, DS, DP, dw, grad, grad_s);
+                hipLaunchKernelGGL(k_diff_shadow, dim3(g_trace), dim3(kBlock), 0, sc.stream, sc.S, P, counters, Sb, DS, grad,
+                                   grad_s);
+            } else {
+                hipLaunchKernelGGL(k_diff_shade<false>, dim3(g_shade), dim3(kBlock), 0, sc.stream, sc.S, sc.Ls, M, D, P, q,
+                                   qn, counters, Hb, order, Sb, DS, DP, dw, grad, grad_s);
+                hipLaunchKernelGGL(k_shadow, dim3(g_trace), dim3(kBlock), 0, sc.stream, sc.S, P, counters, Sb);
+            }
+            hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, sc.stream, counters, dstats);
+            std::swap(q, qn);
+        }
+        HIPCHECK(hipGetLastError());
+    };
+    for (uint32_t s = 0; s < prm.spp; s += slots) {
+        const uint32_t smp = prm.spp_offset + s;
+        const uint32_t ns = std::min(slots, prm.spp - s);
+        const uint32_t np = npix * ns;
+        pass(smp, np, false);
+        if (img) {
+            hipLaunchKernelGGL(k_diff_accum, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, sc.stream, P, ns, img);
+        } else {
+            hipLaunchKernelGGL(k_diff_keep, dim3(grid_for(sc, np, 8)), dim3(kBlock), 0, sc.stream, P, np, DP);
+            pass(smp, np, true);
+        }
+    }
+    if (img) hipLaunchKernelGGL(k_diff_scale, dim3(grid_for(sc, (uint64_t)npix * 3, 4)), dim3(kBlock), 0, sc.stream, img,
+                                npix * 3, inv_spp);
+    HIPCHECK(hipGetLastError());
+}
+
 // validates dp, applies theta's scalars to a copy of the materials and
 // uploads it; returns the kernel parameters (texture pointer set by caller)
 static DiffParams diff_setup(lrt_scene& sc, const lrt_diff_params& dp, const float* theta_host_scalars,
@@ -1169,7 +1280,8 @@ int lrt_diff_render(lrt_scene* sc, const lrt_diff_params* dp, const float* theta
             sc->diff_img.alloc(bytes);
             img = sc->diff_img.as<float>();
         }
-        diff_launch_render(*sc, M, D, *prm, img);
+        if (prm->flags & LRT_DIFF_PER_SAMPLE) diff_launch_render(*sc, M, D, *prm, img);
+        else diff_wave_run(*sc, M, D, *prm, img, nullptr, nullptr, diff_count(*dp));
         if (!device_ptrs) HIPCHECK(hipMemcpyAsync(image, img, bytes, hipMemcpyDeviceToHost, sc->stream));
         HIPCHECK(hipStreamSynchronize(sc->stream));
     });
@@ -1223,11 +1335,15 @@ int lrt_diff_adjoint(lrt_scene* sc, const lrt_diff_params* dp, const float* thet
             g = sc->diff_grad.as<float>();
         }
         hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(*sc, np, 4)), dim3(kBlock), 0, sc->stream, g, (long long)np);
-        uint32_t chunk, nchunks;
-        diff_chunks(*prm, chunk, nchunks);
-        const uint64_t n = (uint64_t)prm->width * prm->height * nchunks;
-        hipLaunchKernelGGL(k_diff_adjoint, dim3(grid_for(*sc, n, 16)), dim3(kBlock), 0, sc->stream, sc->S, sc->Ls, M, D,
-                           sc->cam, chunk, nchunks, dimg, g, g + (np - 3));
+        if (prm->flags & LRT_DIFF_PER_SAMPLE) {
+            uint32_t chunk, nchunks;
+            diff_chunks(*prm, chunk, nchunks);
+            const uint64_t n = (uint64_t)prm->width * prm->height * nchunks;
+            hipLaunchKernelGGL(k_diff_adjoint, dim3(grid_for(*sc, n, 16)), dim3(kBlock), 0, sc->stream, sc->S, sc->Ls, M, D,
+                               sc->cam, chunk, nchunks, dimg, g, g + (np - 3));
+        } else {
+            diff_wave_run(*sc, M, D, *prm, nullptr, dimg, g, np);
+        }
         HIPCHECK(hipGetLastError());
         if (!device_ptrs) HIPCHECK(hipMemcpyAsync(grad, g, (size_t)np * 4, hipMemcpyDeviceToHost, sc->stream));
         HIPCHECK(hipStreamSynchronize(sc->stream));

@@ -18,6 +18,7 @@ import numpy as np
 import linrt
 
 TEX = 64
+DIFF_PER_SAMPLE = 4  # lrt_render_params.flags: LRT_DIFF_PER_SAMPLE
 
 
 class DiffParams(C.Structure):
@@ -42,9 +43,13 @@ class Renderer:
     """Scene + parameter layout + render settings; forward/adjoint on torch
     device tensors (theta float32 [n], image float32 [H, W, 3])."""
 
-    def __init__(self, scene, dp, width, height, spp, max_depth=8, rr_depth=1 << 20, seed=1, spp_offset=0):
+    def __init__(self, scene, dp, width, height, spp, max_depth=8, rr_depth=1 << 20, seed=1, spp_offset=0,
+                 per_sample=False):
+        """per_sample: the per-sample kernels (LRT_DIFF_PER_SAMPLE) instead of the
+        wavefront, an independent implementation of the same estimator."""
         self.scene, self.dp = scene, dp
-        self.prm = linrt.RenderParams(width, height, spp, spp_offset, max_depth, rr_depth, seed, 0, 0, 0)
+        self.prm = linrt.RenderParams(width, height, spp, spp_offset, max_depth, rr_depth, seed, 0, 0,
+                                      DIFF_PER_SAMPLE if per_sample else 0)
         self.shape = (height, width, 3)
         self.n = count(dp)
 
@@ -78,7 +83,7 @@ class Renderer:
         prm = self.prm
         if decorrelate:
             prm = linrt.RenderParams(prm.width, prm.height, prm.spp, prm.spp_offset + (1 << 30), prm.max_depth,
-                                     prm.rr_depth, prm.seed, 0, 0, 0)
+                                     prm.rr_depth, prm.seed, 0, 0, prm.flags)
         linrt._check(linrt.LIB.lrt_diff_adjoint(self.scene._h, C.byref(self.dp), C.c_void_p(theta.data_ptr()),
                                                 C.byref(prm), C.c_void_p(d_image.data_ptr()),
                                                 C.c_void_p(g.data_ptr()), 1))
