@@ -283,6 +283,15 @@ constexpr int kMaxK = 128;
 constexpr int kPkStack = 32;       // > photon kd depth (checked at build)
 constexpr int kGroup = 8;          // lanes per photon query (a leaf holds <= 8 photons)
 constexpr int kGatherBlock = 256;  // 32 queries per block
+// waves per SIMD the eye-pass gather kernel is compiled to fit (0: the compiler's choice)
+#ifndef LRT_GATHER_WAVES
+#define LRT_GATHER_WAVES 6
+#endif
+#if LRT_GATHER_WAVES > 0
+#define LRT_GATHER_ATTR __attribute__((amdgpu_waves_per_eu(LRT_GATHER_WAVES)))
+#else
+#define LRT_GATHER_ATTR
+#endif
 
 // lanes of the wave: value of v in lane src (0..63)
 LRT_D uint32_t lane_read(uint32_t v, int src) { return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v); }
@@ -383,29 +392,26 @@ LRT_D f3 photon_estimate_group(const PhotonMapDev& Mp, f3 p, f3 n, int max_depth
         node = left_near ? nd.z : nd.w;
     }
     if (counter) c.tris += (uint32_t)size;
-    // the set in descending key order: entry t in lane t % 8, slot t / 8
-    uint64_t ord[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) ord[s] = 0;
-    for (int t = 0; t < size; ++t) {
-        const uint64_t mk = group_max(slots_max(key), lane);
-#pragma unroll
-        for (int s = 0; s < S; ++s) {
-            if (key[s] == mk) key[s] = 0;
-            if (s == (t >> 3) && gl == (t & 7)) ord[s] = mk;
-        }
-    }
+    // the set in descending key order, 8 entries at a time: entry b + j goes
+    // to lane j, the 8 lanes load and weigh their photons together, and the
+    // terms are added in order
     const float inv_r2 = 1.0f / r2;
     f3 sum = splat(0.0f);
+    for (int b = 0; b < size; b += kGroup) {
+        const int cnt = min(kGroup, size - b);
+        uint64_t mine = 0;
+        for (int j = 0; j < cnt; ++j) {
+            const uint64_t mk = group_max(slots_max(key), lane);
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-        if (s * kGroup >= size) break;
-        // this lane's entry of the slot, then the slot's 8 entries summed in order
+            for (int s = 0; s < S; ++s)
+                if (key[s] == mk) key[s] = 0;
+            if (gl == j) mine = mk;
+        }
         f3 term = splat(0.0f);
         uint32_t use = 0;
-        if (s * kGroup + gl < size) {
-            const uint32_t idx = (uint32_t)ord[s] - 1u;
-            const float d = __uint_as_float((uint32_t)(ord[s] >> 32));
+        if (gl < cnt) {
+            const uint32_t idx = (uint32_t)mine - 1u;
+            const float d = __uint_as_float((uint32_t)(mine >> 32));
             const float4 q = Mp.pos[idx];
             if (!((int)q.w > max_depth)) {
                 const float* pay = Mp.pay + 9 * (size_t)idx;
@@ -420,7 +426,6 @@ LRT_D f3 photon_estimate_group(const PhotonMapDev& Mp, f3 p, f3 n, int max_depth
                 }
             }
         }
-        const int cnt = min(kGroup, size - s * kGroup);
         for (int j = 0; j < cnt; ++j) {
             const uint32_t u = lane_read(use, gb + j);
             const f3 v{lane_read(term.x, gb + j), lane_read(term.y, gb + j), lane_read(term.z, gb + j)};

@@ -372,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_eye_query_keys(const EyeQuery Q, uin
 // gather queries in the given order (a permutation of 0..n-1), one query per
 // group of 8 lanes; S = key slots per lane (k <= 8 S)
 template <int S>
-__global__ __launch_bounds__(kGatherBlock) void k_eye_gather(const PhotonMapDev G0, const PhotonMapDev G1, const EyeQuery Q,
+__global__ __launch_bounds__(kGatherBlock) LRT_GATHER_ATTR void k_eye_gather(const PhotonMapDev G0, const PhotonMapDev G1, const EyeQuery Q,
                                                              const uint32_t* __restrict__ order, uint32_t n,
                                                              unsigned long long* __restrict__ acc) {
     __shared__ uint32_t stacks[(kGatherBlock / kGroup) * kPkStack];
