@@ -190,3 +190,22 @@ def test_readme_scene_vs_oracle(tmp_path, name):
     within = (np.abs(got - ps) <= 1e-4 * np.abs(ps) + 1e-6).mean()
     print(f"README scene {name}: reference-sampler samples within 1e-4 {within:.4f}, exact {(got == ps).mean():.4f}")
     assert within >= 0.97
+
+
+def test_reference_sampler_displaced_sphere_vs_oracle():
+    """The 1080p workload's scene (seven BSDF regions incl. coatings, Hosek
+    sky) on a crop: with the reference sampler every GPU sample against the
+    oracle's reference-sampler sample, which the CPU tests pin to the
+    reference's own path_integrator."""
+    g = linrt.displaced_sphere(seed=1, segments=256, rings=128)
+    sc = linrt.Scene(g.desc_ptr)
+    W, H, spp = 48, 27, 2
+    ps = oracle.scene(g.desc_ptr).render(W, H, spp, 16, 5, mode=1, per_sample=True)["per_sample"]
+    got = np.zeros_like(ps)
+    for s in range(spp):
+        sm, _, _ = sc.render(W, H, 1, 16, 5, spp_offset=s, reference_sampler=True)
+        got[:, :, s, :] = sm.astype(np.float32)
+    fin = np.isfinite(ps)
+    within = (np.abs(got - ps) <= 1e-4 * np.abs(ps) + 1e-6)[fin].mean()
+    print(f"displaced sphere, reference sampler: within 1e-4 {within:.4f}, exact {(got == ps)[fin].mean():.4f}")
+    assert within >= 0.97
