@@ -259,7 +259,7 @@ struct lrt_scene {
     DevLights Ls;
     Mats M;
     CameraDev cam;
-    DevBuf nodes, refs, tris, cnormals, trimat, mats, tables;
+    DevBuf nodes, refs, tris, cnormals, trimat, mats, tables, obj_box;
     lrt_scene_info info;
     std::vector<lrt_object_desc> objects;  // shapes only (pointers not kept)
     // path-tracer work buffers
@@ -300,6 +300,9 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
     std::vector<uint16_t> tmat;
     bool any_tmat = false;
     int max_depth = 0;
+    // per object: min xyz, max xyz of a mesh's triangles (empty box otherwise)
+    std::vector<float> obj_box(6 * (size_t)kMaxObjects);
+    for (size_t i = 0; i < obj_box.size(); ++i) obj_box[i] = i % 6 < 3 ? INFINITY : -INFINITY;
     std::memset(&sc.S, 0, sizeof(sc.S));
     sc.objects.assign(d.objects, d.objects + d.n_objects);
     for (uint32_t o = 0; o < d.n_objects; ++o) {
@@ -332,6 +335,13 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
                 if (len2(cross(e1, e2)) <= 0.0f) e1 = e2 = splat(0.0f);  // triangle_mesh.h:52-54
                 const float rec[9] = {p0.x, p0.y, p0.z, e1.x, e1.y, e1.z, e2.x, e2.y, e2.z};
                 tris.insert(tris.end(), rec, rec + 9);
+                float* bx = obj_box.data() + 6 * (size_t)o;
+                for (const f3& q : {p0, p1, p2})
+                    for (int a = 0; a < 3; ++a) {
+                        const float v = comp(q, a);
+                        bx[a] = std::min(bx[a], v);
+                        bx[3 + a] = std::max(bx[3 + a], v);
+                    }
                 for (int k = 0; k < 9; ++k) cn.push_back(od.corner_normals ? od.corner_normals[9 * (size_t)t + k] : 0.0f);
                 float* b = &boxes[6 * (size_t)t];
                 for (int a = 0; a < 3; ++a) {
@@ -403,9 +413,24 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
     upload(sc.mats, mats.data(), mats.size() * sizeof(lrt_material), sc.stream);
     std::vector<float> tb(d.tables ? d.tables : nullptr, d.tables ? d.tables + (size_t)d.n_tables * 101 : nullptr);
     if (tb.empty()) tb.assign(101, 0.0f);
+    // the mesh boxes, enlarged so that rounding in the slab test cannot reject
+    // a ray whose hit lies on the box (a triangle hit lies inside its mesh's box)
+    for (uint32_t o = 0; o < d.n_objects; ++o) {
+        float* bx = obj_box.data() + 6 * (size_t)o;
+        if (!(bx[0] <= bx[3])) continue;
+        float ext = 0.0f, mag = 0.0f;
+        for (int a = 0; a < 3; ++a) {
+            ext = std::max(ext, bx[3 + a] - bx[a]);
+            mag = std::max(mag, std::max(std::fabs(bx[a]), std::fabs(bx[3 + a])));
+        }
+        const float m = 1e-3f * std::max(ext, mag) + 1e-4f;
+        for (int a = 0; a < 3; ++a) { bx[a] -= m; bx[3 + a] += m; }
+    }
+    upload(sc.obj_box, obj_box.data(), obj_box.size() * 4, sc.stream);
     upload(sc.tables, tb.data(), tb.size() * 4, sc.stream);
     HIPCHECK(hipStreamSynchronize(sc.stream));
     (void)any_tmat;
+    sc.S.obj_box = sc.obj_box.as<float>();
     sc.S.nodes = sc.nodes.as<uint2>();
     sc.S.refs = sc.refs.as<uint32_t>();
     sc.S.tris = sc.tris.as<float>();

@@ -32,6 +32,7 @@ struct DevScene {
     const float* tris;         // 9 floats per triangle: p0, p1-p0, p2-p0 (degenerate: edges zeroed)
     const float* cnormals;     // 9 floats per triangle: corner normals
     const uint16_t* tri_mat;   // per triangle material (or null)
+    const float* obj_box;      // per object 6 floats: enlarged AABB of a mesh (min xyz, max xyz)
     int32_t n_objects;
     int32_t stack_depth;       // deepest root-to-leaf path over all trees
     DevObject obj[kMaxObjects];
@@ -138,6 +139,26 @@ LRT_D bool sphere_test(const DevObject& ob, const Ray& r, float tmin, float tmax
 struct Counters {
     uint32_t nodes, refs, tris;
 };
+
+// Does [t0, t1] of the ray meet the (enlarged) box of mesh object o?  An
+// early out only: a mesh whose box the segment misses holds no hit, and a
+// mesh whose box it meets is traversed exactly as without the test.
+LRT_D bool mesh_box_overlap(const DevScene& S, int o, const Ray& r, const float inv[3], float t0, float t1) {
+    const float* b = S.obj_box + 6 * o;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float org = comp(r.o, a), d = comp(r.d, a);
+        if (d == 0.0f) {
+            if (org < b[a] || org > b[3 + a]) return false;
+            continue;
+        }
+        float ta = (b[a] - org) * inv[a], tb = (b[3 + a] - org) * inv[a];
+        if (ta > tb) { const float x = ta; ta = tb; tb = x; }
+        t0 = fmaxf(t0, ta);
+        t1 = fminf(t1, tb);
+    }
+    return t0 <= t1;
+}
 
 #ifndef LRT_WHILE_WHILE
 #define LRT_WHILE_WHILE 1
@@ -259,6 +280,7 @@ LRT_D bool trace_closest(const DevScene& S, const Ray& r, float tmin, float tmax
     for (int o = 0; o < S.n_objects; ++o) {
         const DevObject& ob = S.obj[o];
         if (ob.shape == kShapeMesh) {
+            if (!COUNT && !mesh_box_overlap(S, o, r, inv, tmin, closest)) continue;
             float c = closest, u = 0.0f, v = 0.0f;
             int32_t p = -1;
             if (kd_traverse<false, COUNT>(S, ob, r, inv, tmin, c, p, u, v, lds, cnt)) {
@@ -291,6 +313,7 @@ LRT_D bool trace_occluded(const DevScene& S, const Ray& r, float tmin, float tma
         const DevObject& ob = S.obj[o];
         if (ob.emitter >= 0 && ob.emitter == ignore) continue;
         if (ob.shape == kShapeMesh) {
+            if (!mesh_box_overlap(S, o, r, inv, tmin, tmax)) continue;
             float c = tmax, u, v;
             int32_t p;
             if (kd_traverse<true, false>(S, ob, r, inv, tmin, c, p, u, v, lds, cnt)) return true;
@@ -382,6 +405,7 @@ LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* 
             const DevObject& ob = S.obj[T.o];
             if (ANY && ob.emitter >= 0 && ob.emitter == T.ignore) continue;
             if (ob.shape == kShapeMesh) {
+                if (!mesh_box_overlap(S, T.o, T.r, T.inv, T.tmin, T.closest)) continue;
                 T.node = ob.node_base;
                 T.mint = T.tmin;
                 T.maxt = T.closest;
