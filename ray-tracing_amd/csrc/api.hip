@@ -922,9 +922,9 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         const uint32_t n = npix * slots;
         HIPCHECK(hipSetDevice(sc->device));
         const auto t0 = std::chrono::steady_clock::now();
-        // work buffers: path state (14 floats), hits (5), shadow queue (12), two queues
+        // work buffers: path state (64 bytes), hits (5), shadow queue (12), two queues
         if (sc->cap < n) {
-            sc->pstate.alloc((size_t)n * 14 * 4 + 14 * 256);
+            sc->pstate.alloc((size_t)n * 64 + 256);
             sc->hits.alloc((size_t)n * 5 * 4 + 5 * 256);
             sc->shadow.alloc((size_t)n * 12 * 4 + 12 * 256);
             sc->queues.alloc((size_t)n * 3 * 4 + 3 * 256);
@@ -934,8 +934,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         sc->stats.alloc(256);
         PathState P;
         char* c = sc->pstate.as<char>();
-        float** pf[14] = {&P.ox, &P.oy, &P.oz, &P.dx, &P.dy, &P.dz, &P.tr, &P.tg, &P.tb, &P.lr, &P.lg, &P.lb, &P.eta, &P.pdf};
-        for (float** f : pf) *f = alloc_carve<float>(c, n);
+        P.rec = alloc_carve<float4>(c, 4 * (size_t)n);
         P.pixel0 = rb * W;
         P.npix = npix;
         HitBuf Hb;
@@ -1126,7 +1125,7 @@ static void diff_wave_run(lrt_scene& sc, const Mats& M, const DiffParams& D, con
     const uint32_t slots = std::max<uint32_t>(1, std::min<uint32_t>(prm.spp, pass_paths() / npix));
     const uint32_t n = npix * slots;
     if (sc.cap < n) {
-        sc.pstate.alloc((size_t)n * 14 * 4 + 14 * 256);
+        sc.pstate.alloc((size_t)n * 64 + 256);
         sc.hits.alloc((size_t)n * 5 * 4 + 5 * 256);
         sc.shadow.alloc((size_t)n * 12 * 4 + 12 * 256);
         sc.queues.alloc((size_t)n * 3 * 4 + 3 * 256);
@@ -1136,8 +1135,7 @@ static void diff_wave_run(lrt_scene& sc, const Mats& M, const DiffParams& D, con
     sc.stats.alloc(256);
     PathState P;
     char* c = sc.pstate.as<char>();
-    float** pf[14] = {&P.ox, &P.oy, &P.oz, &P.dx, &P.dy, &P.dz, &P.tr, &P.tg, &P.tb, &P.lr, &P.lg, &P.lb, &P.eta, &P.pdf};
-    for (float** f : pf) *f = alloc_carve<float>(c, n);
+    P.rec = alloc_carve<float4>(c, 4 * (size_t)n);
     P.pixel0 = 0;
     P.npix = npix;
     HitBuf Hb;

@@ -82,11 +82,13 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_diff_shade(const DevS
             const uint32_t i = order[j];
             p = queue[i];
             const uint32_t pix = P.pixel0 + p % P.npix, smp = prm.sample + p / P.npix;
-            const Ray r{f3{P.ox[p], P.oy[p], P.oz[p]}, f3{P.dx[p], P.dy[p], P.dz[p]}};
-            f3 T{P.tr[p], P.tg[p], P.tb[p]};
-            f3 L{P.lr[p], P.lg[p], P.lb[p]};
-            float eta = P.eta[p];
-            const float sa_pdf = P.pdf[p];
+            float4* pw = path_words(P, p);
+            const float4 w0 = pw[0], w1 = pw[1], w2 = pw[2], w3 = pw[3];
+            const Ray r{xyz(w0), xyz(w1)};
+            f3 T = xyz(w2);
+            f3 L = xyz(w3);
+            float eta = w0.w;
+            const float sa_pdf = w1.w;
             const bool have = sa_pdf >= 0.0f;
             f3 delta = splat(0.0f), Lt = splat(0.0f);
             bool live = false;
@@ -240,17 +242,15 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_diff_shade(const DevS
                                 else T = f3{T.x / q, T.y / q, T.z / q};
                             }
                             if (cont) {
-                                P.ox[p] = sh.p.x; P.oy[p] = sh.p.y; P.oz[p] = sh.p.z;
-                                P.dx[p] = nd.x; P.dy[p] = nd.y; P.dz[p] = nd.z;
-                                P.tr[p] = T.x; P.tg[p] = T.y; P.tb[p] = T.z;
-                                P.eta[p] = eta;
-                                P.pdf[p] = hp ? sr.pdf : -1.0f;
+                                pw[0] = word(sh.p, eta);
+                                pw[1] = word(nd, hp ? sr.pdf : -1.0f);
+                                pw[2] = word(T, 0.0f);
                             }
                         }
                     }
                 }
             }
-            P.lr[p] = L.x; P.lg[p] = L.y; P.lb[p] = L.z;
+            pw[3] = word(L, 0.0f);
         }
         const uint32_t slot = wave_append(&counters[1], cont);
         if (cont) next_queue[slot] = p;
@@ -301,9 +301,8 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_diff_shadow(const Dev
         }
         if (active && lane_trace_round<true>(S, T, lds, spill)) {
             if (!T.hit) {
-                const uint32_t p = Sb.path[i];
-                const f3 L = f3{P.lr[p], P.lg[p], P.lb[p]} + f3{Sb.cr[i], Sb.cg[i], Sb.cb[i]};
-                P.lr[p] = L.x; P.lg[p] = L.y; P.lb[p] = L.z;
+                float4* w = path_words(P, Sb.path[i]);
+                w[3] = word(xyz(w[3]) + f3{Sb.cr[i], Sb.cg[i], Sb.cb[i]}, 0.0f);
                 const int t = DS.tex[i];
                 if (t >= 0) {
                     const float g3[3] = {DS.gr[i], DS.gg[i], DS.gb[i]};
@@ -328,7 +327,7 @@ __global__ __launch_bounds__(kBlock) void k_diff_accum(const PathState P, uint32
     f3 a{acc[o], acc[o + 1], acc[o + 2]};
     for (uint32_t s = 0; s < slots; ++s) {
         const uint32_t p = s * P.npix + ip;
-        const f3 L{P.lr[p], P.lg[p], P.lb[p]};
+        const f3 L = xyz(path_words(P, p)[3]);
         if (isfinite(L.x + L.y + L.z)) a = a + L;
     }
     acc[o] = a.x; acc[o + 1] = a.y; acc[o + 2] = a.z;
@@ -341,7 +340,8 @@ __global__ __launch_bounds__(kBlock) void k_diff_scale(float* __restrict__ img, 
 // adjoint: keep each path's total radiance for the replay; clear its pending factors
 __global__ __launch_bounds__(kBlock) void k_diff_keep(const PathState P, uint32_t n, DiffPaths DP) {
     for (uint32_t p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-        DP.ltr[p] = P.lr[p]; DP.ltg[p] = P.lg[p]; DP.ltb[p] = P.lb[p];
+        const float4 w = path_words(P, p)[3];
+        DP.ltr[p] = w.x; DP.ltg[p] = w.y; DP.ltb[p] = w.z;
         DP.ptex[p] = -1;
         DP.pflag[p] = 0;
     }

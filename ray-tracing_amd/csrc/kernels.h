@@ -13,14 +13,21 @@ struct CameraDev {
     float origin[3], llc[3], horizontal[3], vertical[3];
 };
 
-// SoA state of the paths of one sample pass (one path per pixel)
+// State of the paths of one sample pass (one path per pixel), 64 bytes per
+// path read and written as four 16-byte words, so that a kernel visiting the
+// paths in any order (material-sorted shading, shadow results) touches one
+// cache line per path rather than one per field:
+//   word 0: origin, eta            word 1: direction, solid-angle pdf (< 0: none)
+//   word 2: throughput, unused     word 3: radiance, unused
 struct PathState {
-    float *ox, *oy, *oz, *dx, *dy, *dz;
-    float *tr, *tg, *tb, *lr, *lg, *lb;
-    float *eta, *pdf;   // pdf < 0: no solid-angle pdf (camera ray or delta lobe)
+    float4* rec;        // 4 words per path
     uint32_t pixel0;    // pixel index of path 0
     uint32_t npix;      // pixels per sample slice: path p is pixel pixel0 + p % npix of sample slot p / npix
 };
+
+LRT_D float4* path_words(const PathState& P, uint32_t p) { return P.rec + 4 * (size_t)p; }
+LRT_D f3 xyz(float4 w) { return f3{w.x, w.y, w.z}; }
+LRT_D float4 word(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
 // hit records of the active queue, by queue slot
 struct HitBuf {

@@ -199,12 +199,11 @@ __global__ __launch_bounds__(kBlock) void k_raygen(PathState P, const CameraDev 
     const float u = ((float)x + jx) / (float)(W - 1);
     const float vv = ((float)y + jy) / (float)(H - 1);
     const f3 d = normalize(ld3(cam.llc) + u * ld3(cam.horizontal) + vv * ld3(cam.vertical) - ld3(cam.origin));
-    P.ox[i] = cam.origin[0]; P.oy[i] = cam.origin[1]; P.oz[i] = cam.origin[2];
-    P.dx[i] = d.x; P.dy[i] = d.y; P.dz[i] = d.z;
-    P.tr[i] = 1.0f; P.tg[i] = 1.0f; P.tb[i] = 1.0f;
-    P.lr[i] = 0.0f; P.lg[i] = 0.0f; P.lb[i] = 0.0f;
-    P.eta[i] = 1.0f;
-    P.pdf[i] = -1.0f;  // no solid-angle pdf yet (camera ray)
+    float4* w = path_words(P, i);
+    w[0] = make_float4(cam.origin[0], cam.origin[1], cam.origin[2], 1.0f);  // eta 1
+    w[1] = word(d, -1.0f);  // no solid-angle pdf yet (camera ray)
+    w[2] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
+    w[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     queue[i] = i;
 }
 
@@ -225,8 +224,8 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_extend(const DevScene
         const uint32_t got = wave_refill(!active, &counters[4], n, more);
         if (got != 0xffffffffu) {
             i = got;
-            const uint32_t p = queue[i];
-            lane_trace_start(T, Ray{f3{P.ox[p], P.oy[p], P.oz[p]}, f3{P.dx[p], P.dy[p], P.dz[p]}}, 0.001f, kInfF, -1);
+            const float4* w = path_words(P, queue[i]);
+            lane_trace_start(T, Ray{xyz(w[0]), xyz(w[1])}, 0.001f, kInfF, -1);
             active = true;
         }
         if (!__ballot(active)) {
@@ -252,8 +251,8 @@ __global__ __launch_bounds__(kBlock) void k_extend_count(const DevScene S, const
     const uint32_t n = counters[0];
     uint64_t a = 0, b = 0, c3 = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t p = queue[i];
-        const Ray r{f3{P.ox[p], P.oy[p], P.oz[p]}, f3{P.dx[p], P.dy[p], P.dz[p]}};
+        const float4* w = path_words(P, queue[i]);
+        const Ray r{xyz(w[0]), xyz(w[1])};
         HitInfo h;
         Counters c{0, 0, 0};
         trace_closest<true>(S, r, 0.001f, kInfF, h, lds, c);
@@ -376,11 +375,13 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_shade(const DevScene 
         if (j < n) {
             const uint32_t i = order[j];  // hit slots grouped by material
             p = queue[i];
-            const Ray r{f3{P.ox[p], P.oy[p], P.oz[p]}, f3{P.dx[p], P.dy[p], P.dz[p]}};
-            f3 T{P.tr[p], P.tg[p], P.tb[p]};
-            f3 L{P.lr[p], P.lg[p], P.lb[p]};
-            float eta = P.eta[p];
-            const float sa_pdf = P.pdf[p];
+            float4* pw = path_words(P, p);
+            const float4 w0 = pw[0], w1 = pw[1], w2 = pw[2], w3 = pw[3];
+            const Ray r{xyz(w0), xyz(w1)};
+            f3 T = xyz(w2);
+            f3 L = xyz(w3);
+            float eta = w0.w;
+            const float sa_pdf = w1.w;
             const bool have = sa_pdf >= 0.0f;
             const int obj = Hb.obj[i];
             if (obj < 0) {  // escape: environment with MIS (path_integrator.h:32-45)
@@ -479,24 +480,15 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_shade(const DevScene 
                                 else T = f3{T.x / q, T.y / q, T.z / q};
                             }
                             if (cont) {
-                                P.ox[p] = sh.p.x; P.oy[p] = sh.p.y; P.oz[p] = sh.p.z;
-                                P.dx[p] = nd.x; P.dy[p] = nd.y; P.dz[p] = nd.z;
-                                P.tr[p] = T.x; P.tg[p] = T.y; P.tb[p] = T.z;
-                                P.eta[p] = eta;
-                                P.pdf[p] = hp ? sr.pdf : -1.0f;
+                                pw[0] = word(sh.p, eta);
+                                pw[1] = word(nd, hp ? sr.pdf : -1.0f);
+                                pw[2] = word(T, 0.0f);
                             }
                         }
                     }
                 }
             }
-            P.lr[p] = L.x; P.lg[p] = L.y; P.lb[p] = L.z;
-#ifdef LRT_NAN_PROBE
-            // diagnostic build only: report the first bounce that turns a path non-finite
-            if (!isfinite(L.x + L.y + L.z) || !isfinite(T.x + T.y + T.z) || (shadow && !isfinite(contrib.x + contrib.y + contrib.z)))
-                printf("nan: path %u depth %u obj %d prim %d t %g L %g %g %g T %g %g %g contrib %g %g %g dir %g %g %g\n", p,
-                       prm.depth, Hb.obj[i], Hb.prim[i], Hb.t[i], L.x, L.y, L.z, T.x, T.y, T.z, contrib.x, contrib.y,
-                       contrib.z, P.dx[p], P.dy[p], P.dz[p]);
-#endif
+            pw[3] = word(L, 0.0f);
         }
         const uint32_t slot = wave_append(&counters[1], cont);
         if (cont) next_queue[slot] = p;
@@ -602,9 +594,8 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_shadow(const DevScene
         }
         if (active && lane_trace_round<true>(S, T, lds, spill)) {
             if (!T.hit) {
-                const uint32_t p = Sb.path[i];
-                const f3 L = f3{P.lr[p], P.lg[p], P.lb[p]} + f3{Sb.cr[i], Sb.cg[i], Sb.cb[i]};
-                P.lr[p] = L.x; P.lg[p] = L.y; P.lb[p] = L.z;
+                float4* w = path_words(P, Sb.path[i]);
+                w[3] = word(xyz(w[3]) + f3{Sb.cr[i], Sb.cg[i], Sb.cb[i]}, 0.0f);
             }
             active = false;
         }
@@ -637,7 +628,8 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(const PathState P, uint32
     if (sumsq) { q[0] = sumsq[3 * pix]; q[1] = sumsq[3 * pix + 1]; q[2] = sumsq[3 * pix + 2]; }
     for (uint32_t j = 0; j < slots; ++j) {
         const size_t i = (size_t)j * P.npix + ip;
-        const float l[3] = {P.lr[i], P.lg[i], P.lb[i]};
+        const float4 w = path_words(P, (uint32_t)i)[3];
+        const float l[3] = {w.x, w.y, w.z};
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             s[c] += (double)l[c];
