@@ -922,10 +922,10 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         const uint32_t n = npix * slots;
         HIPCHECK(hipSetDevice(sc->device));
         const auto t0 = std::chrono::steady_clock::now();
-        // work buffers: path state (64 bytes), hits (5), shadow queue (12), two queues
+        // work buffers: path state (64 bytes), hits (32 bytes), shadow queue (12), two queues
         if (sc->cap < n) {
             sc->pstate.alloc((size_t)n * 64 + 256);
-            sc->hits.alloc((size_t)n * 5 * 4 + 5 * 256);
+            sc->hits.alloc((size_t)n * 32 + 256);
             sc->shadow.alloc((size_t)n * 12 * 4 + 12 * 256);
             sc->queues.alloc((size_t)n * 3 * 4 + 3 * 256);
             sc->cap = n;
@@ -939,11 +939,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         P.npix = npix;
         HitBuf Hb;
         c = sc->hits.as<char>();
-        Hb.obj = alloc_carve<int32_t>(c, n);
-        Hb.prim = alloc_carve<int32_t>(c, n);
-        Hb.t = alloc_carve<float>(c, n);
-        Hb.u = alloc_carve<float>(c, n);
-        Hb.v = alloc_carve<float>(c, n);
+        Hb.rec = alloc_carve<uint4>(c, 2 * (size_t)n);
         ShadowBuf Sb;
         c = sc->shadow.as<char>();
         float** sf[10] = {&Sb.ox, &Sb.oy, &Sb.oz, &Sb.dx, &Sb.dy, &Sb.dz, &Sb.tmax, &Sb.cr, &Sb.cg, &Sb.cb};
@@ -1126,7 +1122,7 @@ static void diff_wave_run(lrt_scene& sc, const Mats& M, const DiffParams& D, con
     const uint32_t n = npix * slots;
     if (sc.cap < n) {
         sc.pstate.alloc((size_t)n * 64 + 256);
-        sc.hits.alloc((size_t)n * 5 * 4 + 5 * 256);
+        sc.hits.alloc((size_t)n * 32 + 256);
         sc.shadow.alloc((size_t)n * 12 * 4 + 12 * 256);
         sc.queues.alloc((size_t)n * 3 * 4 + 3 * 256);
         sc.cap = n;
@@ -1140,11 +1136,7 @@ static void diff_wave_run(lrt_scene& sc, const Mats& M, const DiffParams& D, con
     P.npix = npix;
     HitBuf Hb;
     c = sc.hits.as<char>();
-    Hb.obj = alloc_carve<int32_t>(c, n);
-    Hb.prim = alloc_carve<int32_t>(c, n);
-    Hb.t = alloc_carve<float>(c, n);
-    Hb.u = alloc_carve<float>(c, n);
-    Hb.v = alloc_carve<float>(c, n);
+    Hb.rec = alloc_carve<uint4>(c, 2 * (size_t)n);
     ShadowBuf Sb;
     c = sc.shadow.as<char>();
     float** sf[10] = {&Sb.ox, &Sb.oy, &Sb.oz, &Sb.dx, &Sb.dy, &Sb.dz, &Sb.tmax, &Sb.cr, &Sb.cg, &Sb.cb};
@@ -1655,13 +1647,9 @@ static void carve_items(DevBuf& b, size_t cap, EyeItems& It) {
 }
 
 static void carve_hits(DevBuf& b, size_t cap, HitBuf& Hb) {
-    b.alloc(cap * 5 * 4 + 5 * 256);
+    b.alloc(cap * 32 + 256);
     char* c = b.as<char>();
-    Hb.obj = alloc_carve<int32_t>(c, cap);
-    Hb.prim = alloc_carve<int32_t>(c, cap);
-    Hb.t = alloc_carve<float>(c, cap);
-    Hb.u = alloc_carve<float>(c, cap);
-    Hb.v = alloc_carve<float>(c, cap);
+    Hb.rec = alloc_carve<uint4>(c, 2 * (size_t)cap);
 }
 
 static void carve_shadow(DevBuf& b, size_t cap, EyeShadow& Sh) {

@@ -122,12 +122,13 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_diff_shade(const DevS
                     DP.pflag[p] = 0;
                 }
             }
-            const int obj = Hb.obj[i];
+            const HitRec h = hit_load(Hb, i);
+            const int obj = h.obj;
             if (obj < 0) {  // escape: environment with MIS
                 const f3 Le = env_eval(Ls, r.d);
                 L = L + (have ? T * power_heuristic(sa_pdf, pdf_emitter_direction(S, Ls, splat(0.0f), r.d)) * Le : T * Le);
             } else {
-                const SurfaceHit sh = surface(S, r, obj, Hb.prim[i], Hb.t[i], Hb.u[i], Hb.v[i]);
+                const SurfaceHit sh = surface(S, r, obj, h.prim, h.t, h.u, h.v);
                 if (sh.emitter >= 0 && sh.front) {
                     const f3 out = -normalize(r.d);
                     const f3 Le = dot(sh.n, out) <= 0.0f ? splat(0.0f) : ld3(Ls.l[sh.emitter].radiance);

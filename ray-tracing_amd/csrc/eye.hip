@@ -98,11 +98,7 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_eye_extend(const DevS
             continue;
         }
         if (active && lane_trace_round<false>(S, T, lds, spill)) {
-            Hb.obj[i] = T.hit ? T.obj : -1;
-            Hb.prim[i] = T.prim;
-            Hb.t[i] = T.closest;
-            Hb.u[i] = T.u;
-            Hb.v[i] = T.v;
+            hit_store(Hb, i, T.hit ? T.obj : -1, T.prim, T.closest, T.u, T.v);
             active = false;
         }
     }
@@ -142,9 +138,9 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_eye_shade(const DevSc
             path = It.path[i];
             r = Ray{f3{It.ox[i], It.oy[i], It.oz[i]}, f3{It.dx[i], It.dy[i], It.dz[i]}};
             T = f3{It.tr[i], It.tg[i], It.tb[i]};
-            const int obj = Hb.obj[i];
-            hit = obj >= 0;
-            if (hit) sh = surface(S, r, obj, Hb.prim[i], Hb.t[i], Hb.u[i], Hb.v[i]);
+            const HitRec h = hit_load(Hb, i);
+            hit = h.obj >= 0;
+            if (hit) sh = surface(S, r, h.obj, h.prim, h.t, h.u, h.v);
         }
         const uint32_t depth = meta & kEyeDepthMask;
         // (A) the parent's BSDF-sampled direct light (photon_integrator.h:237-254)

@@ -29,11 +29,30 @@ LRT_D float4* path_words(const PathState& P, uint32_t p) { return P.rec + 4 * (s
 LRT_D f3 xyz(float4 w) { return f3{w.x, w.y, w.z}; }
 LRT_D float4 word(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
-// hit records of the active queue, by queue slot
+// hit records of the active queue, by queue slot: 32 bytes per slot,
+// {obj, prim, t bits, u bits} then {v bits, unused}, so that shading in
+// material order reads one cache line per hit
 struct HitBuf {
-    int32_t *obj, *prim;
-    float *t, *u, *v;
+    uint4* rec;  // 2 words per slot
 };
+
+struct HitRec {
+    int32_t obj, prim;  // obj < 0: no hit
+    float t, u, v;
+};
+
+LRT_D void hit_store(const HitBuf& H, uint32_t i, int32_t obj, int32_t prim, float t, float u, float v) {
+    uint4* w = H.rec + 2 * (size_t)i;
+    w[0] = make_uint4((uint32_t)obj, (uint32_t)prim, __float_as_uint(t), __float_as_uint(u));
+    reinterpret_cast<uint32_t*>(w + 1)[0] = __float_as_uint(v);
+}
+
+LRT_D HitRec hit_load(const HitBuf& H, uint32_t i) {
+    const uint4* w = H.rec + 2 * (size_t)i;
+    const uint4 a = w[0];
+    const uint32_t v = reinterpret_cast<const uint32_t*>(w + 1)[0];
+    return HitRec{(int32_t)a.x, (int32_t)a.y, __uint_as_float(a.z), __uint_as_float(a.w), __uint_as_float(v)};
+}
 
 // pending next-event rays with their contribution if unoccluded
 struct ShadowBuf {

@@ -233,11 +233,7 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_extend(const DevScene
             continue;
         }
         if (active && lane_trace_round<false>(S, T, lds, spill)) {
-            Hb.obj[i] = T.hit ? T.obj : -1;
-            Hb.prim[i] = T.prim;
-            Hb.t[i] = T.closest;
-            Hb.u[i] = T.u;
-            Hb.v[i] = T.v;
+            hit_store(Hb, i, T.hit ? T.obj : -1, T.prim, T.closest, T.u, T.v);
             active = false;
         }
     }
@@ -311,11 +307,12 @@ LRT_D SurfaceHit surface(const DevScene& S, const Ray& r, int obj, int prim, flo
 constexpr int kShadeBins = 16;
 
 LRT_D uint32_t shade_bin(const DevScene& S, const HitBuf& Hb, uint32_t i) {
-    const int obj = Hb.obj[i];
+    const uint4 h = Hb.rec[2 * (size_t)i];
+    const int obj = (int)h.x;
     if (obj < 0) return 0u;
     const DevObject& ob = S.obj[obj];
     int m = ob.material;
-    if (ob.shape == kShapeMesh && ob.per_tri_material) m = (int)S.tri_mat[ob.tri_base + (uint32_t)Hb.prim[i]];
+    if (ob.shape == kShapeMesh && ob.per_tri_material) m = (int)S.tri_mat[ob.tri_base + h.y];
     else if (ob.shape == kShapeRect) m = -1;
     return m < 0 ? 1u : (uint32_t)min(m + 2, kShadeBins - 1);
 }
@@ -383,7 +380,8 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_shade(const DevScene 
             float eta = w0.w;
             const float sa_pdf = w1.w;
             const bool have = sa_pdf >= 0.0f;
-            const int obj = Hb.obj[i];
+            const HitRec h = hit_load(Hb, i);
+            const int obj = h.obj;
             if (obj < 0) {  // escape: environment with MIS (path_integrator.h:32-45)
                 const f3 Le = env_eval(Ls, r.d);
                 if (!have) {
@@ -393,7 +391,7 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_shade(const DevScene 
                     L = L + T * power_heuristic(sa_pdf, lp) * Le;
                 }
             } else {
-                const SurfaceHit sh = surface(S, r, obj, Hb.prim[i], Hb.t[i], Hb.u[i], Hb.v[i]);
+                const SurfaceHit sh = surface(S, r, obj, h.prim, h.t, h.u, h.v);
                 if (sh.emitter >= 0 && sh.front) {  // path_integrator.h:47-59
                     const f3 out = -normalize(r.d);
                     const f3 Le = dot(sh.n, out) <= 0.0f ? splat(0.0f) : ld3(Ls.l[sh.emitter].radiance);
