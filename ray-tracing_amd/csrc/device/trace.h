@@ -140,11 +140,20 @@ struct Counters {
     uint32_t nodes, refs, tris;
 };
 
-// Does [t0, t1] of the ray meet the (enlarged) box of mesh object o?  An
-// early out only: a mesh whose box the segment misses holds no hit, and a
-// mesh whose box it meets is traversed exactly as without the test.
-LRT_D bool mesh_box_overlap(const DevScene& S, int o, const Ray& r, const float inv[3], float t0, float t1) {
+#ifndef LRT_BOX_CLIP
+#define LRT_BOX_CLIP 1
+#endif
+
+// Narrows [t0, t1] of the ray to the (enlarged) box of mesh object o and
+// returns false when nothing is left.  The box holds every triangle with a
+// margin of 1e-3 of its size (plus 2^-16 of |t| for the slab distances'
+// rounding), so no triangle hit lies outside the narrowed window.  Traversing
+// the mesh's tree over the narrowed window visits the reference's leaves in
+// the reference's order minus those whose window lies outside the box, and
+// accepts the same hits: the result, ties included, is unchanged.
+LRT_D bool mesh_box_clip(const DevScene& S, int o, const Ray& r, const float inv[3], float& t0, float& t1) {
     const float* b = S.obj_box + 6 * o;
+    float lo = t0, hi = t1;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         const float org = comp(r.o, a), d = comp(r.d, a);
@@ -154,10 +163,15 @@ LRT_D bool mesh_box_overlap(const DevScene& S, int o, const Ray& r, const float 
         }
         float ta = (b[a] - org) * inv[a], tb = (b[3 + a] - org) * inv[a];
         if (ta > tb) { const float x = ta; ta = tb; tb = x; }
-        t0 = fmaxf(t0, ta);
-        t1 = fminf(t1, tb);
+        lo = fmaxf(lo, ta - fabsf(ta) * 0x1p-16f);
+        hi = fminf(hi, tb + fabsf(tb) * 0x1p-16f);
     }
-    return t0 <= t1;
+    if (!(lo <= hi)) return false;
+    if (LRT_BOX_CLIP) {
+        t0 = lo;
+        t1 = hi;
+    }
+    return true;
 }
 
 #ifndef LRT_WHILE_WHILE
@@ -280,10 +294,11 @@ LRT_D bool trace_closest(const DevScene& S, const Ray& r, float tmin, float tmax
     for (int o = 0; o < S.n_objects; ++o) {
         const DevObject& ob = S.obj[o];
         if (ob.shape == kShapeMesh) {
-            if (!COUNT && !mesh_box_overlap(S, o, r, inv, tmin, closest)) continue;
-            float c = closest, u = 0.0f, v = 0.0f;
+            // the counting pass traverses the reference's full window
+            float t0 = tmin, c = closest, u = 0.0f, v = 0.0f;
+            if (!COUNT && !mesh_box_clip(S, o, r, inv, t0, c)) continue;
             int32_t p = -1;
-            if (kd_traverse<false, COUNT>(S, ob, r, inv, tmin, c, p, u, v, lds, cnt)) {
+            if (kd_traverse<false, COUNT>(S, ob, r, inv, t0, c, p, u, v, lds, cnt)) {
                 closest = c; h.obj = o; h.prim = p; h.u = u; h.v = v; any = true;
             }
         } else if (ob.shape == kShapeRect) {
@@ -313,10 +328,10 @@ LRT_D bool trace_occluded(const DevScene& S, const Ray& r, float tmin, float tma
         const DevObject& ob = S.obj[o];
         if (ob.emitter >= 0 && ob.emitter == ignore) continue;
         if (ob.shape == kShapeMesh) {
-            if (!mesh_box_overlap(S, o, r, inv, tmin, tmax)) continue;
-            float c = tmax, u, v;
+            float t0 = tmin, c = tmax, u, v;
+            if (!mesh_box_clip(S, o, r, inv, t0, c)) continue;
             int32_t p;
-            if (kd_traverse<true, false>(S, ob, r, inv, tmin, c, p, u, v, lds, cnt)) return true;
+            if (kd_traverse<true, false>(S, ob, r, inv, t0, c, p, u, v, lds, cnt)) return true;
         } else if (ob.shape == kShapeRect) {
             float t, a, b;
             if (rect_test(ob, r, tmin, tmax, t, a, b)) return true;
@@ -405,11 +420,12 @@ LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* 
             const DevObject& ob = S.obj[T.o];
             if (ANY && ob.emitter >= 0 && ob.emitter == T.ignore) continue;
             if (ob.shape == kShapeMesh) {
-                if (!mesh_box_overlap(S, T.o, T.r, T.inv, T.tmin, T.closest)) continue;
+                float t0 = T.tmin, t1 = T.closest;
+                if (!mesh_box_clip(S, T.o, T.r, T.inv, t0, t1)) continue;
                 T.node = ob.node_base;
-                T.mint = T.tmin;
-                T.maxt = T.closest;
-                T.maxt0 = T.closest;
+                T.mint = t0;
+                T.maxt = t1;
+                T.maxt0 = t1;
                 T.top = 0;
                 T.in_mesh = true;
                 break;
