@@ -930,7 +930,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
             sc->queues.alloc((size_t)n * 3 * 4 + 3 * 256);
             sc->cap = n;
         }
-        sc->counters.alloc(256);
+        sc->counters.alloc(kCounterWords * 4);
         sc->stats.alloc(256);
         PathState P;
         char* c = sc->pstate.as<char>();
@@ -992,7 +992,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
             const uint32_t smp = prm->spp_offset + s;
             const uint32_t ns = std::min(slots, prm->spp - s);  // samples in this pass
             const uint32_t np = npix * ns;
-            hipLaunchKernelGGL(k_raygen, dim3((np + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, P, sc->cam, W, H, rb,
+            hipLaunchKernelGGL(k_raygen, dim3((std::max(np, 1024u) + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, P, sc->cam, W, H, rb,
                                np, smp, key, sobol, qa, counters);
             uint32_t* q = qa;
             uint32_t* qn = qb;
@@ -1008,6 +1008,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
                 auto e2 = ts.next();
                 HIPCHECK(hipEventRecord(e2.first, sc->stream));
                 hipLaunchKernelGGL(k_shade_bins, dim3(g_bins), dim3(kBlock), 0, sc->stream, sc->S, Hb, counters);
+                hipLaunchKernelGGL(k_shade_scan, dim3(1), dim3(1024), 0, sc->stream, counters);
                 hipLaunchKernelGGL(k_shade_order, dim3(g_bins), dim3(kBlock), 0, sc->stream, sc->S, Hb, counters, order);
                 hipLaunchKernelGGL(k_shade, dim3(g_shade), dim3(kBlock), 0, sc->stream, sc->S, sc->Ls, sc->M, P, q, qn,
                                    counters, Hb, order, Sb, sp);
@@ -1127,7 +1128,7 @@ static void diff_wave_run(lrt_scene& sc, const Mats& M, const DiffParams& D, con
         sc.queues.alloc((size_t)n * 3 * 4 + 3 * 256);
         sc.cap = n;
     }
-    sc.counters.alloc(256);
+    sc.counters.alloc(kCounterWords * 4);
     sc.stats.alloc(256);
     PathState P;
     char* c = sc.pstate.as<char>();
@@ -1175,13 +1176,14 @@ static void diff_wave_run(lrt_scene& sc, const Mats& M, const DiffParams& D, con
     float* grad_s = adj ? grad + (n_params - 3) : nullptr;
     const float inv_spp = 1.0f / (float)prm.spp;
     auto pass = [&](uint32_t smp, uint32_t np, bool replay) {
-        hipLaunchKernelGGL(k_raygen, dim3((np + kBlock - 1) / kBlock), dim3(kBlock), 0, sc.stream, P, sc.cam, W, H, 0u, np,
+        hipLaunchKernelGGL(k_raygen, dim3((std::max(np, 1024u) + kBlock - 1) / kBlock), dim3(kBlock), 0, sc.stream, P, sc.cam, W, H, 0u, np,
                            smp, D.key, (const uint32_t*)nullptr, qa, counters);
         uint32_t* q = qa;
         uint32_t* qn = qb;
         for (uint32_t depth = 0; depth < prm.max_depth; ++depth) {
             hipLaunchKernelGGL(k_extend, dim3(g_trace), dim3(kBlock), 0, sc.stream, sc.S, P, q, counters, Hb);
             hipLaunchKernelGGL(k_shade_bins, dim3(g_bins), dim3(kBlock), 0, sc.stream, sc.S, Hb, counters);
+            hipLaunchKernelGGL(k_shade_scan, dim3(1), dim3(1024), 0, sc.stream, counters);
             hipLaunchKernelGGL(k_shade_order, dim3(g_bins), dim3(kBlock), 0, sc.stream, sc.S, Hb, counters, order);
             const DiffWave dw{depth, prm.max_depth, prm.rr_depth, smp, D.key, replay ? dimg : nullptr, inv_spp};
             if (replay) {

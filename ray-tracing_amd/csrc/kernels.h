@@ -30,8 +30,8 @@ LRT_D f3 xyz(float4 w) { return f3{w.x, w.y, w.z}; }
 LRT_D float4 word(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
 
 // hit records of the active queue, by queue slot: 32 bytes per slot,
-// {obj, prim, t bits, u bits} then {v bits, unused}, so that shading in
-// material order reads one cache line per hit
+// {obj, prim, t bits, u bits} then {v bits, spatial cell of the hit point,
+// unused}, so that shading in any order reads one cache line per hit
 struct HitBuf {
     uint4* rec;  // 2 words per slot
 };
@@ -41,10 +41,33 @@ struct HitRec {
     float t, u, v;
 };
 
-LRT_D void hit_store(const HitBuf& H, uint32_t i, int32_t obj, int32_t prim, float t, float u, float v) {
+LRT_D void hit_store(const HitBuf& H, uint32_t i, int32_t obj, int32_t prim, float t, float u, float v,
+                     uint32_t cell = 0u) {
     uint4* w = H.rec + 2 * (size_t)i;
     w[0] = make_uint4((uint32_t)obj, (uint32_t)prim, __float_as_uint(t), __float_as_uint(u));
-    reinterpret_cast<uint32_t*>(w + 1)[0] = __float_as_uint(v);
+    *reinterpret_cast<uint2*>(w + 1) = make_uint2(__float_as_uint(v), cell);
+}
+
+// Shading order bins: 0 escapes, 1 emitter shapes, then 2 + the hit point's
+// cell in an 8 x 8 x 8 grid over the hit mesh's box, in Morton order
+constexpr int kCellBits = 3;
+constexpr int kShadeBins = 2 + (1 << (3 * kCellBits));
+// counters[] beyond the queue counters: per-bin counts, cursors and starts
+constexpr int kBinCount = 64, kBinCursor = kBinCount + 1024, kBinStart = kBinCursor + 1024;
+constexpr size_t kCounterWords = kBinStart + 1024;
+static_assert(kShadeBins <= 1024, "shading bins");
+
+// Morton cell of point q in the box lo..hi (6 floats), 3 bits per axis
+LRT_D uint32_t grid_cell(const float* box, f3 q) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float x = (comp(q, a) - box[a]) / (box[3 + a] - box[a]) * (float)(1 << kCellBits);
+        const uint32_t k = (uint32_t)fminf(fmaxf(x, 0.0f), (float)((1 << kCellBits) - 1));
+#pragma unroll
+        for (int b = 0; b < kCellBits; ++b) c |= ((k >> b) & 1u) << (3 * b + a);
+    }
+    return c;
 }
 
 LRT_D HitRec hit_load(const HitBuf& H, uint32_t i) {

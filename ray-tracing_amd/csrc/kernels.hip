@@ -180,7 +180,10 @@ __global__ __launch_bounds__(kBlock) void k_raygen(PathState P, const CameraDev 
         counters[3] = 0;
         counters[4] = 0;  // next extension ray to fetch (k_extend)
         counters[5] = 0;  // next shadow ray to fetch (k_shadow)
-        for (int b = 16; b < 48; ++b) counters[b] = 0;  // shading bins: counts, cursors
+    }
+    if (i < 1024) {  // shading bins: counts, cursors
+        counters[kBinCount + i] = 0;
+        counters[kBinCursor + i] = 0;
     }
     if (i >= n) return;
     const uint32_t ip = i % P.npix, slot = i / P.npix;
@@ -233,7 +236,10 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_extend(const DevScene
             continue;
         }
         if (active && lane_trace_round<false>(S, T, lds, spill)) {
-            hit_store(Hb, i, T.hit ? T.obj : -1, T.prim, T.closest, T.u, T.v);
+            uint32_t cell = 0u;
+            if (T.hit && S.obj[T.obj].shape == kShapeMesh)
+                cell = grid_cell(S.obj_box + 6 * T.obj, T.r.o + T.closest * T.r.d);
+            hit_store(Hb, i, T.hit ? T.obj : -1, T.prim, T.closest, T.u, T.v, cell);
             active = false;
         }
     }
@@ -300,52 +306,67 @@ LRT_D SurfaceHit surface(const DevScene& S, const Ray& r, int obj, int prim, flo
     return s;
 }
 
-// Shading order: hit slots grouped by what is hit (escapes, emitter shapes,
-// then one bin per material), so that the lanes of a shading wave run the
-// same BSDF code.  The results do not depend on the order: every path's state
-// is its own and the pixel sums are taken per path afterwards.
-constexpr int kShadeBins = 16;
-
+// Shading order: hit slots grouped by where the hit lies (escapes, emitter
+// shapes, then cells of the hit mesh's box in Morton order).  The shading
+// kernel appends the continued paths and their shadow rays in this order, so
+// the next extension and shadow rays leave from nearby points, and the
+// traversal of nearby rays shares the tree's nodes in the caches.  Hits of one
+// region mostly share a material as well.  The results do not depend on the
+// order: every path's state is its own and the pixel sums are taken per path
+// afterwards.
 LRT_D uint32_t shade_bin(const DevScene& S, const HitBuf& Hb, uint32_t i) {
     const uint4 h = Hb.rec[2 * (size_t)i];
     const int obj = (int)h.x;
     if (obj < 0) return 0u;
-    const DevObject& ob = S.obj[obj];
-    int m = ob.material;
-    if (ob.shape == kShapeMesh && ob.per_tri_material) m = (int)S.tri_mat[ob.tri_base + h.y];
-    else if (ob.shape == kShapeRect) m = -1;
-    return m < 0 ? 1u : (uint32_t)min(m + 2, kShadeBins - 1);
+    if (S.obj[obj].shape == kShapeRect) return 1u;
+    return 2u + reinterpret_cast<const uint32_t*>(Hb.rec + 2 * (size_t)i + 1)[1];
 }
 
-// counts per bin into counters[16 + b]
+// counts per bin into counters[kBinCount + b]
 __global__ __launch_bounds__(kBlock) void k_shade_bins(const DevScene S, const HitBuf Hb, uint32_t* __restrict__ counters) {
     __shared__ uint32_t cnt[kShadeBins];
-    if (threadIdx.x < kShadeBins) cnt[threadIdx.x] = 0u;
+    for (int b = threadIdx.x; b < kShadeBins; b += blockDim.x) cnt[b] = 0u;
     __syncthreads();
     const uint32_t n = counters[0];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         atomicAdd(&cnt[shade_bin(S, Hb, i)], 1u);
     __syncthreads();
-    if (threadIdx.x < kShadeBins && cnt[threadIdx.x]) atomicAdd(&counters[16 + threadIdx.x], cnt[threadIdx.x]);
+    for (int b = threadIdx.x; b < kShadeBins; b += blockDim.x)
+        if (cnt[b]) atomicAdd(&counters[kBinCount + b], cnt[b]);
+}
+
+// bin starts: exclusive prefix sum of the counts (one block of 1024 threads)
+__global__ __launch_bounds__(1024) void k_shade_scan(uint32_t* __restrict__ counters) {
+    __shared__ uint32_t v[1024];
+    const int t = threadIdx.x;
+    const uint32_t mine = t < kShadeBins ? counters[kBinCount + t] : 0u;
+    v[t] = mine;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const uint32_t add = t >= d ? v[t - d] : 0u;
+        __syncthreads();
+        v[t] += add;
+        __syncthreads();
+    }
+    if (t < kShadeBins) counters[kBinStart + t] = v[t] - mine;
 }
 
 // order[]: slots bin by bin.  Each block takes a contiguous chunk of slots and
-// reserves its range in every bin with one atomic (cursors counters[32 + b]).
+// reserves its range in every bin it uses with one atomic (cursors
+// counters[kBinCursor + b]).
 __global__ __launch_bounds__(kBlock) void k_shade_order(const DevScene S, const HitBuf Hb, uint32_t* __restrict__ counters,
                                                         uint32_t* __restrict__ order) {
     __shared__ uint32_t cnt[kShadeBins], base[kShadeBins];
-    if (threadIdx.x < kShadeBins) cnt[threadIdx.x] = 0u;
+    for (int b = threadIdx.x; b < kShadeBins; b += blockDim.x) cnt[b] = 0u;
     __syncthreads();
     const uint32_t n = counters[0];
     const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
     const uint32_t b0 = min(n, blockIdx.x * per), b1 = min(n, b0 + per);
     for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) atomicAdd(&cnt[shade_bin(S, Hb, i)], 1u);
     __syncthreads();
-    if (threadIdx.x < kShadeBins) {
-        uint32_t start = 0;
-        for (int b = 0; b < (int)threadIdx.x; ++b) start += counters[16 + b];
-        base[threadIdx.x] = cnt[threadIdx.x] ? start + atomicAdd(&counters[32 + threadIdx.x], cnt[threadIdx.x]) : 0u;
-        cnt[threadIdx.x] = 0u;
+    for (int b = threadIdx.x; b < kShadeBins; b += blockDim.x) {
+        base[b] = cnt[b] ? counters[kBinStart + b] + atomicAdd(&counters[kBinCursor + b], cnt[b]) : 0u;
+        cnt[b] = 0u;
     }
     __syncthreads();
     for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
@@ -612,7 +633,10 @@ __global__ void k_advance(uint32_t* counters, unsigned long long* stats) {
         counters[4] = 0;
         counters[5] = 0;
     }
-    if (blockIdx.x == 0 && threadIdx.x >= 16 && threadIdx.x < 48) counters[threadIdx.x] = 0;  // shading bins
+    for (uint32_t b = threadIdx.x; b < 1024; b += blockDim.x) {  // shading bins: counts, cursors
+        counters[kBinCount + b] = 0;
+        counters[kBinCursor + b] = 0;
+    }
 }
 
 // per pixel, the pass's samples in sample order
