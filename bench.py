@@ -233,7 +233,8 @@ def main():
     st_c = scene.render_device(W, H, args.spp, MAX_DEPTH, RR_DEPTH, acc.data_ptr(), seed=SEED, count=True)
     scene.sync()
     log(f"count pass: ext {st_c.extension_rays} shadow {st_c.shadow_rays} nodes {st_c.nodes_visited} "
-        f"refs {st_c.refs_read} tris {st_c.tris_tested}")
+        f"refs {st_c.refs_read} tris {st_c.tris_tested} shadow nodes {st_c.shadow_nodes_visited} "
+        f"refs {st_c.shadow_refs_read} tris {st_c.shadow_tris_tested}")
     for _ in range(args.warmup):
         scene.render_device(W, H, args.spp, MAX_DEPTH, RR_DEPTH, acc.data_ptr(), seed=SEED)
     scene.sync()
@@ -260,16 +261,20 @@ def main():
     ms_s, n_s, _ = scene.kernel_timing(linrt.KERNEL_SHADE)
     value = samples / elapsed / 1e6
 
-    # roofline of the dominant kernel (closest-hit traversal of extension rays):
-    # B_ray = 32 (ray) + 8 n_node + 4 n_ref + 36 n_tri + 16 (hit) bytes
-    per_frame_bytes = (32 + 16) * st_c.extension_rays + 8 * st_c.nodes_visited + 4 * st_c.refs_read + \
-        36 * st_c.tris_tested
+    # roofline of the dominant kernels, the traversal launches (k_extend for
+    # camera rays, then per bounce k_trace_queues over the shadow rays and the
+    # next extension rays): B_ray = 32 (ray) + 8 n_node + 4 n_ref + 36 n_tri
+    # + 16 (result) bytes, n_* from the counting pass (the reference's
+    # closest-hit traversal, also for shadow rays)
+    per_frame_bytes = (32 + 16) * (st_c.extension_rays + st_c.shadow_rays) + \
+        8 * (st_c.nodes_visited + st_c.shadow_nodes_visited) + 4 * (st_c.refs_read + st_c.shadow_refs_read) + \
+        36 * (st_c.tris_tested + st_c.shadow_tris_tested)
     launches_per_frame = n_c / max(args.steps, 1)
     bytes_per_launch = per_frame_bytes / launches_per_frame
     avg_ms = ms_c / max(n_c, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tf = REPO / "profiles" / "traffic_extend_displaced_sphere.json"
+    tf = REPO / "profiles" / "traffic_trace_displaced_sphere.json"
     if tf.exists():
         traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
 
@@ -299,9 +304,10 @@ def main():
                    "triangles": int(info.n_triangles), "parallelism": "single GPU"},
         "mrays_per_s": round((ext + shadow) / elapsed / 1e6, 2),
         "rays_per_sample": round((ext + shadow) / max(samples, 1), 3),
-        "kernel_ms_per_step": {"extend": round(ms_c / args.steps, 2), "shadow": round(ms_o / args.steps, 2),
-                               "shade": round(ms_s / args.steps, 2)},
-        "roofline": {"bound": "hbm", "kernel": "k_extend (closest-hit kd traversal)", "achieved": round(achieved, 1),
+        "kernel_ms_per_step": {"trace": round(ms_c / args.steps, 2), "shade": round(ms_s / args.steps, 2)},
+        "roofline": {"bound": "hbm",
+                     "kernel": "k_extend + k_trace_queues (kd traversal: extension rays closest hit, shadow rays any hit)",
+                     "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(avg_ms, 4),
                      "launches": int(n_c)},

@@ -262,6 +262,10 @@ typedef struct lrt_render_stats {
     /* with LRT_RENDER_COUNT: extension-ray traversal work (nodes visited,
        leaf references read, triangle tests), from a separate counting kernel */
     uint64_t nodes_visited, refs_read, tris_tested;
+    /* with LRT_RENDER_COUNT: the same for the shadow rays, counted as the
+       reference's closest-hit shadow query over [0.001, tmax]
+       (path_integrator.h:113-116); lrt_render only */
+    uint64_t shadow_nodes_visited, shadow_refs_read, shadow_tris_tested;
 } lrt_render_stats;
 
 /* Renders and ADDS the per-pixel sum of sample radiance into sum_rgb

@@ -970,7 +970,8 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         HIPCHECK(hipMemcpyAsync(dstats, zero, sizeof(zero), hipMemcpyHostToDevice, sc->stream));
         const bool counting = (prm->flags & LRT_RENDER_COUNT) != 0;
         sc->totals.alloc(256);
-        HIPCHECK(hipMemcpyAsync(sc->totals.p, zero, sizeof(zero), hipMemcpyHostToDevice, sc->stream));
+        const unsigned long long zero8[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // extension, shadow work
+        HIPCHECK(hipMemcpyAsync(sc->totals.p, zero8, sizeof(zero8), hipMemcpyHostToDevice, sc->stream));
         const uint2 key = make_uint2((uint32_t)prm->seed, (uint32_t)(prm->seed >> 32));
         const uint32_t* sobol = nullptr;
         if (prm->flags & LRT_RENDER_REFERENCE_SAMPLER) {
@@ -996,14 +997,16 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
                                np, smp, key, sobol, qa, counters);
             uint32_t* q = qa;
             uint32_t* qn = qb;
+            unsigned long long* totals = sc->totals.as<unsigned long long>();
+            // camera rays; every later bounce traces its extension rays in the
+            // previous bounce's k_trace_queues launch
+            if (counting)
+                hipLaunchKernelGGL(k_extend_count, dim3(g_trace), dim3(kBlock), 0, sc->stream, sc->S, P, q, counters, totals);
+            auto e0 = tc.next();
+            HIPCHECK(hipEventRecord(e0.first, sc->stream));
+            hipLaunchKernelGGL(k_extend, dim3(g_trace), dim3(kBlock), 0, sc->stream, sc->S, P, q, counters, Hb);
+            HIPCHECK(hipEventRecord(e0.second, sc->stream));
             for (uint32_t depth = 0; depth < prm->max_depth; ++depth) {
-                if (counting)
-                    hipLaunchKernelGGL(k_extend_count, dim3(g_trace), dim3(kBlock), 0, sc->stream, sc->S, P, q, counters,
-                                       sc->totals.as<unsigned long long>());
-                auto e1 = tc.next();
-                HIPCHECK(hipEventRecord(e1.first, sc->stream));
-                hipLaunchKernelGGL(k_extend, dim3(g_trace), dim3(kBlock), 0, sc->stream, sc->S, P, q, counters, Hb);
-                HIPCHECK(hipEventRecord(e1.second, sc->stream));
                 ShadeParams sp{depth, prm->max_depth, prm->rr_depth, smp, key, sobol};
                 auto e2 = ts.next();
                 HIPCHECK(hipEventRecord(e2.first, sc->stream));
@@ -1013,9 +1016,16 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
                 hipLaunchKernelGGL(k_shade, dim3(g_shade), dim3(kBlock), 0, sc->stream, sc->S, sc->Ls, sc->M, P, q, qn,
                                    counters, Hb, order, Sb, sp);
                 HIPCHECK(hipEventRecord(e2.second, sc->stream));
-                auto e3 = to.next();
+                if (counting) {
+                    hipLaunchKernelGGL(k_shadow_count, dim3(g_trace), dim3(kBlock), 0, sc->stream, sc->S, Sb, counters,
+                                       totals);
+                    hipLaunchKernelGGL(k_extend_count, dim3(g_trace), dim3(kBlock), 0, sc->stream, sc->S, P, qn,
+                                       counters + 1, totals);
+                }
+                auto e3 = tc.next();
                 HIPCHECK(hipEventRecord(e3.first, sc->stream));
-                hipLaunchKernelGGL(k_shadow, dim3(g_trace), dim3(kBlock), 0, sc->stream, sc->S, P, counters, Sb);
+                hipLaunchKernelGGL(k_trace_queues, dim3(g_trace), dim3(kBlock), 0, sc->stream, sc->S, P, qn, counters, Hb,
+                                   Sb);
                 HIPCHECK(hipEventRecord(e3.second, sc->stream));
                 hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, sc->stream, counters, dstats);
                 std::swap(q, qn);
@@ -1023,7 +1033,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
             hipLaunchKernelGGL(k_accumulate, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, P, ns, dsum, dsq);
             HIPCHECK(hipGetLastError());
         }
-        unsigned long long hst[4] = {0, 0, 0, 0}, work[4] = {0, 0, 0, 0};
+        unsigned long long hst[4] = {0, 0, 0, 0}, work[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         HIPCHECK(hipMemcpyAsync(hst, dstats, sizeof(hst), hipMemcpyDeviceToHost, sc->stream));
         HIPCHECK(hipMemcpyAsync(work, sc->totals.p, sizeof(work), hipMemcpyDeviceToHost, sc->stream));
         if (!device_ptrs) {
@@ -1053,6 +1063,9 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
             stats->nodes_visited = work[0];
             stats->refs_read = work[1];
             stats->tris_tested = work[2];
+            stats->shadow_nodes_visited = work[3];
+            stats->shadow_refs_read = work[4];
+            stats->shadow_tris_tested = work[5];
         }
     }
 }

@@ -367,6 +367,7 @@ struct LaneTrace {
     float mint, maxt, maxt0;
     int32_t top;
     bool in_mesh, hit;
+    bool any;              // any hit ends the ray (shadow ray), also when ANY is false
 };
 
 LRT_D void lane_trace_start(LaneTrace& T, const Ray& r, float tmin, float tmax, int ignore) {
@@ -384,6 +385,7 @@ LRT_D void lane_trace_start(LaneTrace& T, const Ray& r, float tmin, float tmax, 
     T.ignore = ignore;
     T.in_mesh = false;
     T.hit = false;
+    T.any = false;
 }
 
 // One round; returns true when the ray is finished (T.hit / T.obj, T.prim,
@@ -414,11 +416,12 @@ LRT_D uint32_t kd_step(const Ray& r, const float inv[3], uint2 nd, uint32_t node
 template <bool ANY>
 LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* spill) {
     const uint32_t stride = blockDim.x;
+    const bool any = ANY || T.any;
     if (!T.in_mesh) {
         // the objects in order: analytic shapes at once, a mesh starts a kd traversal
         for (; T.o < S.n_objects; ++T.o) {
             const DevObject& ob = S.obj[T.o];
-            if (ANY && ob.emitter >= 0 && ob.emitter == T.ignore) continue;
+            if (any && ob.emitter >= 0 && ob.emitter == T.ignore) continue;
             if (ob.shape == kShapeMesh) {
                 float t0 = T.tmin, t1 = T.closest;
                 if (!mesh_box_clip(S, T.o, T.r, T.inv, t0, t1)) continue;
@@ -433,14 +436,14 @@ LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* 
                 float t, a, b;
                 if (rect_test(ob, T.r, T.tmin, T.closest, t, a, b)) {
                     T.hit = true;
-                    if (ANY) return true;
+                    if (any) return true;
                     T.closest = t; T.obj = T.o; T.prim = 0; T.u = a; T.v = b;
                 }
             } else {
                 float t;
                 if (sphere_test(ob, T.r, T.tmin, T.closest, t)) {
                     T.hit = true;
-                    if (ANY) return true;
+                    if (any) return true;
                     T.closest = t; T.obj = T.o; T.prim = 0; T.u = 0.0f; T.v = 0.0f;
                 }
             }
@@ -478,7 +481,7 @@ LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* 
                     float t, u, v;
                     if (tri_test(rec[j], r, mint, limit, t, u, v)) {
                         T.hit = true;
-                        if (ANY) return true;
+                        if (any) return true;
                         T.closest = t;
                         T.obj = T.o;
                         T.prim = (int32_t)(tri[j] - ob.tri_base);

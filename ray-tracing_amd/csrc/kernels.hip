@@ -246,11 +246,12 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_extend(const DevScene
 }
 
 // same rays with algorithmic-work counters (node visits, references, tests)
+// into totals[0..2]; *count is the queue's length
 __global__ __launch_bounds__(kBlock) void k_extend_count(const DevScene S, const PathState P, const uint32_t* __restrict__ queue,
-                                                         const uint32_t* __restrict__ counters, unsigned long long* totals) {
+                                                         const uint32_t* __restrict__ count, unsigned long long* totals) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;
-    const uint32_t n = counters[0];
+    const uint32_t n = *count;
     uint64_t a = 0, b = 0, c3 = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float4* w = path_words(P, queue[i]);
@@ -263,6 +264,26 @@ __global__ __launch_bounds__(kBlock) void k_extend_count(const DevScene S, const
     atomicAdd(&totals[0], (unsigned long long)a);
     atomicAdd(&totals[1], (unsigned long long)b);
     atomicAdd(&totals[2], (unsigned long long)c3);
+}
+
+// the shadow queue's algorithmic work into totals[3..5]: the reference's
+// shadow test is a closest-hit query over [0.001, tmax] (path_integrator.h:113-116)
+__global__ __launch_bounds__(kBlock) void k_shadow_count(const DevScene S, const ShadowBuf Sb,
+                                                         const uint32_t* __restrict__ counters, unsigned long long* totals) {
+    __shared__ uint2 stack[kLdsStack * kBlock];
+    uint2* lds = stack + threadIdx.x;
+    const uint32_t n = counters[2];
+    uint64_t a = 0, b = 0, c3 = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const Ray r{f3{Sb.ox[i], Sb.oy[i], Sb.oz[i]}, f3{Sb.dx[i], Sb.dy[i], Sb.dz[i]}};
+        HitInfo h;
+        Counters c{0, 0, 0};
+        trace_closest<true>(S, r, 0.001f, Sb.tmax[i], h, lds, c);
+        a += c.nodes; b += c.refs; c3 += c.tris;
+    }
+    atomicAdd(&totals[3], (unsigned long long)a);
+    atomicAdd(&totals[4], (unsigned long long)b);
+    atomicAdd(&totals[5], (unsigned long long)c3);
 }
 
 struct SurfaceHit {
@@ -615,6 +636,59 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_shadow(const DevScene
             if (!T.hit) {
                 float4* w = path_words(P, Sb.path[i]);
                 w[3] = word(xyz(w[3]) + f3{Sb.cr[i], Sb.cg[i], Sb.cb[i]}, 0.0f);
+            }
+            active = false;
+        }
+    }
+}
+
+// One persistent launch over two queues: the shadow rays of this bounce
+// (counters[2] records, any hit, unoccluded ones add their contribution to
+// the path radiance) and the extension rays of the next bounce (counters[1]
+// paths of next_queue, closest hit into the hit records).  Both only read the
+// state shading wrote, so they can share a launch, and the launch ends with the
+// slowest ray of the two queues instead of one tail per queue.  Items are
+// fetched from counters[5]: shadow records first, then extension rays.
+__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_trace_queues(const DevScene S, PathState P,
+                                                                        const uint32_t* __restrict__ next_queue,
+                                                                        uint32_t* __restrict__ counters, HitBuf Hb,
+                                                                        const ShadowBuf Sb) {
+    __shared__ uint2 stack[kLdsStack * kBlock];
+    uint2* lds = stack + threadIdx.x;
+    uint2 spill[kMaxStack - kLdsStack];
+    const uint32_t ns = counters[2], n = ns + counters[1];
+    LaneTrace T;
+    uint32_t i = 0;
+    bool active = false, more = true;
+    for (;;) {
+        const uint32_t got = wave_refill(!active, &counters[5], n, more);
+        if (got != 0xffffffffu) {
+            i = got;
+            if (i < ns) {
+                lane_trace_start(T, Ray{f3{Sb.ox[i], Sb.oy[i], Sb.oz[i]}, f3{Sb.dx[i], Sb.dy[i], Sb.dz[i]}}, 0.001f,
+                                 Sb.tmax[i], Sb.ignore[i]);
+                T.any = true;
+            } else {
+                const float4* w = path_words(P, next_queue[i - ns]);
+                lane_trace_start(T, Ray{xyz(w[0]), xyz(w[1])}, 0.001f, kInfF, -1);
+            }
+            active = true;
+        }
+        if (!__ballot(active)) {
+            if (!more) break;
+            continue;
+        }
+        if (active && lane_trace_round<false>(S, T, lds, spill)) {
+            if (i < ns) {
+                if (!T.hit) {
+                    float4* w = path_words(P, Sb.path[i]);
+                    w[3] = word(xyz(w[3]) + f3{Sb.cr[i], Sb.cg[i], Sb.cb[i]}, 0.0f);
+                }
+            } else {
+                uint32_t cell = 0u;
+                if (T.hit && S.obj[T.obj].shape == kShapeMesh)
+                    cell = grid_cell(S.obj_box + 6 * T.obj, T.r.o + T.closest * T.r.d);
+                hit_store(Hb, i - ns, T.hit ? T.obj : -1, T.prim, T.closest, T.u, T.v, cell);
             }
             active = false;
         }

@@ -100,7 +100,9 @@ class RenderStats(C.Structure):
                 ("segments", C.c_uint64), ("seconds", C.c_double), ("trace_ms", C.c_double),
                 ("occluded_ms", C.c_double), ("shade_ms", C.c_double),
                 ("trace_launches", C.c_uint64), ("occluded_launches", C.c_uint64),
-                ("nodes_visited", C.c_uint64), ("refs_read", C.c_uint64), ("tris_tested", C.c_uint64)]
+                ("nodes_visited", C.c_uint64), ("refs_read", C.c_uint64), ("tris_tested", C.c_uint64),
+                ("shadow_nodes_visited", C.c_uint64), ("shadow_refs_read", C.c_uint64),
+                ("shadow_tris_tested", C.c_uint64)]
 
 
 class PhotonParams(C.Structure):

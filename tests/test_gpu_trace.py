@@ -66,6 +66,12 @@ def test_counts_match_oracle(sphere_small):
     a, b, c = gs.trace_count(rays)
     _, _, _, cnt = os_.trace(rays, counts=True)
     assert np.array_equal(a, cnt[:, 0]) and np.array_equal(b, cnt[:, 1]) and np.array_equal(c, cnt[:, 2])
+    # finite windows, as the shadow rays' count uses (the reference's shadow
+    # test is a closest-hit query over [0.001, tmax])
+    rays[7] = np.random.default_rng(32).uniform(0.01, 1.5, rays.shape[1]).astype(np.float32)
+    a, b, c = gs.trace_count(rays)
+    _, _, _, cnt = os_.trace(rays, counts=True)
+    assert np.array_equal(a, cnt[:, 0]) and np.array_equal(b, cnt[:, 1]) and np.array_equal(c, cnt[:, 2])
 
 
 def test_occluded(box_scene):
