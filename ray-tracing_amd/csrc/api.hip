@@ -16,10 +16,11 @@
 #include "diff_wave.hip"
 #include "device/nn.h"
 
-// paths per render pass (2^26: ~8.6 GiB of path state, hit and shadow
-// queues); the environment variable LINRT_PASS_PATHS overrides it
+// paths per render pass (2^27: ~21 GiB of path records, hit records, shadow
+// and path queues, one pass for a 64-spp 1080p frame); the environment
+// variable LINRT_PASS_PATHS overrides it
 #ifndef LRT_PASS_PATHS
-#define LRT_PASS_PATHS (1u << 26)
+#define LRT_PASS_PATHS (1u << 27)
 #endif
 static uint32_t pass_paths() {
     static const uint32_t v = [] {
