@@ -22,6 +22,22 @@
 #ifndef LRT_PASS_PATHS
 #define LRT_PASS_PATHS (1u << 27)
 #endif
+// camera samples per eye-pass chunk (2^22: the work items of the first
+// generations take up to ~25 GiB at 32 BSDF samples per first hit; bigger
+// chunks mean fewer launches and host round trips per frame);
+// LINRT_EYE_CHUNK overrides
+#ifndef LRT_EYE_CHUNK
+#define LRT_EYE_CHUNK (1u << 22)
+#endif
+static uint32_t eye_chunk() {
+    static const uint32_t v = [] {
+        const char* e = std::getenv("LINRT_EYE_CHUNK");
+        const long long x = e ? std::atoll(e) : 0;
+        return x >= 1024 && x <= (1ll << 24) ? (uint32_t)x : (uint32_t)LRT_EYE_CHUNK;
+    }();
+    return v;
+}
+
 static uint32_t pass_paths() {
     static const uint32_t v = [] {
         const char* e = std::getenv("LINRT_PASS_PATHS");
@@ -1725,10 +1741,13 @@ static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, doub
         HIPCHECK(hipMemsetAsync(sc->totals.p, 0, 32, sc->stream));
     }
     const uint64_t pixels = (uint64_t)(re - rb) * W;
-    const uint32_t chunk = (uint32_t)std::min<uint64_t>(pixels, 1u << 18);
+    // a chunk: up to eye_chunk() camera samples, several samples of every
+    // pixel when the band has fewer pixels
+    const uint32_t chunk = (uint32_t)std::min<uint64_t>(pixels, eye_chunk());
+    const uint32_t eslots = std::max<uint32_t>(1, std::min<uint32_t>(prm->spp, (uint32_t)(eye_chunk() / chunk)));
     sc->counters.alloc(256);
     uint32_t* counters = sc->counters.as<uint32_t>();
-    sc->eye_acc.alloc((size_t)chunk * 3 * 8);
+    sc->eye_acc.alloc((size_t)chunk * eslots * 3 * 8);
     unsigned long long* acc = sc->eye_acc.as<unsigned long long>();
     const uint2 key = make_uint2((uint32_t)prm->seed, (uint32_t)(prm->seed >> 32));
     const f3 box_lo{sc->box_lo[0], sc->box_lo[1], sc->box_lo[2]};
@@ -1749,20 +1768,22 @@ static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, doub
     tg.collect();
     const double before_c = tc.total_ms, before_o = to.total_ms, before_s = ts.total_ms;
     const uint64_t lc0 = tc.launches, lo0 = to.launches;
-    for (uint32_t s = 0; s < prm->spp; ++s) {
+    for (uint32_t s = 0; s < prm->spp; s += eslots) {
+        const uint32_t ns = std::min(eslots, prm->spp - s);
         for (uint64_t c0 = 0; c0 < pixels; c0 += chunk) {
             const uint32_t nc = (uint32_t)std::min<uint64_t>(chunk, pixels - c0);
-            EyeParams ep{prm->max_depth, prm->spp_offset + s, (uint32_t)((uint64_t)rb * W + c0), key, maps, 0, 0, 0};
+            const uint32_t n0 = nc * ns;  // camera samples of this chunk
+            EyeParams ep{prm->max_depth, prm->spp_offset + s, (uint32_t)((uint64_t)rb * W + c0), key, maps, 0, 0, 0, nc};
             int cur = 0;
-            sc->eye_cap[cur] = grow_cap(sc->eye_cap[cur], nc);
+            sc->eye_cap[cur] = grow_cap(sc->eye_cap[cur], n0);
             EyeItems A, B;
             carve_items(sc->eye_items[cur], sc->eye_cap[cur], A);
-            hipLaunchKernelGGL(k_fill_u64, dim3(std::min<uint32_t>((3 * nc + 255) / 256, 4096)), dim3(256), 0, sc->stream,
-                               acc, (size_t)3 * nc);
-            hipLaunchKernelGGL(k_eye_raygen, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, sc->cam, W, H,
-                               nc, ep, A);
+            hipLaunchKernelGGL(k_fill_u64, dim3(std::min<uint32_t>((3 * n0 + 255) / 256, 4096)), dim3(256), 0, sc->stream,
+                               acc, (size_t)3 * n0);
+            hipLaunchKernelGGL(k_eye_raygen, dim3((n0 + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, sc->cam, W, H,
+                               n0, ep, A);
             HIPCHECK(hipGetLastError());
-            uint32_t n = nc;
+            uint32_t n = n0;
             for (uint32_t depth = 1; n > 0; ++depth) {
                 // children bound: 32 BSDF samples at depth 1, else 1, plus the
                 // specular components while depth < 4
@@ -1851,7 +1872,7 @@ static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, doub
                 std::swap(A, B);
                 n = cnt[0];
             }
-            hipLaunchKernelGGL(k_eye_resolve, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, acc, nc,
+            hipLaunchKernelGGL(k_eye_resolve, dim3((nc + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, acc, nc, ns,
                                ep.pixel0, dsum, dsq);
             HIPCHECK(hipGetLastError());
         }

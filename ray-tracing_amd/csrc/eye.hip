@@ -61,9 +61,9 @@ LRT_D void eye_add(unsigned long long* acc, uint32_t sid, f3 c) {
 __global__ __launch_bounds__(kBlock) void k_eye_raygen(const CameraDev cam, uint32_t W, uint32_t H, uint32_t n,
                                                        EyeParams prm, EyeItems It) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t pix = prm.pixel0 + i;
+        const uint32_t pix = prm.pixel0 + i % prm.npix;
         const uint32_t x = pix % W, y = pix / W;
-        const uint4 v = philox(make_uint4(pix, prm.sample, 0u, 0u), prm.key);
+        const uint4 v = philox(make_uint4(pix, prm.sample + i / prm.npix, 0u, 0u), prm.key);
         const float jx = u24(v.x), jy = u24(v.y);
         const float u = ((float)x + jx) / (float)(W - 1);
         const float vv = ((float)y + jy) / (float)(H - 1);
@@ -227,7 +227,8 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_eye_shade(const DevSc
                 eye_push(Out, s, sid, eye_child(path, (uint32_t)c), (depth + 1u) | kEyeEmission | kEyeLi, sh.p,
                          to_world(fr, sr.wo), T * w, splat(0.0f), 0.0f);
         }
-        EyeRng rng{prm.pixel0 + sid, prm.sample, path, path == 0u ? 2u : 0u, prm.key, 0xffffffffu, make_uint4(0, 0, 0, 0)};
+        EyeRng rng{prm.pixel0 + sid % prm.npix, prm.sample + sid / prm.npix, path, path == 0u ? 2u : 0u, prm.key, 0xffffffffu,
+                   make_uint4(0, 0, 0, 0)};
         const bool first = depth == 1u;
         // emitter samples (photon_integrator.h:157-210)
         const int ce = (shade && smooth) ? (first ? 16 : 1) : 0;
@@ -408,17 +409,21 @@ __global__ __launch_bounds__(kGatherBlock) void k_eye_gather_count(const PhotonM
 }
 
 // per-sample radiance (rounded to FP32 like the reference's color) into the
-// pixel sums
-__global__ __launch_bounds__(kBlock) void k_eye_resolve(const unsigned long long* __restrict__ acc, uint32_t n,
-                                                        uint32_t pixel0, double* __restrict__ sum,
+// pixel sums, a pixel's samples in sample order (slot j holds sample j of the
+// chunk's npix pixels)
+__global__ __launch_bounds__(kBlock) void k_eye_resolve(const unsigned long long* __restrict__ acc, uint32_t npix,
+                                                        uint32_t slots, uint32_t pixel0, double* __restrict__ sum,
                                                         double* __restrict__ sumsq) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < npix; i += gridDim.x * blockDim.x) {
+        for (uint32_t j = 0; j < slots; ++j) {
+            const size_t sid = (size_t)j * npix + i;
 #pragma unroll
-        for (int ch = 0; ch < 3; ++ch) {
-            const double L = (double)(float)((double)(long long)acc[3 * (size_t)i + ch] / kEyeFix);
-            const size_t o = 3 * ((size_t)pixel0 + i) + ch;
-            sum[o] += L;
-            if (sumsq) sumsq[o] += L * L;
+            for (int ch = 0; ch < 3; ++ch) {
+                const double L = (double)(float)((double)(long long)acc[3 * sid + ch] / kEyeFix);
+                const size_t o = 3 * ((size_t)pixel0 + i) + ch;
+                sum[o] += L;
+                if (sumsq) sumsq[o] += L * L;
+            }
         }
     }
 }

@@ -124,6 +124,7 @@ struct EyeParams {
     uint2 key;
     uint32_t maps;   // bit m: map m (0 global, 1 caustic) is not empty
     uint32_t cap_items, cap_shadow, cap_query;  // appends beyond these are dropped
+    uint32_t npix;   // pixels per sample slice: item sid is pixel pixel0 + sid % npix of sample sample + sid / npix
 };
 
 struct ShadeParams {
