@@ -351,7 +351,7 @@ LRT_D bool trace_occluded(const DevScene& S, const Ray& r, float tmin, float tma
 // trace_closest / trace_occluded.
 
 #ifndef LRT_REFILL
-#define LRT_REFILL 24
+#define LRT_REFILL 48
 #endif
 constexpr int kRefill = LRT_REFILL;  // idle lanes that make a wave take new rays
 
