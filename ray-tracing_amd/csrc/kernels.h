@@ -15,7 +15,7 @@ struct CameraDev {
 
 // State of the paths of one sample pass (one path per pixel), 64 bytes per
 // path read and written as four 16-byte words, so that a kernel visiting the
-// paths in any order (material-sorted shading, shadow results) touches one
+// paths in any order (shading in bin order, shadow results) touches one
 // cache line per path rather than one per field:
 //   word 0: origin, eta            word 1: direction, solid-angle pdf (< 0: none)
 //   word 2: throughput, unused     word 3: radiance, unused
