@@ -355,36 +355,46 @@ LRT_D bool trace_occluded(const DevScene& S, const Ray& r, float tmin, float tma
 #endif
 constexpr int kRefill = LRT_REFILL;  // idle lanes that make a wave take new rays
 
+// A lane's state between rounds.  The small integers share one register
+// (ids: object being traversed, hit object + 1, ignored emitter + 1, stack
+// top) and the window's upper end is not kept (it is the mint of the entry
+// below the top, or maxt0), so the persistent kernels spill fewer registers.
 struct LaneTrace {
     Ray r;
     float tmin, closest;   // closest: nearest accepted t (ANY: the ray's tmax)
-    int32_t obj, prim;     // hit so far (closest hit)
+    int32_t prim;          // hit so far (closest hit)
     float u, v;
-    int32_t o;             // object being traversed
-    int32_t ignore;        // emitter whose surfaces do not occlude (ANY)
+    uint32_t ids;          // bits 0-4 object o, 5-9 hit object + 1, 10-13 ignored emitter + 1, 16-21 top
     uint32_t node;         // kd state of object o
-    float mint, maxt, maxt0;
-    int32_t top;
+    float mint, maxt0;
     bool in_mesh, hit;
     bool any;              // any hit ends the ray (shadow ray), also when ANY is false
+    LRT_D int o() const { return (int)(ids & 31u); }
+    LRT_D int obj() const { return (int)((ids >> 5) & 31u) - 1; }
+    LRT_D int ignore() const { return (int)((ids >> 10) & 15u) - 1; }
+    LRT_D int top() const { return (int)((ids >> 16) & 63u); }
+    LRT_D void set_o(int v) { ids = (ids & ~31u) | (uint32_t)v; }
+    LRT_D void set_obj(int v) { ids = (ids & ~(31u << 5)) | ((uint32_t)(v + 1) << 5); }
+    LRT_D void set_top(int v) { ids = (ids & ~(63u << 16)) | ((uint32_t)v << 16); }
 };
+static_assert(kMaxObjects < 31 && kMaxStack < 64, "LaneTrace::ids fields");
 
 LRT_D void lane_trace_start(LaneTrace& T, const Ray& r, float tmin, float tmax, int ignore) {
     T.r = r;
     T.tmin = tmin;
     T.closest = tmax;
-    T.obj = -1;
     T.prim = -1;
     T.u = 0.0f;
     T.v = 0.0f;
-    T.o = 0;
-    T.ignore = ignore;
+    // an ignored emitter outside 0..14 matches no emitter (a scene has at most 4 lights)
+    const int ig = ignore >= 0 && ignore < 15 ? ignore : -1;
+    T.ids = (uint32_t)(ig + 1) << 10;  // o 0, no hit object, top 0
     T.in_mesh = false;
     T.hit = false;
     T.any = false;
 }
 
-// One round; returns true when the ray is finished (T.hit / T.obj, T.prim,
+// One round; returns true when the ray is finished (T.hit / T.obj(), T.prim,
 // T.closest, T.u, T.v hold the result).
 // one interior node of kd_traverse: the child to visit next; the far child
 // is pushed with its entry distance when the split plane lies in [mint, maxt]
@@ -415,18 +425,17 @@ LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* 
     const bool any = ANY || T.any;
     if (!T.in_mesh) {
         // the objects in order: analytic shapes at once, a mesh starts a kd traversal
-        for (; T.o < S.n_objects; ++T.o) {
-            const DevObject& ob = S.obj[T.o];
-            if (any && ob.emitter >= 0 && ob.emitter == T.ignore) continue;
+        for (int o = T.o(); o < S.n_objects; ++o, T.set_o(o)) {
+            const DevObject& ob = S.obj[o];
+            if (any && ob.emitter >= 0 && ob.emitter == T.ignore()) continue;
             if (ob.shape == kShapeMesh) {
                 float t0 = T.tmin, t1 = T.closest;
                 const float inv[3] = {1.0f / T.r.d.x, 1.0f / T.r.d.y, 1.0f / T.r.d.z};
-                if (!mesh_box_clip(S, T.o, T.r, inv, t0, t1)) continue;
+                if (!mesh_box_clip(S, o, T.r, inv, t0, t1)) continue;
                 T.node = ob.node_base;
                 T.mint = t0;
-                T.maxt = t1;
                 T.maxt0 = t1;
-                T.top = 0;
+                T.set_top(0);
                 T.in_mesh = true;
                 break;
             } else if (ob.shape == kShapeRect) {
@@ -434,27 +443,29 @@ LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* 
                 if (rect_test(ob, T.r, T.tmin, T.closest, t, a, b)) {
                     T.hit = true;
                     if (any) return true;
-                    T.closest = t; T.obj = T.o; T.prim = 0; T.u = a; T.v = b;
+                    T.closest = t; T.set_obj(o); T.prim = 0; T.u = a; T.v = b;
                 }
             } else {
                 float t;
                 if (sphere_test(ob, T.r, T.tmin, T.closest, t)) {
                     T.hit = true;
                     if (any) return true;
-                    T.closest = t; T.obj = T.o; T.prim = 0; T.u = 0.0f; T.v = 0.0f;
+                    T.closest = t; T.set_obj(o); T.prim = 0; T.u = 0.0f; T.v = 0.0f;
                 }
             }
         }
         if (!T.in_mesh) return true;
     }
-    const DevObject& ob = S.obj[T.o];
+    const int o = T.o();
+    const DevObject& ob = S.obj[o];
     const Ray& r = T.r;
     bool done = T.closest < T.mint;
     if (!done) {
         uint32_t node = T.node;
-        float maxt = T.maxt;
         const float mint = T.mint;
-        int top = T.top;
+        int top = T.top();
+        float maxt = T.maxt0;  // the window's end: the mint of the entry below the top
+        if (top > 0) maxt = __uint_as_float((top - 1 < kLdsStack ? lds[(top - 1) * stride] : spill[top - 1 - kLdsStack]).y);
         uint2 nd = S.nodes[node];
         while ((nd.y & 3u) != 3u) {
             node = kd_step(r, nd, node, ob.node_base, mint, maxt, top, lds, spill);
@@ -480,7 +491,7 @@ LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* 
                         T.hit = true;
                         if (any) return true;
                         T.closest = t;
-                        T.obj = T.o;
+                        T.set_obj(o);
                         T.prim = (int32_t)(tri[j] - ob.tri_base);
                         T.u = u;
                         T.v = v;
@@ -496,17 +507,12 @@ LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* 
             const uint2 e = top < kLdsStack ? lds[top * stride] : spill[top - kLdsStack];
             T.node = e.x;
             T.mint = __uint_as_float(e.y);
-            if (top == 0) T.maxt = T.maxt0;
-            else {
-                const uint2 below = (top - 1) < kLdsStack ? lds[(top - 1) * stride] : spill[top - 1 - kLdsStack];
-                T.maxt = __uint_as_float(below.y);
-            }
-            T.top = top;
+            T.set_top(top);
         }
     }
     if (done) {  // object o finished: the next round goes on with the following objects
         T.in_mesh = false;
-        ++T.o;
+        T.set_o(o + 1);
     }
     return false;
 }

@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_eye_extend(const DevS
             continue;
         }
         if (active && lane_trace_round<false>(S, T, lds, spill)) {
-            hit_store(Hb, i, T.hit ? T.obj : -1, T.prim, T.closest, T.u, T.v);
+            hit_store(Hb, i, T.hit ? T.obj() : -1, T.prim, T.closest, T.u, T.v);
             active = false;
         }
     }

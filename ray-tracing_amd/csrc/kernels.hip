@@ -56,7 +56,7 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_trace_closest(const D
             continue;
         }
         if (active && lane_trace_round<false>(S, T, lds, spill)) {
-            obj_prim[2 * (size_t)i] = T.hit ? T.obj : -1;
+            obj_prim[2 * (size_t)i] = T.hit ? T.obj() : -1;
             obj_prim[2 * (size_t)i + 1] = T.hit ? T.prim : -1;
             t_out[i] = T.hit ? T.closest : 0.0f;
             if (uv_out) {
@@ -237,9 +237,9 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_extend(const DevScene
         }
         if (active && lane_trace_round<false>(S, T, lds, spill)) {
             uint32_t cell = 0u;
-            if (T.hit && S.obj[T.obj].shape == kShapeMesh)
-                cell = grid_cell(S.obj_box + 6 * T.obj, T.r.o + T.closest * T.r.d);
-            hit_store(Hb, i, T.hit ? T.obj : -1, T.prim, T.closest, T.u, T.v, cell);
+            if (T.hit && S.obj[T.obj()].shape == kShapeMesh)
+                cell = grid_cell(S.obj_box + 6 * T.obj(), T.r.o + T.closest * T.r.d);
+            hit_store(Hb, i, T.hit ? T.obj() : -1, T.prim, T.closest, T.u, T.v, cell);
             active = false;
         }
     }
@@ -686,9 +686,9 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_trace_queues(const De
                 }
             } else {
                 uint32_t cell = 0u;
-                if (T.hit && S.obj[T.obj].shape == kShapeMesh)
-                    cell = grid_cell(S.obj_box + 6 * T.obj, T.r.o + T.closest * T.r.d);
-                hit_store(Hb, i - ns, T.hit ? T.obj : -1, T.prim, T.closest, T.u, T.v, cell);
+                if (T.hit && S.obj[T.obj()].shape == kShapeMesh)
+                    cell = grid_cell(S.obj_box + 6 * T.obj(), T.r.o + T.closest * T.r.d);
+                hit_store(Hb, i - ns, T.hit ? T.obj() : -1, T.prim, T.closest, T.u, T.v, cell);
             }
             active = false;
         }
