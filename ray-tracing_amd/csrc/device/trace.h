@@ -56,6 +56,17 @@ struct DevScene {
 #else
 #define LRT_TRACE_ATTR
 #endif
+// the path tracer's traversal launches (k_extend, k_trace_queues, k_shadow):
+// measured faster at 8 waves/SIMD on the 1080p frame, while the batch ray
+// queries are faster at 7
+#ifndef LRT_PATH_TRACE_WAVES
+#define LRT_PATH_TRACE_WAVES 8
+#endif
+#if LRT_PATH_TRACE_WAVES > 0
+#define LRT_PATH_TRACE_ATTR __attribute__((amdgpu_waves_per_eu(LRT_PATH_TRACE_WAVES)))
+#else
+#define LRT_PATH_TRACE_ATTR
+#endif
 #ifndef LRT_SHADE_WAVES
 #define LRT_SHADE_WAVES 2
 #endif

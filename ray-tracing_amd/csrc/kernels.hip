@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(PathState P, const CameraDev 
 // extension rays of the active queue -> hit records by queue slot.
 // Persistent: every lane takes a new ray as soon as its ray is finished (the
 // wave fetches rays for its idle lanes from counters[4]).
-__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_extend(const DevScene S, const PathState P,
+__global__ __launch_bounds__(kBlock) LRT_PATH_TRACE_ATTR void k_extend(const DevScene S, const PathState P,
                                                                   const uint32_t* __restrict__ queue,
                                                                   uint32_t* __restrict__ counters, HitBuf Hb) {
     __shared__ uint2 stack[kLdsStack * kBlock];
@@ -611,7 +611,7 @@ __global__ __launch_bounds__(kBlock) void k_aov(const DevScene S, const Mats M, 
 
 // shadow rays: unoccluded ones add their contribution to the path radiance.
 // Persistent like k_extend (rays fetched from counters[5]).
-__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_shadow(const DevScene S, PathState P,
+__global__ __launch_bounds__(kBlock) LRT_PATH_TRACE_ATTR void k_shadow(const DevScene S, PathState P,
                                                                   uint32_t* __restrict__ counters, const ShadowBuf Sb) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;
@@ -649,7 +649,7 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_shadow(const DevScene
 // state shading wrote, so they can share a launch, and the launch ends with the
 // slowest ray of the two queues instead of one tail per queue.  Items are
 // fetched from counters[5]: shadow records first, then extension rays.
-__global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_trace_queues(const DevScene S, PathState P,
+__global__ __launch_bounds__(kBlock) LRT_PATH_TRACE_ATTR void k_trace_queues(const DevScene S, PathState P,
                                                                         const uint32_t* __restrict__ next_queue,
                                                                         uint32_t* __restrict__ counters, HitBuf Hb,
                                                                         const ShadowBuf Sb) {
