@@ -276,6 +276,20 @@ int lrt_render(lrt_scene* scene, const lrt_render_params* params,
                double* sum_rgb, double* sumsq_rgb, int device_ptrs,
                lrt_render_stats* stats);
 
+/* Renders like lrt_render (the pixel sums are discarded) and copies the rays
+ * the frame traces after its camera rays, in the layout lrt_trace_closest
+ * takes (8 arrays of *n floats back to back): the extension rays of bounces
+ * >= 1 (window [0.001, inf), path_integrator.h:140) into ext_rays and the
+ * next-event shadow rays (window [0.001, tmax], path_integrator.h:104-116)
+ * into shadow_rays, with the emitter each shadow ray ignores in
+ * shadow_ignore.  Rays beyond a capacity are dropped; *n_ext / *n_shadow
+ * return how many were stored.  Host pointers.  A test and benchmark source
+ * of the frame's own incoherent rays; the reference has no counterpart. */
+int lrt_render_capture_rays(lrt_scene* scene, const lrt_render_params* params,
+                            float* ext_rays, uint32_t ext_cap, float* shadow_rays,
+                            int32_t* shadow_ignore, uint32_t shadow_cap,
+                            uint32_t* n_ext, uint32_t* n_shadow);
+
 /* ---- photon mapping (photon_integrator.h, photon_gather.h, photon_tracer.h,
  *      photon_map.h) ---------------------------------------------------------
  * A scene holds two photon maps, as photon_integrator does: kind 0 the global

@@ -917,9 +917,18 @@ int lrt_sky_eval(int hip_device, const lrt_light_desc* sky, const float* dirs, u
 }
 
 // ---- path tracer ----------------------------------------------------------------
-// pm: photon-map eye pass (k_shade<true> + k_photon_eye with the scene's map)
+// device buffers that receive a frame's rays (lrt_render_capture_rays)
+struct RayCapture {
+    float* ext;
+    uint32_t ext_cap;
+    float* sh;
+    int32_t* sh_ignore;
+    uint32_t sh_cap;
+    uint32_t* fill;  // 2 counters
+};
+
 static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum_rgb, double* sumsq_rgb,
-                        int device_ptrs, lrt_render_stats* stats) {
+                        int device_ptrs, lrt_render_stats* stats, const RayCapture* capture = nullptr) {
     {
         require(sc && prm && sum_rgb, "null pointer");
         const uint32_t W = prm->width, H = prm->height;
@@ -1033,6 +1042,10 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
                 hipLaunchKernelGGL(k_shade, dim3(g_shade), dim3(kBlock), 0, sc->stream, sc->S, sc->Ls, sc->M, P, q, qn,
                                    counters, Hb, order, Sb, sp);
                 HIPCHECK(hipEventRecord(e2.second, sc->stream));
+                if (capture)
+                    hipLaunchKernelGGL(k_capture_rays, dim3(g_bins), dim3(kBlock), 0, sc->stream, P, qn, counters, Sb,
+                                       capture->ext, capture->ext_cap, capture->sh, capture->sh_ignore,
+                                       capture->sh_cap, capture->fill);
                 if (counting) {
                     hipLaunchKernelGGL(k_shadow_count, dim3(g_trace), dim3(kBlock), 0, sc->stream, sc->S, Sb, counters,
                                        totals);
@@ -1090,6 +1103,44 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
 int lrt_render(lrt_scene* sc, const lrt_render_params* prm, double* sum_rgb, double* sumsq_rgb, int device_ptrs,
                lrt_render_stats* stats) {
     return guarded([&] { render_impl(sc, prm, sum_rgb, sumsq_rgb, device_ptrs, stats); });
+}
+
+int lrt_render_capture_rays(lrt_scene* sc, const lrt_render_params* prm, float* ext_rays, uint32_t ext_cap,
+                            float* shadow_rays, int32_t* shadow_ignore, uint32_t shadow_cap, uint32_t* n_ext,
+                            uint32_t* n_shadow) {
+    return guarded([&] {
+        require(sc && prm && n_ext && n_shadow, "null pointer");
+        require((ext_cap == 0 || ext_rays) && (shadow_cap == 0 || (shadow_rays && shadow_ignore)), "null pointer");
+        HIPCHECK(hipSetDevice(sc->device));
+        DevBuf img, e, s, si, fill;
+        const size_t npix = (size_t)prm->width * prm->height;
+        img.alloc(npix * 3 * 8);
+        HIPCHECK(hipMemsetAsync(img.p, 0, npix * 3 * 8, sc->stream));
+        e.alloc((size_t)ext_cap * 32);
+        s.alloc((size_t)shadow_cap * 32);
+        si.alloc((size_t)shadow_cap * 4);
+        fill.alloc(8);
+        HIPCHECK(hipMemsetAsync(fill.p, 0, 8, sc->stream));
+        const RayCapture cap{e.as<float>(), ext_cap, s.as<float>(), si.as<int32_t>(), shadow_cap, fill.as<uint32_t>()};
+        render_impl(sc, prm, img.as<double>(), nullptr, 1, nullptr, &cap);
+        uint32_t f[2];
+        HIPCHECK(hipMemcpyAsync(f, fill.p, 8, hipMemcpyDeviceToHost, sc->stream));
+        HIPCHECK(hipStreamSynchronize(sc->stream));
+        *n_ext = std::min(f[0], ext_cap);
+        *n_shadow = std::min(f[1], shadow_cap);
+        // compact the SoA columns to the stored count
+        for (int k = 0; k < 8; ++k) {
+            if (*n_ext)
+                HIPCHECK(hipMemcpyAsync(ext_rays + (size_t)k * *n_ext, e.as<float>() + (size_t)k * ext_cap,
+                                        (size_t)*n_ext * 4, hipMemcpyDeviceToHost, sc->stream));
+            if (*n_shadow)
+                HIPCHECK(hipMemcpyAsync(shadow_rays + (size_t)k * *n_shadow, s.as<float>() + (size_t)k * shadow_cap,
+                                        (size_t)*n_shadow * 4, hipMemcpyDeviceToHost, sc->stream));
+        }
+        if (*n_shadow)
+            HIPCHECK(hipMemcpyAsync(shadow_ignore, si.p, (size_t)*n_shadow * 4, hipMemcpyDeviceToHost, sc->stream));
+        HIPCHECK(hipStreamSynchronize(sc->stream));
+    });
 }
 
 int lrt_render_aov(lrt_scene* sc, const lrt_render_params* prm, float* aov, int device_ptrs) {

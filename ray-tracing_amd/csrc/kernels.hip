@@ -695,6 +695,43 @@ __global__ __launch_bounds__(kBlock) LRT_PATH_TRACE_ATTR void k_trace_queues(con
     }
 }
 
+// Copies the rays a bounce is about to trace into a caller's buffers, in the
+// layout of the batch ray queries (SoA: ox oy oz dx dy dz tmin tmax with a
+// stride of the buffer's capacity): the next bounce's extension rays (window
+// [0.001, inf), path_integrator.h:140) and this bounce's shadow rays (window
+// [0.001, tmax] and the emitter they ignore, path_integrator.h:104-116).
+// fill[0] / fill[1] count the rays offered; those beyond a capacity are dropped.
+__global__ void k_capture_rays(const PathState P, const uint32_t* __restrict__ next_queue,
+                               const uint32_t* __restrict__ counters, const ShadowBuf Sb, float* __restrict__ ext,
+                               uint32_t ext_cap, float* __restrict__ sh, int32_t* __restrict__ sh_ignore,
+                               uint32_t sh_cap, uint32_t* __restrict__ fill) {
+    const uint32_t ne = counters[1], ns = counters[2];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ne + ns; i += gridDim.x * blockDim.x) {
+        float r[8];
+        float* out;
+        uint32_t cap, slot;
+        if (i < ne) {
+            const float4* w = path_words(P, next_queue[i]);
+            const float4 a = w[0], b = w[1];
+            r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = b.x; r[4] = b.y; r[5] = b.z; r[6] = 0.001f; r[7] = kInfF;
+            out = ext;
+            cap = ext_cap;
+            slot = atomicAdd(&fill[0], 1u);
+        } else {
+            const uint32_t j = i - ne;
+            r[0] = Sb.ox[j]; r[1] = Sb.oy[j]; r[2] = Sb.oz[j]; r[3] = Sb.dx[j]; r[4] = Sb.dy[j]; r[5] = Sb.dz[j];
+            r[6] = 0.001f; r[7] = Sb.tmax[j];
+            out = sh;
+            cap = sh_cap;
+            slot = atomicAdd(&fill[1], 1u);
+            if (slot < cap) sh_ignore[slot] = Sb.ignore[j];
+        }
+        if (slot < cap)
+#pragma unroll
+            for (int k = 0; k < 8; ++k) out[(size_t)k * cap + slot] = r[k];
+    }
+}
+
 // swap: next queue becomes active, counters reset (no memset nodes, graph safe)
 __global__ void k_advance(uint32_t* counters, unsigned long long* stats) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {

@@ -157,7 +157,7 @@ EXPORTED = [
     "lrt_denoiser_forward", "lrt_denoiser_loss_grad", "lrt_denoiser_adam_step", "lrt_conv3x3",
     "lrt_diff_param_count", "lrt_diff_render", "lrt_diff_adjoint", "lrt_diff_render_held",
     "lrt_load_obj", "lrt_mesh_view", "lrt_mesh_free", "lrt_generate_obj_scene", "lrt_image_write",
-    "lrt_image_to_bytes",
+    "lrt_image_to_bytes", "lrt_render_capture_rays",
 ]
 
 
@@ -457,6 +457,20 @@ class Scene:
         _check(LIB.lrt_trace_count(self._h, _p(rays), C.c_uint32(n), _p(a, C.c_uint32), _p(b, C.c_uint32),
                                    _p(c, C.c_uint32), 0))
         return a, b, c
+
+    def capture_rays(self, width, height, spp, max_depth, rr_depth, seed=1, ext_cap=1 << 22, shadow_cap=1 << 22):
+        """The rays a frame traces after its camera rays (lrt_render_capture_rays):
+        (extension rays [8, n], shadow rays [8, m], shadow ignore [m])."""
+        ext = np.zeros((8, ext_cap), np.float32)
+        sh = np.zeros((8, shadow_cap), np.float32)
+        ign = np.zeros(shadow_cap, np.int32)
+        ne, ns = C.c_uint32(), C.c_uint32()
+        prm = RenderParams(width, height, spp, 0, max_depth, rr_depth, seed, 0, 0, 0)
+        _check(LIB.lrt_render_capture_rays(self._h, C.byref(prm), _p(ext), C.c_uint32(ext_cap), _p(sh),
+                                           _p(ign, C.c_int32), C.c_uint32(shadow_cap), C.byref(ne), C.byref(ns)))
+        e = ext.reshape(-1)[: 8 * ne.value].reshape(8, ne.value).copy()
+        s = sh.reshape(-1)[: 8 * ns.value].reshape(8, ns.value).copy()
+        return e, s, ign[: ns.value].copy()
 
     def trace_closest_device(self, rays_ptr, n, obj_prim_ptr, t_ptr, uv_ptr=None):
         _check(LIB.lrt_trace_closest(self._h, C.c_void_p(rays_ptr), C.c_uint32(n), C.c_void_p(obj_prim_ptr),

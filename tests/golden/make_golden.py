@@ -164,6 +164,27 @@ def make_render_fixture():
     np.savez_compressed(OUT / "render_box_icosphere_16x16x4.npz", per_sample=r["per_sample"], fsum=r["fsum"])
 
 
+REF_IMAGE_SPP = 256
+
+
+def make_ref_image_fixture():
+    """Per-pixel mean and unbiased sample variance of the reference's own
+    render of the 256x256 box-room workload's scene (box_icosphere(seed=1,
+    subdivisions=5)): its path_integrator at its compiled depth 50 / RR from 5
+    (path_integrator.h:19,173) with its Sobol sampler, REF_IMAGE_SPP samples
+    per pixel in the src/main.cpp:118-136 loop.  Moments are taken in float64
+    from the per-sample radiance and stored as float32."""
+    g = linrt.box_icosphere(seed=1, subdivisions=5)
+    W = H = 256
+    ps = OL.reference.scene(g.desc_ptr).render(W, H, REF_IMAGE_SPP, 50, 5, per_sample=True)["per_sample"]
+    ps = ps.astype(np.float64)
+    mean = ps.mean(axis=2)
+    var = ps.var(axis=2, ddof=1)
+    np.savez_compressed(OUT / "ref_image_box_icosphere_256x256.npz", mean=mean.astype(np.float32),
+                        var=var.astype(np.float32), spp=np.int32(REF_IMAGE_SPP), max_depth=np.int32(50),
+                        rr_depth=np.int32(5))
+
+
 PHOTON_EST = ((16, 0.1, 100), (64, 0.3, 100), (64, 0.3, 3))  # (k, radius, max_depth)
 
 
@@ -211,6 +232,9 @@ def main():
         make_photon_fixture()
         make_eye_fixture()
         return
+    if sys.argv[1:] == ["ref_image"]:
+        make_ref_image_fixture()
+        return
     if sys.argv[1:] == ["readme"]:
         make_tables(README_SLICES, "tables_readme.json")
         return
@@ -222,6 +246,7 @@ def main():
     make_hits_fixture()
     make_misc_fixture(tables)
     make_render_fixture()
+    make_ref_image_fixture()
     make_photon_fixture()
     make_eye_fixture()
     print("data and fixtures written")

@@ -120,3 +120,46 @@ def test_render_oracle_is_the_reference():
     r = OL.oracle.scene(g.desc_ptr).render(16, 16, 4, 50, 5, mode=1, per_sample=True, fsum=True)
     assert np.array_equal(r["per_sample"], d["per_sample"])
     assert np.array_equal(r["fsum"], d["fsum"])
+
+
+def pooled_z(m_a, v_a, n_a, m_b, v_b, n_b):
+    """Per pixel-channel z and the pooled z of the image sums: sum(m_a - m_b) /
+    sqrt(sum(se^2)).  The per-pixel mean(z) of right-skewed radiance samples is
+    biased (a low sample mean comes with a low sample variance), the pooled z
+    is not: it is the test of a bias of the whole image."""
+    se2 = v_a / n_a + v_b / n_b
+    with np.errstate(divide="ignore", invalid="ignore"):
+        z = np.where(se2 > 0, (m_a - m_b) / np.sqrt(se2), np.where(np.abs(m_a - m_b) > 1e-6 * np.abs(m_b) + 1e-9,
+                                                                    np.inf, 0.0))
+    return z, (m_a - m_b).sum() / np.sqrt(se2.sum())
+
+
+def test_oracle_image_vs_reference_render():
+    """Independent streams: the oracle's Philox render (128 spp, depth 50, RR
+    from 5) of the middle 128 rows of the box-room scene against the
+    reference's own 256-spp render (ref_image_box_icosphere_256x256.npz, its
+    Sobol sampler).  Per pixel-channel z = (m_o - m_r) / sqrt(s2_o / n_o +
+    s2_r / n_r): |z| > 3 on no more than a binomial bound of 0.27%; the pooled
+    z of the image sums within 4 (a 0.3% bias of the image gives ~15); mean(z)
+    within 0.15 (its skew bias) and var(z) below 1.1."""
+    d = np.load(GOLD / "ref_image_box_icosphere_256x256.npz")
+    mr, vr, nr = d["mean"].astype(np.float64), d["var"].astype(np.float64), int(d["spp"])
+    H, W, _ = mr.shape
+    n = 128
+    rows = (64, 192)
+    g = linrt.box_icosphere(seed=1, subdivisions=5)
+    o = OL.oracle.scene(g.desc_ptr).render(W, H, n, int(d["max_depth"]), int(d["rr_depth"]), seed=99, mode=0,
+                                           rows=rows)
+    sl = slice(*rows)
+    mo = o["sum"][sl] / n
+    vo = np.maximum(o["sumsq"][sl] / n - mo * mo, 0.0) * n / (n - 1)
+    z, zp = pooled_z(mo, vo, n, mr[sl], vr[sl], nr)
+    z = z.ravel()
+    frac3 = np.mean(np.abs(z) > 3)
+    bound = 0.0027 + 5 * np.sqrt(0.0027 * 0.9973 / z.size)
+    print(f"oracle vs reference render: |z|>3 {frac3:.5f} (bound {bound:.5f}), pooled z {zp:+.3f}, "
+          f"mean z {z.mean():+.4f}, var z {z.var():.4f}")
+    assert frac3 <= bound
+    assert abs(zp) < 4
+    assert abs(z.mean()) < 0.15
+    assert z.var() < 1.1

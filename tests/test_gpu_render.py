@@ -1,11 +1,16 @@
-"""GPU path tracer (lrt_render) against the oracle's CPU render of the same
-scene with the same Philox sample streams.
+"""GPU path tracer (lrt_render) against the oracle and against the reference's
+own render.
 
-Per-pixel means must agree within 3 sigma of the estimated standard error,
-applied distributionally: the fraction of pixel-channels with |z| > 3 stays
-within a binomial bound of 0.27%, mean(z) ~ 0 and var(z) ~ 1.  Because both
-sides consume identical random numbers, most pixels agree far closer than
-that; the test also reports the fraction equal to 1e-4 relative.
+Two kinds of checks:
+* identical sample streams: the GPU and the oracle's CPU render draw the same
+  Philox numbers, so their per-pixel sums agree to float rounding except where
+  a device transcendental's last-ulp difference forks a path; the bar is the
+  fraction of pixel-channels equal to 1e-4 relative.
+* independent streams (the distributional 3-sigma test): the GPU's Philox
+  render against the reference's own render (its Sobol sampler; fixture made
+  by the compiled reference) with z = (mean_gpu - mean_ref) / sqrt(s2_gpu /
+  n_gpu + s2_ref / n_ref) per pixel-channel; the fraction with |z| > 3 must
+  stay within a binomial bound of 0.27%, mean(z) ~ 0 and var(z) <~ 1.
 """
 import numpy as np
 import pytest
@@ -16,17 +21,7 @@ from oracle_lib import oracle
 pytestmark = pytest.mark.gpu
 
 
-def z_scores(sum_a, sq_a, n_a, sum_b, sq_b, n_b):
-    ma, mb = sum_a / n_a, sum_b / n_b
-    va = np.maximum(sq_a / n_a - ma * ma, 0.0) * n_a / max(n_a - 1, 1)
-    vb = np.maximum(sq_b / n_b - mb * mb, 0.0) * n_b / max(n_b - 1, 1)
-    se = np.sqrt(va / n_a + vb / n_b)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        z = np.where(se > 0, (ma - mb) / se, np.where(np.abs(ma - mb) > 1e-6 * np.abs(mb) + 1e-9, np.inf, 0.0))
-    return z
-
-
-def check_z(z, label):
+def check_z(z, label, mean_bar=0.1, var_range=None):
     z = z.ravel()
     n = z.size
     frac3 = np.mean(np.abs(z) > 3)
@@ -36,36 +31,72 @@ def check_z(z, label):
     print(f"{label}: {n} pixel-channels, |z|>3 fraction {frac3:.5f} (bound {bound:.5f}), "
           f"mean z {finite.mean():+.4f}, var z {finite.var():.4f}, worst {np.sort(np.abs(finite))[-5:]}")
     assert frac3 <= bound
-    assert abs(finite.mean()) < 0.1
+    assert abs(finite.mean()) < mean_bar
+    if var_range is not None:
+        assert var_range[0] < finite.var() < var_range[1]
 
 
-@pytest.mark.parametrize("spp", [16])
-def test_box_icosphere_image(spp):
-    W = H = 64
+def test_box_icosphere_image():
+    """The 256x256, 16-spp box-room workload (depth 8, RR from depth 3) at its
+    full size, identical streams."""
+    W = H = 256
+    spp = 16
     g = linrt.box_icosphere(seed=1, subdivisions=5)
     gs = linrt.Scene(g.desc_ptr)
     s_g, q_g, st = gs.render(W, H, spp, max_depth=8, rr_depth=3, seed=7)
-    o = oracle.scene(g.desc_ptr).render(W, H, spp, 8, 3, seed=7, mode=0)
-    z = z_scores(s_g, q_g, spp, o["sum"], o["sumsq"], spp)
+    o = oracle.scene(g.desc_ptr).render(W, H, spp, 8, 3, seed=7, mode=0, threads=16)
     close = np.isclose(s_g, o["sum"], rtol=1e-4, atol=1e-6).mean()
     print(f"box_icosphere {W}x{H} {spp}spp: samples {st.samples}, ext rays {st.extension_rays}, "
-          f"shadow {st.shadow_rays} (oracle {o['ext']}, {o['shadow']}); equal to 1e-4: {close:.4f}")
-    check_z(z, "box_icosphere")
-    assert close > 0.95
+          f"shadow {st.shadow_rays} (oracle {o['ext']}, {o['shadow']}); equal to 1e-4: {close:.5f}")
+    assert close > 0.999
     assert st.samples == W * H * spp
+    assert st.extension_rays == o["ext"] or abs(st.extension_rays - o["ext"]) < 1e-3 * o["ext"]
+
+
+def test_box_icosphere_vs_reference_render():
+    """Independent streams: the GPU's Philox render (1024 spp) of the box-room
+    scene against the reference's own render of it (its Sobol sampler, 256
+    spp, its compiled depth 50 and RR from 5, path_integrator.h:19,173;
+    fixture tests/golden/ref_image_box_icosphere_256x256.npz made by the
+    compiled reference).  Per pixel-channel z: |z| > 3 within the binomial
+    bound of 0.27%; the pooled z of the image sums within 4 (a 0.3% bias of
+    the GPU image gives ~12); mean(z) within 0.15 (right-skewed samples bias
+    it by ~0.07 here) and var(z) in (0.5, 1.1) (the reference's stratified
+    Sobol samples estimate its mean better than their variance says, so var(z)
+    sits below 1).  The worst pixels are listed."""
+    from pathlib import Path
+    from test_cpu_golden import pooled_z
+    d = np.load(Path(__file__).resolve().parent / "golden" / "ref_image_box_icosphere_256x256.npz")
+    mr, vr, nr = d["mean"].astype(np.float64), d["var"].astype(np.float64), int(d["spp"])
+    H, W, _ = mr.shape
+    ng = 1024
+    g = linrt.box_icosphere(seed=1, subdivisions=5)
+    gs = linrt.Scene(g.desc_ptr)
+    s_g, q_g, _ = gs.render(W, H, ng, max_depth=int(d["max_depth"]), rr_depth=int(d["rr_depth"]), seed=2024)
+    mg = s_g / ng
+    vg = np.maximum(q_g / ng - mg * mg, 0.0) * ng / (ng - 1)
+    z, zp = pooled_z(mg, vg, ng, mr, vr, nr)
+    worst = np.argsort(np.abs(z).ravel())[-8:][::-1]
+    for k in worst:
+        y, x, c = np.unravel_index(k, z.shape)
+        print(f"  worst: pixel ({x}, {y}) channel {c}: gpu {mg[y, x, c]:.5f} reference {mr[y, x, c]:.5f} "
+              f"z {z[y, x, c]:+.3f}")
+    print(f"image mean: gpu {mg.mean():.6f}, reference {mr.mean():.6f}, pooled z {zp:+.3f}")
+    assert abs(zp) < 4
+    check_z(z, "box_icosphere vs reference render", mean_bar=0.15, var_range=(0.5, 1.1))
 
 
 def test_displaced_sphere_crop():
+    """Identical streams on a crop of the 1080p workload's scene (65k-triangle
+    mesh, seven BSDF regions, sky)."""
     W, H, spp = 96, 54, 8
     g = linrt.displaced_sphere(seed=1, segments=256, rings=128)
     gs = linrt.Scene(g.desc_ptr)
     s_g, q_g, st = gs.render(W, H, spp, max_depth=16, rr_depth=5, seed=3)
     o = oracle.scene(g.desc_ptr).render(W, H, spp, 16, 5, seed=3, mode=0)
-    z = z_scores(s_g, q_g, spp, o["sum"], o["sumsq"], spp)
     close = np.isclose(s_g, o["sum"], rtol=1e-4, atol=1e-6).mean()
     print(f"displaced_sphere {W}x{H} {spp}spp: equal to 1e-4: {close:.4f}; ext {st.extension_rays} vs {o['ext']}")
-    check_z(z, "displaced_sphere")
-    assert close > 0.9
+    assert close > 0.99
 
 
 def test_progressive_offset_matches_single_pass():
@@ -164,7 +195,7 @@ def _write_obj(path, pos, idx):
 def test_readme_scene_vs_oracle(tmp_path, name):
     """The README scene of src/main.cpp (lrt_generate_obj_scene) around an
     icosphere written as OBJ, in each README material: the GPU path tracer
-    against the oracle (Philox streams: z-test and closeness), and with the
+    against the oracle (identical Philox streams: closeness), and with the
     reference sampler every GPU sample against the oracle's reference-sampler
     sample (which the CPU tests pin to the reference itself on the bunny)."""
     pos, idx = linrt.box_icosphere(seed=1, subdivisions=3).mesh(1)
@@ -177,10 +208,8 @@ def test_readme_scene_vs_oracle(tmp_path, name):
     spp = 16
     s_g, q_g, st = sc.render(W, H, spp, 16, 5, seed=5)
     o = oracle.scene(g.desc_ptr).render(W, H, spp, 16, 5, seed=5, mode=0)
-    z = z_scores(s_g, q_g, spp, o["sum"], o["sumsq"], spp)
     close = np.isclose(s_g, o["sum"], rtol=1e-4, atol=1e-6).mean()
     print(f"README scene {name}: equal to 1e-4 {close:.4f}")
-    check_z(z, f"readme {name}")
     assert close > 0.95
     ps = oracle.scene(g.desc_ptr).render(16, 16, 2, 50, 5, mode=1, per_sample=True)["per_sample"]
     got = np.zeros_like(ps)
