@@ -178,6 +178,18 @@ int lrt_scene_info_get(const lrt_scene* scene, lrt_scene_info* info);
  * the call is asynchronous on the scene's stream (use lrt_scene_sync).      */
 int lrt_trace_closest(lrt_scene* scene, const float* rays, uint32_t n,
                       int32_t* hit_obj_prim, float* hit_t, float* hit_uv, int device_ptrs);
+/* lrt_trace_closest on device pointers that allocates nothing: no
+ * counterpart in the reference; it is for callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same rays, outputs and results as lrt_trace_closest with
+ * device_ptrs = 1 (rays, hit_obj_prim, hit_t and hit_uv are device memory on
+ * the scene's device; asynchronous on the scene's stream, lrt_scene_sync
+ * waits).  Its only scratch, the traversal's ray-fetch counter, is a
+ * 256-byte static device array per device (resolved by lrt_scene_create);
+ * every call on a device shares it, so calls that overlap on one device race.
+ * n is limited to 2^31 rays per call (LRT_ERR_ARG beyond). */
+int lrt_trace_closest_noalloc(lrt_scene* scene, const float* rays, uint32_t n,
+                              int32_t* hit_obj_prim, float* hit_t, float* hit_uv);
 /* occlusion query: occluded[i] = 1 if any surface other than those of
  * emitter ignore_emitter[i] (-1: none) lies in [tmin, tmax]
  * (path_integrator.h:113-116). ignore_emitter may be NULL. */

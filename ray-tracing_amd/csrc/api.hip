@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -15,6 +16,21 @@
 #include "eye.hip"
 #include "diff_wave.hip"
 #include "device/nn.h"
+
+// Scratch of lrt_trace_closest_noalloc: the persistent traversal's ray-fetch
+// counter lives here instead of in memory the library allocates, so that
+// callers which cannot allocate (code under graph capture, serving loops) can
+// query rays.  256 bytes per device, held until the process ends; every call
+// on a device shares its copy (calls that overlap on one device race).  Host
+// code never passes `scratch` itself to a kernel (in host code the name is a
+// stand-in, not a device address): lrt_scene_create resolves the device
+// address once per device with hipGetSymbolAddress, and the entry point uses
+// that address.
+constexpr uint32_t kNoAllocScratchWords = 64;
+static __device__ __align__(256) uint32_t scratch[kNoAllocScratchWords];
+// rays per call of lrt_trace_closest_noalloc: the fetch counter runs up to
+// n plus 64 per wave of the grid, which must not wrap 32 bits
+constexpr uint64_t kNoAllocMaxRays = 1ull << 31;
 
 // paths per render pass (2^27: ~21 GiB of path records, hit records, shadow
 // and path queues, one pass for a 64-spp 1080p frame); the environment
@@ -651,6 +667,22 @@ int lrt_rough_transmittance_slice(const void* table_file, size_t table_bytes, fl
     });
 }
 
+// device address of `scratch` per device, resolved once per device (with the
+// device current) by lrt_scene_create
+constexpr int kMaxDevices = 64;
+static uint32_t* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static void scratch_setup(int dev) {
+    require(dev < kMaxDevices, "device index beyond " + std::to_string(kMaxDevices), LRT_ERR_UNSUPPORTED);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[dev]) return;
+    void* p = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    require(p != nullptr, "hipGetSymbolAddress(scratch) returned null", LRT_ERR_HIP);
+    g_scratch_ptr[dev] = (uint32_t*)p;
+}
+
 int lrt_scene_create(int hip_device, const lrt_scene_desc* desc, lrt_scene** out) {
     return guarded([&] {
         require(desc && out, "null pointer");
@@ -659,6 +691,7 @@ int lrt_scene_create(int hip_device, const lrt_scene_desc* desc, lrt_scene** out
         require(hipGetDeviceCount(&ndev) == hipSuccess && hip_device >= 0 && hip_device < ndev,
                 "no HIP device " + std::to_string(hip_device), LRT_ERR_HIP);
         HIPCHECK(hipSetDevice(hip_device));
+        scratch_setup(hip_device);
         auto sc = std::make_unique<lrt_scene>();
         sc->device = hip_device;
         sc->cus = cu_count(hip_device);
@@ -743,6 +776,27 @@ int lrt_trace_closest(lrt_scene* sc, const float* rays, uint32_t n, int32_t* hit
             if (hit_uv) HIPCHECK(hipMemcpyAsync(hit_uv, d_uv, (size_t)n * 8, hipMemcpyDeviceToHost, sc->stream));
             HIPCHECK(hipStreamSynchronize(sc->stream));
         }
+    });
+}
+
+int lrt_trace_closest_noalloc(lrt_scene* sc, const float* rays, uint32_t n, int32_t* hit_obj_prim, float* hit_t,
+                              float* hit_uv) {
+    // no allocation on any path that succeeds: no strings are built unless a
+    // check fails, no timers, no staging; the fetch counter is `scratch` word 0
+    // (cleared by a kernel, not a memset node)
+    return guarded([&] {
+        if (!sc || !rays || !hit_obj_prim || !hit_t) throw Error(LRT_ERR_ARG, "null pointer");
+        if ((uint64_t)n > kNoAllocMaxRays)
+            throw Error(LRT_ERR_ARG, "lrt_trace_closest_noalloc: n = " + std::to_string(n) +
+                                         " rays exceeds the maximum of 2^31 rays per call");
+        if (n == 0) return;
+        uint32_t* next = (sc->device >= 0 && sc->device < kMaxDevices) ? g_scratch_ptr[sc->device] : nullptr;
+        if (!next) throw Error(LRT_ERR_INTERNAL, "scratch of the device was not set up");
+        HIPCHECK(hipSetDevice(sc->device));
+        hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, sc->stream, next, 1u, 0u);
+        hipLaunchKernelGGL(k_trace_closest, dim3(grid_for(*sc, n, 8)), dim3(kBlock), 0, sc->stream, sc->S, rays, n,
+                           hit_obj_prim, hit_t, hit_uv, next);
+        HIPCHECK(hipGetLastError());
     });
 }
 

@@ -157,7 +157,7 @@ EXPORTED = [
     "lrt_denoiser_forward", "lrt_denoiser_loss_grad", "lrt_denoiser_adam_step", "lrt_conv3x3",
     "lrt_diff_param_count", "lrt_diff_render", "lrt_diff_adjoint", "lrt_diff_render_held",
     "lrt_load_obj", "lrt_mesh_view", "lrt_mesh_free", "lrt_generate_obj_scene", "lrt_image_write",
-    "lrt_image_to_bytes", "lrt_render_capture_rays",
+    "lrt_image_to_bytes", "lrt_render_capture_rays", "lrt_trace_closest_noalloc",
 ]
 
 
@@ -475,6 +475,11 @@ class Scene:
     def trace_closest_device(self, rays_ptr, n, obj_prim_ptr, t_ptr, uv_ptr=None):
         _check(LIB.lrt_trace_closest(self._h, C.c_void_p(rays_ptr), C.c_uint32(n), C.c_void_p(obj_prim_ptr),
                                      C.c_void_p(t_ptr), C.c_void_p(uv_ptr) if uv_ptr else None, 1))
+
+    def trace_closest_noalloc_device(self, rays_ptr, n, obj_prim_ptr, t_ptr, uv_ptr=None):
+        """lrt_trace_closest_noalloc: device pointers, asynchronous, no allocation."""
+        _check(LIB.lrt_trace_closest_noalloc(self._h, C.c_void_p(rays_ptr), C.c_uint32(n), C.c_void_p(obj_prim_ptr),
+                                             C.c_void_p(t_ptr), C.c_void_p(uv_ptr) if uv_ptr else None))
 
     def sync(self):
         _check(LIB.lrt_scene_sync(self._h))

@@ -119,3 +119,26 @@ def test_oracle_kd_traversal_matches_brute_force():
     ob, tb = sc.trace_brute(rays)
     frac, bad = hit_parity(oa, ta, ob, tb)
     assert frac >= 0.999, bad[:20]
+
+
+def test_device_code_defines_registered_scratch():
+    """The library's gfx950 code object defines the static device array that
+    its host code registers for lrt_trace_closest_noalloc (a library whose
+    host and device passes saw different sources would abort when HIP loads
+    it).  Reads the library built here, with the ROCm tools."""
+    import shutil
+    import subprocess
+    import tempfile
+    bundler = Path("/opt/rocm/llvm/bin/clang-offload-bundler")
+    readelf = Path("/opt/rocm/llvm/bin/llvm-readelf")
+    if not (bundler.exists() and readelf.exists() and shutil.which("objcopy")):
+        pytest.skip("ROCm LLVM tools or objcopy not found")
+    with tempfile.TemporaryDirectory() as d:
+        fat, co = Path(d) / "fat.bin", Path(d) / "gfx950.co"
+        subprocess.run(["objcopy", "--dump-section", f".hip_fatbin={fat}", str(linrt.LIB_PATH), str(Path(d) / "x.so")],
+                       check=True, capture_output=True)
+        subprocess.run([str(bundler), "--type=o", "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--input={fat}",
+                        f"--output={co}", "--unbundle"], check=True, capture_output=True)
+        syms = subprocess.run([str(readelf), "-s", str(co)], check=True, capture_output=True, text=True).stdout
+    assert "_ZL7scratch" in syms
+    assert "k_trace_closest" in syms
