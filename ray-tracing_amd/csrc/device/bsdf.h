@@ -85,7 +85,7 @@ LRT_D f3 ggx_sample(Ggx g, f3 wi, float ux, float uy) {
     const float r = sqrtf(ux);
     const float phi = 2.0f * kPiF * uy;
     float sp, cp;
-    sincosf(phi, &sp, &cp);
+    sincos_rn(phi, &sp, &cp);
     const float dx = r * cp;
     float dy = r * sp;
     const float s = 0.5f * (1.0f + vh.z);
@@ -178,7 +178,7 @@ LRT_D f3 cos_hemi(float ux, float uy, float& pdf) {
     const float ct = sqrtf(ux), st = sqrtf(1.0f - ux);
     const float phi = 2.0f * kPiF * uy;
     float sp, cp;
-    sincosf(phi, &sp, &cp);
+    sincos_rn(phi, &sp, &cp);
     pdf = ct / kPiF;
     return f3{cp * st, sp * st, ct};
 }
@@ -186,7 +186,7 @@ LRT_D f3 cos_hemi(float ux, float uy, float& pdf) {
 // rough-transmittance slice lookup (rough_transmittance.h:17-29)
 LRT_D float trans_eval(const float* tb, float ct) {
     if (!(ct >= 0.0f)) return 0.0f;
-    const float x = powf(ct, 0.25f);
+    const float x = pow_rn(ct, 0.25f);
     if (!(x >= 0.0f && x <= 1.0f)) return 0.0f;
     float t = ((x - 0.0f) * 99.0f) / (1.0f - 0.0f);
     int k = (int)t;
@@ -645,14 +645,14 @@ LRT_D f3 refract_to(bool interior, f3 wi, float eta, float inv_eta) {
 LRT_D float sa_factor(float inv_eta, f3 wo, f3 woi) { return inv_eta * inv_eta * wo.z / woi.z; }
 LRT_D float coat_w(const lrt_material& M) {
     const f3 o = col(M.sigma_a) * (-2.0f * M.thickness);
-    const float avg = (expf(o.x) + expf(o.y) + expf(o.z)) / 3.0f;
+    const float avg = (exp_rn(o.x) + exp_rn(o.y) + exp_rn(o.z)) / 3.0f;
     return 1.0f / (avg + 1.0f);
 }
 LRT_D f3 absorb(const lrt_material& M, f3 wi, f3 wo) {
     if (is_zero(col(M.sigma_a))) return splat(1.0f);
     const float l = (1.0f / fabsf(wi.z)) + (1.0f / fabsf(wo.z));
     const f3 e = col(M.sigma_a) * (-M.thickness * l);
-    return f3{expf(e.x), expf(e.y), expf(e.z)};
+    return f3{exp_rn(e.x), exp_rn(e.y), exp_rn(e.z)};
 }
 LRT_D float coat_prob(float R, float w) {
     const float s = R * w, n = (1.0f - R) * (1.0f - w);

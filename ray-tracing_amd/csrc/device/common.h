@@ -36,6 +36,23 @@ LRT_HD bool is_zero(f3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
 LRT_HD float comp(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 LRT_HD f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
 
+// sin, cos, pow and exp of a float as the reference's std::sin(float) etc.
+// compute them on its CPU: in double precision, rounded once to float.  That
+// is the correctly rounded float except where the double result lies within a
+// double ulp of a float rounding boundary.  The host C library's float
+// versions are within 0.56 ulp (glibc: they equal the correctly rounded value
+// on 98.7% of sin/cos and 99.9% of pow/exp arguments); the device library's
+// float versions are not correctly rounded and differ from the host more
+// often, which forks sampled paths.
+LRT_D void sincos_rn(float x, float* s, float* c) {
+    double sd, cd;
+    sincos((double)x, &sd, &cd);
+    *s = (float)sd;
+    *c = (float)cd;
+}
+LRT_D float pow_rn(float x, float y) { return (float)pow((double)x, (double)y); }
+LRT_D float exp_rn(float x) { return (float)exp((double)x); }
+
 // orthonormal basis around a normal (onb.h:8-23)
 struct Frame {
     f3 s, t, n;

@@ -71,7 +71,7 @@ LRT_D bool light_sample(const DevScene& S, const DevLight& L, f3 p, float sx, fl
         const float st = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
         const float phi = 2.0f * kPiF * sy;
         float sp, cp;
-        sincosf(phi, &sp, &cp);
+        sincos_rn(phi, &sp, &cp);
         const f3 local{cp * st, sp * st, ct};
         out.dir = local.x * ld3(L.fs) + local.y * ld3(L.ft) + local.z * ld3(L.fn);
         out.pdf = L.cone_pdf;

@@ -52,7 +52,7 @@ LRT_D bool emitter_ray(const DevScene& S, const DevLights& Ls, const EmitCdf& E,
     const float ct = sqrtf(dx), st = sqrtf(1.0f - dx);
     const float phi = 2.0f * kPiF * dy;
     float sp, cp;
-    sincosf(phi, &sp, &cp);
+    sincos_rn(phi, &sp, &cp);
     const Frame f = frame_from(n);
     r = Ray{p, to_world(f, f3{cp * st, sp * st, ct})};
     const f3 power = ld3(L.radiance) * (kPiF * ob.area);

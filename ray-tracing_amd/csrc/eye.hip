@@ -12,7 +12,7 @@
 // rays (emitter_samples, :157-210) and one child per BSDF sample (:212-266).
 // A child's radiance is weighted by the product of the factors on its way
 // from the camera; every contribution goes to a fixed-point per-sample
-// accumulator (2^-36 steps), so results do not depend on the order in which
+// accumulator (2^-32 steps), so results do not depend on the order in which
 // items run.
 //
 // Sampler streams: every item draws from Philox {pixel, sample, d/4, path}
@@ -23,8 +23,13 @@
 
 namespace lrt {
 
-constexpr double kEyeFix = 68719476736.0;  // 2^36
-constexpr float kEyeClamp = 1.0e8f;
+// Fixed point of the per-sample accumulators: a step of 2^-32 (the sample's
+// radiance is rounded to float when it is resolved) and every contribution
+// clamped to 2^24, so one add moves an accumulator by at most 2^56 and a
+// sample takes 128 adds at the clamp before the int64 could wrap (radiance in
+// these scenes stays below 10^3, four orders under the clamp).
+constexpr double kEyeFix = 4294967296.0;  // 2^32
+constexpr float kEyeClamp = 16777216.0f;  // 2^24
 
 LRT_D uint32_t eye_child(uint32_t parent, uint32_t code) {
     uint32_t h = parent * 0x9e3779b9u + (code + 1u) * 0x632be5abu;
@@ -47,7 +52,7 @@ struct EyeRng {
     }
 };
 
-// non-finite contributions are dropped; magnitudes are clamped to 1e8
+// non-finite contributions are dropped; magnitudes are clamped to kEyeClamp
 LRT_D void eye_add(unsigned long long* acc, uint32_t sid, f3 c) {
     const float v[3] = {c.x, c.y, c.z};
 #pragma unroll

@@ -5,16 +5,14 @@ irradiance estimate (lrt_photon_estimate), and the photon-integrator eye pass
 
 Bars: photon estimates on identical photon lists equal to 1e-5 relative
 (the search and the summation order are the reference's, so they are normally
-bit-exact).  Traced photons: the device's sinf/cosf differ from the host
-library's by an ulp now and then, so sampled directions agree to ~1e-7 and the
-difference grows along a path: >= 99% of photon records within 1e-3 relative
-and <= 1% of particles forked (some field off by more than 1e-2) for the
-global map; >= 90% and <= 3% for the caustic map, whose delta bounces off the
-displaced mesh amplify the ulps; the maps estimate the oracle maps' irradiance
-within 2% (global) and 8% (caustic: independent seeds differ by 12%).  Eye pass on
-the same photon maps: the sphere room (analytic shapes) >= 99% of
-pixel-channels within 1e-4 relative; the displaced-sphere box, whose glossy
-chains and k-nearest sets amplify those ulps, >= 97% within 1e-2; both with
+bit-exact).  Traced photons: the device computes sin/cos/pow/exp in double and
+rounds once, as the host library nearly always does (glibc's float sin/cos
+differ from the correctly rounded value on ~1.3% of arguments), so a few
+sampled directions still differ by an ulp and the difference can grow along
+a path: for both maps >= 99% of photon records within 1e-3 relative, <= 1% of
+particles forked (some field off by more than 1e-2), and the flux density of
+the two maps within 2%.  Eye pass on the same photon maps: the sphere room
+and the displaced-sphere box >= 99% of pixel-channels within 1e-4 relative,
 |z| > 3 on at most 1% of pixel-channels.
 """
 from pathlib import Path
@@ -86,23 +84,34 @@ def test_photon_trace_vs_oracle(pbox):
         parts = np.unique(odp[:, 1])
         print(f"  {len(common)} common records, {np.mean(np.all(rel == 0, axis=1)):.2%} bit-exact, "
               f"{near.mean():.4%} within 1e-3, particles forked: {len(forked)} of {len(parts)}")
-        # caustic paths pass delta bounces off the displaced mesh, which
-        # amplify direction ulps into forks more often
-        near_bar, fork_bar = (0.99, 0.01) if kind == linrt.PHOTON_SURFACE else (0.90, 0.03)
-        assert near.mean() >= near_bar
-        assert len(forked) <= fork_bar * len(parts)
-        # the two maps estimate the same irradiance
+        assert near.mean() >= 0.99
+        assert len(forked) <= 0.01 * len(parts)
+        # the two maps carry the same flux: exact kernel density (every photon
+        # within r weighted by (1 - d^2/r^2)^2, power / shot) at 5000 points.
+        # Not the reference's k-nearest estimator: its search is approximate
+        # (split ties, the far-child plane test), so a change of a few photons
+        # reshapes the tree and moves its sum over these points by up to 4% on
+        # the caustic map (dropping 33 particles' photons from the oracle's own
+        # map does), while this sum moves by 0.2%.  The estimator itself is
+        # checked bit-exact on identical photons (test_estimate_large_map_vs_oracle).
         rng = np.random.default_rng(kind)
         pick = rng.integers(0, len(oph), 5000)
-        q = np.concatenate([oph[pick, :3] + rng.normal(scale=0.05, size=(5000, 3)).astype(np.float32),
-                            oph[pick, 6:9]], 1).astype(np.float32)
-        e_dev = OL.photon_estimate(OL.oracle, ph, dp, 1.0 / shot, q, 0.1, 100, 64)
-        e_orc = OL.photon_estimate(OL.oracle, oph, odp, 1.0 / oshot, q, 0.1, 100, 64)
-        ratio = e_dev.sum() / e_orc.sum()
-        print(f"  irradiance at 5000 points: device map / oracle map = {ratio:.5f}")
-        # the caustic map is a few thousand high-power photons: the same sum
-        # from independent oracle seeds spreads by +-12%
-        assert abs(ratio - 1) < (0.02 if kind == linrt.PHOTON_SURFACE else 0.08)
+        q = oph[pick, :3] + rng.normal(scale=0.05, size=(5000, 3)).astype(np.float32)
+        ratio = _density(ph, q).sum() * oshot / (_density(oph, q).sum() * shot)
+        print(f"  flux density at 5000 points: device map / oracle map = {ratio:.5f}")
+        assert abs(ratio - 1) < 0.02
+
+
+def _density(ph, q, r=0.1):
+    from scipy.spatial import cKDTree
+    pos = ph[:, :3].astype(np.float64)
+    pw = ph[:, 9:12].astype(np.float64).sum(1)
+    out = np.zeros(len(q))
+    for i, idx in enumerate(cKDTree(pos).query_ball_point(q.astype(np.float64), r)):
+        if idx:
+            d2 = ((pos[idx] - q[i]) ** 2).sum(1)
+            out[i] = (pw[idx] * (1 - d2 / (r * r)) ** 2).sum()
+    return out
 
 
 def test_estimate_large_map_vs_oracle(pbox):
@@ -169,7 +178,7 @@ def test_eye_pass_photon_box_vs_oracle(pbox):
     s, q, st = sc.render_photon(W, H, spp, 128, seed=3)
     maps = [(gp, gd, 1.0 / gs, 0.05 * R), (cp, cd, 1.0 / cs, 0.0125 * R)]
     ps = orc.render_photon(W, H, spp, maps, mode=0, seed=3)
-    compare_images(s, q, ps, "photon box", 1e-2, 0.97)
+    compare_images(s, q, ps, "photon box", 1e-4, 0.99)
 
 
 def test_photon_integrator_host_mirror():
