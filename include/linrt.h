@@ -241,6 +241,20 @@ int lrt_fresnel_diffuse_reflectance(float eta, float* out);
 int lrt_rough_transmittance_slice(const void* table_file, size_t table_bytes,
                                   float eta, float alpha, float* out101);
 
+/* ---- sky model state (ArHosekSkyModel.cc:147-289, 602-635) --------------
+ * Cooks the Hosek-Wilkie RGB sky state that arhosek_rgb_skymodelstate_alloc_init
+ * computes: config27 = 9 configuration values per channel (R, G, B), radiance3
+ * = the radiance scale per channel, from the model's RGB datasets
+ * (coefficients: 3 x 1080 doubles, radiances: 3 x 120 doubles, the layout of
+ * ray-tracing_amd/data/hosek_rgb_dataset.json).  turbidity in [1, 10], albedo
+ * in [0, 1]; an elevation below 0 gives NaN, as in the model.  The result fills
+ * lrt_light_desc.config / channel_radiance of a sky light. */
+int lrt_sky_cook(const double* coefficients, const double* radiances, double turbidity, double albedo,
+                 double elevation, double* config27, double* radiance3);
+/* the solar elevation env_emitter cooks the model at (env_emitter.h:13-30):
+ * asin of the normalised (float) sun direction's y, clamped to [-1, 1] */
+int lrt_sky_elevation(const float* sun_dir, double* elevation);
+
 /* ---- sky emitter evaluation (env_emitter.h:71-88) ---------------------
  * rgb radiance of n world directions for light lights[0] of a scene
  * description (device evaluation) */
@@ -400,12 +414,33 @@ int lrt_mesh_free(lrt_mesh* mesh);
  * and HDR (Radiance RGBE) keep the rows in memory order, as the reference's
  * writer does; JPEG (lossy) is not written.  Lossless additions: PFM and
  * raw little-endian float32 with a JSON sidecar (path + ".json"), rows
- * bottom first (flipped first when asked). */
+ * bottom first (flipped first when asked); OpenEXR (uncompressed scanlines,
+ * 32-bit float B, G, R channels, rows top first, so a bottom-up frame is
+ * stored upright unless flipped). */
 enum { LRT_IMAGE_PNG = 0, LRT_IMAGE_JPEG = 1, LRT_IMAGE_PPM = 2, LRT_IMAGE_HDR = 3, LRT_IMAGE_PFM = 4,
-       LRT_IMAGE_RAW = 5 };
+       LRT_IMAGE_RAW = 5, LRT_IMAGE_EXR = 6 };
 int lrt_image_write(const char* path, const float* rgb, uint32_t width, uint32_t height, int format,
                     int flip_vertically);
 int lrt_image_to_bytes(const float* rgb, uint32_t width, uint32_t height, int flip_vertically, uint8_t* out);
+
+/* ---- FP64 accumulators of a progressive render (no counterpart in the
+ *      reference, which keeps its FP32 frame in memory, src/main.cpp:105) --
+ * The per-pixel sums (and sums of squares) lrt_render adds into, written as
+ * little-endian float64 (sum, then sumsq when non-null; rows bottom first)
+ * with a JSON sidecar path + ".json" holding the shape and what a resumed
+ * render needs: continue with spp_offset = next_sample and the same seed and
+ * depth limits.  lrt_accum_read with sum = NULL reads the sidecar only;
+ * sumsq is zero-filled when the file holds no sums of squares. */
+typedef struct lrt_accum_info {
+    uint32_t width, height;
+    uint64_t spp_done;     /* samples per pixel summed so far */
+    uint64_t next_sample;  /* index of the next sample to render */
+    uint64_t seed;
+    uint32_t max_depth, rr_depth;
+    int32_t has_sumsq;     /* read: the file holds sums of squares */
+} lrt_accum_info;
+int lrt_accum_write(const char* path, const double* sum, const double* sumsq, const lrt_accum_info* info);
+int lrt_accum_read(const char* path, double* sum, double* sumsq, lrt_accum_info* info);
 
 /* ---- Philox-4x32-10 (the counter-based sampler of every kernel) -------- */
 int lrt_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -954,6 +954,28 @@ int lrt_bsdf_flags(const lrt_material* materials, uint32_t n_materials, int32_t 
     });
 }
 
+int lrt_sky_cook(const double* coefficients, const double* radiances, double turbidity, double albedo,
+                 double elevation, double* config27, double* radiance3) {
+    return guarded([&] {
+        require(coefficients && radiances && config27 && radiance3, "null pointer");
+        require(turbidity >= 1.0 && turbidity <= 10.0, "turbidity outside [1, 10]");
+        require(albedo >= 0.0 && albedo <= 1.0, "ground albedo outside [0, 1]");
+        sky_cook(coefficients, radiances, turbidity, albedo, elevation, config27, radiance3);
+    });
+}
+
+int lrt_sky_elevation(const float* sun_dir, double* elevation) {
+    return guarded([&] {
+        require(sun_dir && elevation, "null pointer");
+        // vec3 normalize (each component over the float length), then
+        // std::asin of the clamped float y
+        const float l = std::sqrt(sun_dir[0] * sun_dir[0] + sun_dir[1] * sun_dir[1] + sun_dir[2] * sun_dir[2]);
+        require(l > 0.0f, "zero sun direction");
+        const float y = sun_dir[1] / l;
+        *elevation = (double)std::asin(std::min(std::max(y, -1.0f), 1.0f));
+    });
+}
+
 int lrt_sky_eval(int hip_device, const lrt_light_desc* sky, const float* dirs, uint32_t n, float* rgb) {
     return guarded([&] {
         require(sky && dirs && rgb, "null pointer");
@@ -2053,9 +2075,26 @@ int lrt_image_write(const char* path, const float* rgb, uint32_t width, uint32_t
         require(path && rgb, "null pointer");
         require(width >= 1 && height >= 1, "empty image");
         if (format == LRT_IMAGE_JPEG) throw Error(LRT_ERR_UNSUPPORTED, "JPEG is not written (lossy)");
-        require(format >= LRT_IMAGE_PNG && format <= LRT_IMAGE_RAW, "unknown image format");
+        require(format >= LRT_IMAGE_PNG && format <= LRT_IMAGE_EXR, "unknown image format");
         std::string err;
         if (!write_image(path, rgb, width, height, format, flip_vertically != 0, err)) throw Error(LRT_ERR_ARG, err);
+    });
+}
+
+int lrt_accum_write(const char* path, const double* sum, const double* sumsq, const lrt_accum_info* info) {
+    return guarded([&] {
+        require(path && sum && info, "null pointer");
+        require(info->width >= 1 && info->height >= 1 && info->width <= 65536 && info->height <= 65536, "bad image size");
+        std::string err;
+        if (!write_accum(path, sum, sumsq, *info, err)) throw Error(LRT_ERR_ARG, err);
+    });
+}
+
+int lrt_accum_read(const char* path, double* sum, double* sumsq, lrt_accum_info* info) {
+    return guarded([&] {
+        require(path && info, "null pointer");
+        std::string err;
+        if (!read_accum(path, sum, sumsq, *info, err)) throw Error(LRT_ERR_ARG, err);
     });
 }
 

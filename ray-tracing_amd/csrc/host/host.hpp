@@ -82,10 +82,21 @@ bool load_obj(const char* path, std::vector<float>& positions, std::vector<uint3
 uint8_t image_to_byte(float value);
 void image_to_bytes(const float* rgb, uint32_t w, uint32_t h, bool flip, uint8_t* out);
 bool write_image(const char* path, const float* rgb, uint32_t w, uint32_t h, int format, bool flip, std::string& err);
+// FP64 accumulators of a progressive render with their JSON sidecar
+bool write_accum(const char* path, const double* sum, const double* sumsq, const lrt_accum_info& info,
+                 std::string& err);
+bool read_accum(const char* path, double* sum, double* sumsq, lrt_accum_info& info, std::string& err);
 
 // direction numbers of the reference sampler's 16-dimensional Gray-code Sobol
 // sequence (sobol.h:43-106): out[d * 32 + b] for dimension d, bit b
 void sobol_directions(uint32_t out[16 * 32]);
+
+// Hosek-Wilkie RGB sky state from the model's RGB datasets (per channel 1080
+// coefficients and 120 radiances): 9 configuration values and the radiance
+// scale per channel (ArHosekSkyModel.cc:602-635)
+constexpr size_t kHosekCoefficients = 2 * 10 * 6 * 9, kHosekRadiances = 2 * 10 * 6;
+void sky_cook(const double* coefficients, const double* radiances, double turbidity, double albedo, double elevation,
+              double* config27, double* radiance3);
 
 void set_error(const std::string& msg);
 

@@ -69,6 +69,75 @@ uint8_t trunc_u8(float v) {
     return (uint8_t)(uint32_t)(int32_t)v;
 }
 
+// OpenEXR 2 single-part scanline file: uncompressed, one scanline per block,
+// channels B, G, R (the format lists channels alphabetically) as 32-bit float,
+// rows top first (lineOrder INCREASING_Y).  Frame rows are stored bottom first,
+// so file row y is frame row h - 1 - y (row y when the input is flipped).
+void put_le32(std::vector<uint8_t>& o, uint32_t v) {
+    for (int i = 0; i < 4; ++i) o.push_back((uint8_t)(v >> (8 * i)));
+}
+void put_le64(std::vector<uint8_t>& o, uint64_t v) {
+    for (int i = 0; i < 8; ++i) o.push_back((uint8_t)(v >> (8 * i)));
+}
+void put_f32(std::vector<uint8_t>& o, float v) {
+    uint32_t b;
+    std::memcpy(&b, &v, 4);
+    put_le32(o, b);
+}
+void exr_attr(std::vector<uint8_t>& o, const char* name, const char* type, const std::vector<uint8_t>& value) {
+    o.insert(o.end(), name, name + std::strlen(name) + 1);
+    o.insert(o.end(), type, type + std::strlen(type) + 1);
+    put_le32(o, (uint32_t)value.size());
+    o.insert(o.end(), value.begin(), value.end());
+}
+
+bool write_exr(const char* path, const float* rgb, uint32_t w, uint32_t h, bool flip, std::string& err) {
+    std::vector<uint8_t> o;
+    put_le32(o, 20000630u);  // magic
+    put_le32(o, 2u);         // version 2, single-part scanline
+    std::vector<uint8_t> v;
+    for (const char* ch : {"B", "G", "R"}) {
+        v.push_back((uint8_t)ch[0]);
+        v.push_back(0);
+        put_le32(v, 2u);  // FLOAT
+        for (int i = 0; i < 4; ++i) v.push_back(0);  // pLinear, reserved
+        put_le32(v, 1u);  // x sampling
+        put_le32(v, 1u);  // y sampling
+    }
+    v.push_back(0);
+    exr_attr(o, "channels", "chlist", v);
+    exr_attr(o, "compression", "compression", {0});
+    v.clear();
+    for (uint32_t x : {0u, 0u, w - 1, h - 1}) put_le32(v, x);
+    exr_attr(o, "dataWindow", "box2i", v);
+    exr_attr(o, "displayWindow", "box2i", v);
+    exr_attr(o, "lineOrder", "lineOrder", {0});
+    v.clear();
+    put_f32(v, 1.0f);
+    exr_attr(o, "pixelAspectRatio", "float", v);
+    v.clear();
+    put_f32(v, 0.0f);
+    put_f32(v, 0.0f);
+    exr_attr(o, "screenWindowCenter", "v2f", v);
+    v.clear();
+    put_f32(v, 1.0f);
+    exr_attr(o, "screenWindowWidth", "float", v);
+    o.push_back(0);  // end of header
+    const uint32_t line_bytes = 3 * 4 * w;
+    const uint64_t table = o.size(), first = table + 8ull * h;
+    for (uint32_t y = 0; y < h; ++y) put_le64(o, first + (uint64_t)y * (8 + line_bytes));
+    for (uint32_t y = 0; y < h; ++y) {
+        put_le32(o, y);
+        put_le32(o, line_bytes);
+        const uint32_t src = flip ? y : h - 1 - y;
+        const float* row = rgb + 3 * (size_t)w * src;
+        for (int c : {2, 1, 0})
+            for (uint32_t x = 0; x < w; ++x) put_f32(o, row[3 * (size_t)x + c]);
+    }
+    if (!write_file(path, o.data(), o.size())) { err = std::string("cannot write ") + path; return false; }
+    return true;
+}
+
 }  // namespace
 
 uint8_t image_to_byte(float value) {
@@ -184,8 +253,90 @@ bool write_image(const char* path, const float* rgb, uint32_t w, uint32_t h, int
         }
         return true;
     }
+    if (format == LRT_IMAGE_EXR) return write_exr(path, rgb, w, h, flip, err);
     err = "unsupported image format";
     return false;
+}
+
+// ---- accumulators of a progressive render ----------------------------------
+// path: little-endian float64, the per-pixel sums (3 * w * h, the frame's
+// rows bottom first) followed by the sums of squares when kept; path + ".json":
+// the shape and what a resumed render needs (samples done, next sample index,
+// seed, depth limits).
+bool write_accum(const char* path, const double* sum, const double* sumsq, const lrt_accum_info& info,
+                 std::string& err) {
+    const size_t n = 3 * (size_t)info.width * info.height;
+    FILE* f = std::fopen(path, "wb");
+    if (!f) { err = std::string("cannot write ") + path; return false; }
+    bool ok = std::fwrite(sum, sizeof(double), n, f) == n;
+    if (ok && sumsq) ok = std::fwrite(sumsq, sizeof(double), n, f) == n;
+    ok = std::fclose(f) == 0 && ok;
+    if (!ok) { err = std::string("cannot write ") + path; return false; }
+    char side[768];
+    std::snprintf(side, sizeof(side),
+                  "{\"width\": %u, \"height\": %u, \"channels\": 3, \"dtype\": \"float64\", \"byte_order\": \"little\", "
+                  "\"row_order\": \"bottom_up\", \"arrays\": [%s], \"spp_done\": %llu, \"next_sample\": %llu, "
+                  "\"seed\": %llu, \"max_depth\": %u, \"rr_depth\": %u}\n",
+                  info.width, info.height, sumsq ? "\"sum\", \"sumsq\"" : "\"sum\"", (unsigned long long)info.spp_done,
+                  (unsigned long long)info.next_sample, (unsigned long long)info.seed, info.max_depth, info.rr_depth);
+    const std::string jp = std::string(path) + ".json";
+    if (!write_file(jp.c_str(), side, std::strlen(side))) { err = "cannot write " + jp; return false; }
+    return true;
+}
+
+namespace {
+// the unsigned integer after "key": in a flat JSON object; false if absent
+bool json_uint(const std::string& doc, const char* key, unsigned long long& out) {
+    const std::string k = std::string("\"") + key + "\"";
+    size_t at = doc.find(k);
+    if (at == std::string::npos) return false;
+    at = doc.find(':', at + k.size());
+    if (at == std::string::npos) return false;
+    const char* p = doc.c_str() + at + 1;
+    while (*p == ' ') ++p;
+    if (*p < '0' || *p > '9') return false;
+    char* end = nullptr;
+    out = std::strtoull(p, &end, 10);
+    return end != p;
+}
+}  // namespace
+
+bool read_accum(const char* path, double* sum, double* sumsq, lrt_accum_info& info, std::string& err) {
+    const std::string jp = std::string(path) + ".json";
+    FILE* f = std::fopen(jp.c_str(), "rb");
+    if (!f) { err = "cannot read " + jp; return false; }
+    std::string doc;
+    char buf[512];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof(buf), f)) > 0) doc.append(buf, got);
+    std::fclose(f);
+    unsigned long long w, h, done, next, seed, md, rr;
+    if (doc.find("\"float64\"") == std::string::npos || !json_uint(doc, "width", w) || !json_uint(doc, "height", h) ||
+        !json_uint(doc, "spp_done", done) || !json_uint(doc, "next_sample", next) || !json_uint(doc, "seed", seed) ||
+        !json_uint(doc, "max_depth", md) || !json_uint(doc, "rr_depth", rr) || w == 0 || h == 0 || w > 65536 || h > 65536) {
+        err = "not a float64 accumulator sidecar: " + jp;
+        return false;
+    }
+    info.width = (uint32_t)w;
+    info.height = (uint32_t)h;
+    info.spp_done = done;
+    info.next_sample = next;
+    info.seed = seed;
+    info.max_depth = (uint32_t)md;
+    info.rr_depth = (uint32_t)rr;
+    info.has_sumsq = doc.find("\"sumsq\"") != std::string::npos ? 1 : 0;
+    if (!sum) return true;
+    const size_t n = 3 * (size_t)w * h;
+    f = std::fopen(path, "rb");
+    if (!f) { err = std::string("cannot read ") + path; return false; }
+    bool ok = std::fread(sum, sizeof(double), n, f) == n;
+    if (ok && sumsq) {
+        if (info.has_sumsq) ok = std::fread(sumsq, sizeof(double), n, f) == n;
+        else std::memset(sumsq, 0, n * sizeof(double));
+    }
+    std::fclose(f);
+    if (!ok) { err = std::string("truncated accumulator file ") + path; return false; }
+    return true;
 }
 
 }  // namespace lrt

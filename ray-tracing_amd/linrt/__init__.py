@@ -157,7 +157,8 @@ EXPORTED = [
     "lrt_denoiser_forward", "lrt_denoiser_loss_grad", "lrt_denoiser_adam_step", "lrt_conv3x3",
     "lrt_diff_param_count", "lrt_diff_render", "lrt_diff_adjoint", "lrt_diff_render_held",
     "lrt_load_obj", "lrt_mesh_view", "lrt_mesh_free", "lrt_generate_obj_scene", "lrt_image_write",
-    "lrt_image_to_bytes", "lrt_render_capture_rays", "lrt_trace_closest_noalloc",
+    "lrt_image_to_bytes", "lrt_render_capture_rays", "lrt_trace_closest_noalloc", "lrt_sky_cook",
+    "lrt_sky_elevation", "lrt_accum_write", "lrt_accum_read",
 ]
 
 
@@ -241,6 +242,57 @@ def load_sky(name="sky_t3_a0.3_e30.json"):
     return L
 
 
+_HOSEK = None
+
+
+def hosek_dataset():
+    """The Hosek-Wilkie RGB datasets (coefficients [3, 1080], radiances [3, 120])
+    from data/hosek_rgb_dataset.json (BSD-3, notice in the file)."""
+    global _HOSEK
+    if _HOSEK is None:
+        d = json.loads((DATA_DIR / "hosek_rgb_dataset.json").read_text())
+        _HOSEK = (np.ascontiguousarray(d["coefficients"], np.float64), np.ascontiguousarray(d["radiances"], np.float64))
+    return _HOSEK
+
+
+def sky_elevation(sun_dir):
+    """Solar elevation the sky emitter cooks its model at (lrt_sky_elevation)."""
+    e = C.c_double()
+    _check(LIB.lrt_sky_elevation(_p(np.ascontiguousarray(sun_dir, np.float32)), C.byref(e)))
+    return e.value
+
+
+def sky_cook(turbidity, albedo, elevation):
+    """Hosek-Wilkie RGB state (lrt_sky_cook): config [3, 9], radiance [3]."""
+    coef, rad = hosek_dataset()
+    cfg = np.zeros((3, 9), np.float64)
+    r3 = np.zeros(3, np.float64)
+    _check(LIB.lrt_sky_cook(_p(coef, C.c_double), _p(rad, C.c_double), C.c_double(turbidity), C.c_double(albedo),
+                            C.c_double(elevation), _p(cfg, C.c_double), _p(r3, C.c_double)))
+    return cfg, r3
+
+
+def cook_sky(sun_dir, turbidity=3.0, ground_albedo=0.3, scale=0.05):
+    """A sky light (env_emitter, env_emitter.h:13-30) cooked by the library:
+    the sun direction, turbidity and albedo are float, as the emitter keeps
+    them, and the model is cooked at the emitter's elevation."""
+    L = LightDesc()
+    L.kind = LIGHT_SKY
+    L.object = -1
+    for i in range(3):
+        L.sun_dir[i] = float(sun_dir[i])
+    L.turbidity = float(turbidity)
+    L.ground_albedo = float(ground_albedo)
+    L.scale = float(scale)
+    # the emitter passes its float turbidity and albedo to the double cook
+    cfg, r3 = sky_cook(L.turbidity, L.ground_albedo, sky_elevation(np.array(L.sun_dir[:], np.float32)))
+    for c in range(3):
+        for k in range(9):
+            L.config[c][k] = cfg[c, k]
+        L.channel_radiance[c] = r3[c]
+    return L
+
+
 class Generated:
     """A seeded scene owned by the library; ``desc`` stays valid while alive."""
 
@@ -306,12 +358,41 @@ def load_obj(path):
     return pos, idx, cn
 
 
-IMAGE_FORMATS = {"png": 0, "jpeg": 1, "ppm": 2, "hdr": 3, "pfm": 4, "raw": 5}
+IMAGE_FORMATS = {"png": 0, "jpeg": 1, "ppm": 2, "hdr": 3, "pfm": 4, "raw": 5, "exr": 6}
+
+
+class AccumInfo(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("spp_done", C.c_uint64),
+                ("next_sample", C.c_uint64), ("seed", C.c_uint64), ("max_depth", C.c_uint32),
+                ("rr_depth", C.c_uint32), ("has_sumsq", C.c_int32)]
+
+
+def write_accum(path, sum_rgb, sumsq_rgb=None, spp_done=0, next_sample=None, seed=1, max_depth=16, rr_depth=5):
+    """lrt_accum_write: float64 sums [H, W, 3] (and sums of squares) of a
+    progressive render with their JSON sidecar (path + ".json")."""
+    s = np.ascontiguousarray(sum_rgb, np.float64)
+    q = None if sumsq_rgb is None else np.ascontiguousarray(sumsq_rgb, np.float64)
+    H, W = s.shape[:2]
+    info = AccumInfo(W, H, spp_done, spp_done if next_sample is None else next_sample, seed, max_depth, rr_depth, 0)
+    _check(LIB.lrt_accum_write(str(path).encode(), _p(s, C.c_double), _p(q, C.c_double), C.byref(info)))
+
+
+def read_accum(path, info_only=False):
+    """lrt_accum_read: (sum [H, W, 3], sumsq [H, W, 3] or None, info dict)."""
+    info = AccumInfo()
+    _check(LIB.lrt_accum_read(str(path).encode(), None, None, C.byref(info)))
+    meta = {k: getattr(info, k) for k, _ in AccumInfo._fields_}
+    if info_only:
+        return None, None, meta
+    s = np.zeros((info.height, info.width, 3), np.float64)
+    q = np.zeros_like(s) if info.has_sumsq else None
+    _check(LIB.lrt_accum_read(str(path).encode(), _p(s, C.c_double), _p(q, C.c_double), C.byref(info)))
+    return s, q, meta
 
 
 def write_image(path, rgb, fmt=None, flip=False):
     """lrt_image_write: rgb float [H, W, 3] (row 0 first); fmt from the file
-    suffix by default (png, ppm, hdr, pfm, raw).  The reference's main.cpp
+    suffix by default (png, ppm, hdr, pfm, raw, exr).  The reference's main.cpp
     saves its bottom-up frame as PNG with flip=True."""
     rgb = np.ascontiguousarray(rgb, np.float32)
     fmt = fmt or str(path).rsplit(".", 1)[-1].lower()

@@ -4,6 +4,7 @@ sample, ray hits, Hosek sky radiance, fdr, rough-transmittance slices, the
 reference sampler sequence and per-sample path radiance.  These run anywhere
 (no reference tree, no GPU)."""
 import ctypes as C
+import math
 from pathlib import Path
 
 import numpy as np
@@ -163,3 +164,54 @@ def test_oracle_image_vs_reference_render():
     assert abs(zp) < 4
     assert abs(z.mean()) < 0.15
     assert z.var() < 1.1
+
+
+def _sky_json_pairs():
+    import json
+    data = Path(linrt.__file__).resolve().parent.parent / "data"
+    out = [json.loads((data / "sky_t3_a0.3_e30.json").read_text())]
+    out += json.loads((data / "skies_train.json").read_text())["skies"]
+    return out
+
+
+def test_sky_cook_equals_reference_cooked_skies():
+    """lrt_sky_cook (the library's own cook from the vendored RGB datasets)
+    reproduces, bit for bit, every sky state the reference's
+    arhosek_rgb_skymodelstate_alloc_init cooked into the committed data."""
+    skies = _sky_json_pairs()
+    for d in skies:
+        L = linrt.cook_sky(d["sun_dir"], d["turbidity"], d["ground_albedo"], d["scale"])
+        cfg = np.array([[L.config[c][k] for k in range(9)] for c in range(3)])
+        rad = np.array(L.channel_radiance[:])
+        assert np.array_equal(cfg, np.array(d["config"])), d
+        assert np.array_equal(rad, np.array(d["channel_radiance"])), d
+    print(f"{len(skies)} skies bit-identical")
+
+
+def test_sky_cook_layout_against_control_points():
+    """At elevation pi/2 the quintic's parameter is 1, so only the last control
+    point counts; at integer turbidity and albedo 0 or 1 no blending is left:
+    the state is that control point of the dataset itself.  Between corners it
+    is the bilinear blend (checked in float64 with numpy, to 1e-15)."""
+    coef, rad = linrt.hosek_dataset()
+    c4 = coef.reshape(3, 2, 10, 6, 9)
+    r4 = rad.reshape(3, 2, 10, 6)
+    for t in (1, 4, 10):
+        for a in (0, 1):
+            cfg, r3 = linrt.sky_cook(float(t), float(a), math.pi / 2)
+            assert np.array_equal(cfg, c4[:, a, t - 1, 5, :])
+            assert np.array_equal(r3, r4[:, a, t - 1, 5])
+    t, a = 3.25, 0.4
+    cfg, r3 = linrt.sky_cook(t, a, math.pi / 2)
+    w = np.array([[(1 - a) * 0.75, a * 0.75], [(1 - a) * 0.25, a * 0.25]])  # [turbidity 3/4][albedo]
+    want = sum(w[i, j] * c4[:, j, 2 + i, 5, :] for i in range(2) for j in range(2))
+    assert np.allclose(cfg, want, rtol=1e-15, atol=1e-15)
+    # elevation 0: the first control point
+    cfg0, _ = linrt.sky_cook(2.0, 0.0, 0.0)
+    assert np.array_equal(cfg0, c4[:, 0, 1, 0, :])
+    # out of range and below the horizon
+    with pytest.raises(linrt.LinrtError):
+        linrt.sky_cook(0.5, 0.3, 0.5)
+    with pytest.raises(linrt.LinrtError):
+        linrt.sky_cook(3.0, 1.5, 0.5)
+    assert np.isnan(linrt.sky_cook(3.0, 0.3, -0.1)[0]).all()

@@ -148,3 +148,91 @@ def test_jpeg_is_not_written(tmp_path):
     with pytest.raises(linrt.LinrtError) as e:
         linrt.write_image(tmp_path / "a.jpg", frame(), fmt="jpeg")
     assert e.value.code == linrt.ERR_UNSUPPORTED
+
+
+def read_exr(data):
+    """Minimal OpenEXR reader (single-part scanline, uncompressed, FLOAT or
+    HALF channels) written from the file layout: magic, version, attributes
+    (name, type, size, value) until an empty name, the line offset table,
+    then (y, size, channel rows) blocks.  Returns (header dict, {channel:
+    [H, W] array}) with rows in file order."""
+    assert struct.unpack("<I", data[:4])[0] == 20000630
+    ver, = struct.unpack("<I", data[4:8])
+    assert ver & 0xFF == 2 and not ver & 0x1200  # version 2, scanline, single part
+    at, hdr = 8, {}
+    while data[at] != 0:
+        e = data.index(b"\0", at)
+        name = data[at:e].decode()
+        t = data.index(b"\0", e + 1)
+        typ = data[e + 1:t].decode()
+        size, = struct.unpack("<i", data[t + 1:t + 5])
+        hdr[name] = (typ, data[t + 5:t + 5 + size])
+        at = t + 5 + size
+    at += 1
+    chans, p, raw = [], 0, hdr["channels"][1]
+    while raw[p] != 0:
+        e = raw.index(b"\0", p)
+        pt, = struct.unpack("<i", raw[e + 1:e + 5])
+        xs, ys = struct.unpack("<ii", raw[e + 9:e + 17])
+        assert xs == 1 and ys == 1
+        chans.append((raw[p:e].decode(), pt))
+        p = e + 17
+    assert hdr["compression"][1] == b"\0"
+    x0, y0, x1, y1 = struct.unpack("<4i", hdr["dataWindow"][1])
+    w, h = x1 - x0 + 1, y1 - y0 + 1
+    offsets = struct.unpack(f"<{h}Q", data[at:at + 8 * h])
+    out = {c: np.zeros((h, w), np.float32) for c, _ in chans}
+    for i, off in enumerate(offsets):
+        y, n = struct.unpack("<ii", data[off:off + 8])
+        q = off + 8
+        for c, pt in chans:
+            dt, sz = (np.dtype("<f4"), 4) if pt == 2 else (np.dtype("<f2"), 2)
+            out[c][y - y0] = np.frombuffer(data[q:q + sz * w], dt)
+            q += sz * w
+        assert q - off - 8 == n
+    return hdr, out
+
+
+def test_exr_roundtrip(tmp_path):
+    """EXR holds the frame exactly (float32), upright: file row 0 is the
+    frame's top row (frame rows are bottom first), or the input's row 0 when
+    the input is flipped; the required header attributes are present."""
+    img = frame()
+    h, w, _ = img.shape
+    for flip in (False, True):
+        p = tmp_path / f"f{int(flip)}.exr"
+        linrt.write_image(p, img, flip=flip)
+        hdr, ch = read_exr(p.read_bytes())
+        for k in ("channels", "compression", "dataWindow", "displayWindow", "lineOrder", "pixelAspectRatio",
+                  "screenWindowCenter", "screenWindowWidth"):
+            assert k in hdr
+        assert list(ch) == ["B", "G", "R"]
+        got = np.stack([ch["R"], ch["G"], ch["B"]], axis=-1)
+        want = img if flip else img[::-1]
+        assert np.array_equal(got, want, equal_nan=True)
+
+
+def test_accum_roundtrip_and_resume(tmp_path):
+    """FP64 accumulators and their sidecar round-trip exactly; the sidecar
+    says where a resumed render continues; sums without squares read back
+    with zero squares; a missing or foreign sidecar is an error."""
+    rng = np.random.default_rng(9)
+    s = rng.normal(size=(7, 5, 3))
+    q = s * s + rng.random((7, 5, 3))
+    p = tmp_path / "acc.f64"
+    linrt.write_accum(p, s, q, spp_done=4096, next_sample=4096, seed=(1 << 40) + 3, max_depth=16, rr_depth=5)
+    side = json.loads((tmp_path / "acc.f64.json").read_text())
+    assert side["dtype"] == "float64" and side["arrays"] == ["sum", "sumsq"] and side["seed"] == (1 << 40) + 3
+    assert p.stat().st_size == 2 * s.size * 8
+    s2, q2, info = linrt.read_accum(p)
+    assert np.array_equal(s, s2) and np.array_equal(q, q2)
+    assert (info["width"], info["height"], info["spp_done"], info["next_sample"]) == (5, 7, 4096, 4096)
+    assert (info["seed"], info["max_depth"], info["rr_depth"], info["has_sumsq"]) == ((1 << 40) + 3, 16, 5, 1)
+    linrt.write_accum(p, s, None, spp_done=8)
+    s3, q3, info = linrt.read_accum(p)
+    assert q3 is None and info["has_sumsq"] == 0 and np.array_equal(s, s3)
+    with pytest.raises(linrt.LinrtError):
+        linrt.read_accum(tmp_path / "missing.f64")
+    (tmp_path / "bad.f64.json").write_text('{"width": 5, "height": 7, "dtype": "float32"}')
+    with pytest.raises(linrt.LinrtError):
+        linrt.read_accum(tmp_path / "bad.f64")
