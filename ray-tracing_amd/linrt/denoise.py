@@ -9,7 +9,8 @@ are used only as allocations); every network step runs the library's BF16
 matrix-core kernels through the C ABI.
 
   python -m linrt.denoise train [--scenes N] [--steps K] [--out path]
-  python -m linrt.denoise reference [--spp S] [--out path]
+  python -m linrt.denoise oracle --spp 65536 [--out path]   (the FP64 65,536-spp oracle of the whole frame)
+  python -m linrt.denoise evaluate                            (time to relMSE <= 1e-2 over the whole frame)
 """
 from __future__ import annotations
 
@@ -25,12 +26,7 @@ import linrt
 
 DATA = Path(__file__).resolve().parent.parent / "data"
 WEIGHTS = DATA / "denoiser_weights.npz"
-REFERENCE = DATA / "reference_1080p_band.npz"
 MAX_DEPTH, RR_DEPTH = 16, 5
-# the evaluation band of the 1920x1080 frame (rows bottom-up) and the margin
-# the network sees around it
-BAND = (476, 604)
-MARGIN = 64
 EPS = 1e-2  # relMSE = mean((x - ref)^2 / (ref^2 + EPS))
 
 
@@ -152,44 +148,79 @@ def c1_scene():
     return gen, linrt.Scene(gen.desc_ptr)
 
 
-def make_reference(spp=16384, out=REFERENCE, chunk=1024, log=print):
-    """The band of the 1080p frame at spp samples (float64 sum on the device),
-    stored as float32 [rows, 1920, 3]."""
+ORACLE = DATA / "oracle_1080p_65536spp.npz"
+ORACLE_SEED = (1 << 32) + 7  # a Philox key no timed render uses: the oracle's errors are independent of theirs
+
+
+def make_oracle(spp=65536, out=ORACLE, chunk=512, accum=None, log=print):
+    """The error judge of the 1080p frame: spp samples per pixel of the whole
+    frame added into float64 sums and sums of squares on the device (the
+    reference accumulates in float32, src/main.cpp:120-136).  Saves the
+    float32 mean and the oracle's own relMSE, mean(var(mean) / (mean^2 +
+    eps)) from the sums of squares; `accum`: also the float64 accumulators
+    (lrt_accum_write, resumable)."""
     torch = _torch()
     gen, sc = c1_scene()
     W, H = 1920, 1080
     acc = torch.zeros((H, W, 3), dtype=torch.float64, device="cuda:0")
+    acc2 = torch.zeros_like(acc)
     t0 = time.perf_counter()
+    last = t0
     for off in range(0, spp, chunk):
-        prm = linrt.RenderParams(W, H, chunk, (1 << 24) + off, MAX_DEPTH, RR_DEPTH, 1, BAND[0], BAND[1], 0)
+        n = min(chunk, spp - off)
+        prm = linrt.RenderParams(W, H, n, off, MAX_DEPTH, RR_DEPTH, ORACLE_SEED, 0, 0, 0)
         st = linrt.RenderStats()
-        linrt._check(linrt.LIB.lrt_render(sc._h, linrt.C.byref(prm), linrt.C.c_void_p(acc.data_ptr()), None, 1,
-                                          linrt.C.byref(st)))
+        linrt._check(linrt.LIB.lrt_render(sc._h, linrt.C.byref(prm), linrt.C.c_void_p(acc.data_ptr()),
+                                          linrt.C.c_void_p(acc2.data_ptr()), 1, linrt.C.byref(st)))
         sc.sync()
-        log(f"  reference: {off + chunk}/{spp} spp, {time.perf_counter() - t0:.1f}s")
-    band = (acc[BAND[0]:BAND[1]] / spp).float().cpu().numpy()
-    np.savez_compressed(out, band=band.astype(np.float16), rows=np.array(BAND), spp=spp)
-    return band
+        if time.perf_counter() - last > 30 or off + n == spp:
+            last = time.perf_counter()
+            log(f"  oracle: {off + n}/{spp} spp, {last - t0:.1f}s")
+    s, q = acc.cpu().numpy(), acc2.cpu().numpy()
+    if accum is not None:
+        linrt.write_accum(accum, s, q, spp_done=spp, next_sample=spp, seed=ORACLE_SEED, max_depth=MAX_DEPTH,
+                          rr_depth=RR_DEPTH)
+    mean = s / spp
+    var_mean = np.maximum(q / spp - mean * mean, 0.0) / (spp - 1)
+    fin = np.isfinite(mean)
+    self_relmse = float(np.mean((var_mean / (mean * mean + EPS))[fin]))
+    np.savez_compressed(out, mean=mean.astype(np.float32), spp=spp, seed=ORACLE_SEED, max_depth=MAX_DEPTH,
+                        rr_depth=RR_DEPTH, eps=EPS, self_relmse=self_relmse, nan_pixels=int((~fin).any(-1).sum()))
+    log(f"oracle: {spp} spp in {time.perf_counter() - t0:.1f}s, own relMSE {self_relmse:.3e}, "
+        f"non-finite pixels {int((~fin).any(-1).sum())}")
+    sc.close()
+    gen.close()
+    return mean
 
 
-def time_to_quality(den=None, ref=None, target=0.01, log=print, spp_list=(1, 2, 4, 8, 16), weights=WEIGHTS):
-    """Time for the plain path tracer and for denoised frames to reach
-    relMSE <= target on the band of the 1080p frame.  Returns a dict."""
+def time_to_quality(den=None, target=0.01, log=print, spp_list=(1, 2, 4), weights=WEIGHTS, oracle=ORACLE):
+    """Wall time for the plain path tracer and for denoised frames to reach
+    relMSE <= target over the whole 1080p frame, judged by the FP64-accumulated
+    65,536-spp oracle.  Each timed render includes its device accumulation
+    (and for the denoised frames the features and the network)."""
     torch = _torch()
-    if ref is None:
-        ref = np.load(REFERENCE)["band"].astype(np.float32)
+    d = np.load(oracle)
+    ref = torch.from_numpy(d["mean"]).to("cuda:0")
     if den is None:
         den = linrt.Denoiser()
         load(den, weights)
     gen, sc = c1_scene()
     W, H = 1920, 1080
     out = torch.zeros((H, W, 3), dtype=torch.float32, device="cuda:0")
-    # warm-up of every kernel
+    fin = torch.isfinite(ref)
+
+    def err(x):
+        x = torch.nan_to_num(x, nan=0.0, posinf=0.0, neginf=0.0)
+        e = (x - ref) ** 2 / (ref * ref + EPS)
+        return float(e[fin].mean())
+
     r = radiance(sc, W, H, 1, 1, 0)
     a = features(sc, W, H, 1, 1, 0)
     den.forward_device(r.data_ptr(), a.data_ptr(), 1, H, W, out.data_ptr())
     torch.cuda.synchronize()
-    res = {"band_rows": list(BAND), "eps": EPS, "target": target, "denoised": [], "plain": []}
+    res = {"oracle": {"file": oracle.name, "spp": int(d["spp"]), "accumulation": "float64",
+                      "self_relmse": float(d["self_relmse"]), "pixels": W * H},
+           "eps": EPS, "target": target, "denoised": [], "plain": []}
     best = None
     for spp in spp_list:
         torch.cuda.synchronize()
@@ -199,8 +230,7 @@ def time_to_quality(den=None, ref=None, target=0.01, log=print, spp_list=(1, 2, 
         den.forward_device(r.data_ptr(), a.data_ptr(), 1, H, W, out.data_ptr())
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        e = relmse(out[BAND[0]:BAND[1]].cpu().numpy(), ref)
-        e0 = relmse(r[BAND[0]:BAND[1]].cpu().numpy(), ref)
+        e, e0 = err(out), err(r)
         res["denoised"].append({"spp": spp, "seconds": dt, "relmse": e, "relmse_input": e0})
         log(f"  denoised {spp} spp: {dt * 1e3:.1f} ms, relMSE {e:.5f} (input {e0:.4f})")
         if e <= target and best is None:
@@ -208,11 +238,12 @@ def time_to_quality(den=None, ref=None, target=0.01, log=print, spp_list=(1, 2, 
     res["denoised_best"] = None if best is None else {"spp": best[0], "seconds": best[1], "relmse": best[2]}
     # plain: relMSE ~ c / spp; measure at 64 spp, render the predicted spp for
     # the target and step up by 10% until it is reached
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     r = radiance(sc, W, H, 64, 1, 0)
     torch.cuda.synchronize()
     t64 = time.perf_counter() - t0
-    e64 = relmse(r[BAND[0]:BAND[1]].cpu().numpy(), ref)
+    e64 = err(r)
     res["plain"].append({"spp": 64, "seconds": t64, "relmse": e64})
     need = max(2, int(math.ceil(64 * e64 / target / 2)) * 2)
     for _ in range(6):
@@ -221,7 +252,7 @@ def time_to_quality(den=None, ref=None, target=0.01, log=print, spp_list=(1, 2, 
         r = radiance(sc, W, H, need, 1, 0)
         torch.cuda.synchronize()
         tn = time.perf_counter() - t0
-        en = relmse(r[BAND[0]:BAND[1]].cpu().numpy(), ref)
+        en = err(r)
         res["plain"].append({"spp": need, "seconds": tn, "relmse": en})
         log(f"  plain {need} spp: {tn:.2f} s, relMSE {en:.5f}")
         if en <= target:
@@ -229,7 +260,7 @@ def time_to_quality(den=None, ref=None, target=0.01, log=print, spp_list=(1, 2, 
         need = int(math.ceil(need * 1.1 / 2)) * 2
     res["plain_best"] = {"spp": need, "seconds": tn, "relmse": en}
     res["speedup"] = tn / best[1] if (best is not None and en <= target) else None
-    four = [d for d in res["denoised"] if d["spp"] == 4]
+    four = [x for x in res["denoised"] if x["spp"] == 4]
     res["speedup_4spp"] = tn / four[0]["seconds"] if (four and four[0]["relmse"] <= target and en <= target) else None
     sc.close()
     gen.close()
@@ -238,7 +269,7 @@ def time_to_quality(den=None, ref=None, target=0.01, log=print, spp_list=(1, 2, 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["train", "reference", "evaluate"])
+    ap.add_argument("cmd", choices=["train", "oracle", "evaluate"])
     ap.add_argument("--scenes", type=int, default=160)
     ap.add_argument("--steps", type=int, default=12000)
     ap.add_argument("--spp-target", type=int, default=512)
@@ -250,8 +281,8 @@ def main(argv=None):
     args = ap.parse_args(argv)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     _torch()
-    if args.cmd == "reference":
-        make_reference(args.spp, Path(args.out) if args.out else REFERENCE, log=log)
+    if args.cmd == "oracle":
+        make_oracle(args.spp, Path(args.out) if args.out else ORACLE, log=log)
         return
     if args.cmd == "evaluate":
         import json
