@@ -323,7 +323,7 @@ int lrt_render_capture_rays(lrt_scene* scene, const lrt_render_params* params,
  * particles 0, 1, 2, ... in order until `requested` photons are returned
  * (photon_gather.h:15-101 with one particle per work unit), keeps the first
  * `requested`, and builds the reference's median kd tree over them
- * (photon_map.h:83-114).  A gather that fails (photon_gather.h:99-101) leaves
+ * (photon_map.h:83-114): by default on the device (see `tree`).  A gather that fails (photon_gather.h:99-101) leaves
  * the map empty, like photon_integrator::preprocess (:25-36). */
 typedef struct lrt_photon_params {
     int32_t kind;          /* 0 global (surface) map, 1 caustic map */
@@ -333,7 +333,12 @@ typedef struct lrt_photon_params {
     uint64_t seed;
     float radius;          /* query radius (photon_integrator.h:17-18: 0.05 / 0.0125 x bounding radius) */
     uint32_t k;            /* photons per lookup, <= 128 (photon_integrator.h:64: 120) */
+    uint32_t tree;         /* LRT_PHOTON_TREE_DEVICE: the median tree built on the device from the
+                              photons in HBM (split = the median photon's coordinate, exact search);
+                              LRT_PHOTON_TREE_REFERENCE: the reference's std::nth_element tree and
+                              split read, built on the host (positions copied out and back) */
 } lrt_photon_params;
+enum { LRT_PHOTON_TREE_DEVICE = 0, LRT_PHOTON_TREE_REFERENCE = 1 };
 
 typedef struct lrt_photon_stats {
     uint64_t photons, shot, particles_traced, returned;
@@ -349,6 +354,13 @@ int lrt_photon_export(lrt_scene* scene, int32_t kind, float* photons12, uint32_t
 /* builds map `kind` from given photons (scale = 1 / shot; n = 0 empties it) */
 int lrt_photon_map_load(lrt_scene* scene, int32_t kind, const float* photons12, const uint32_t* depth_particle,
                         uint64_t n, float scale, float radius, uint32_t k);
+/* the kd tree of map `kind` (no counterpart in the reference, whose tree is
+ * private): nodes, 4 words each in pre-order (interior {split bits, axis, left,
+ * right}, leaf {0, 3, first, count}), and order[i] = the particle-order index
+ * (lrt_photon_export's) of the photon at tree position i.  Null buffers: only
+ * *n_nodes and *n_photons are returned. */
+int lrt_photon_map_tree(lrt_scene* scene, int32_t kind, uint32_t* nodes4, uint64_t node_capacity,
+                        uint32_t* order, uint64_t order_capacity, uint64_t* n_nodes, uint64_t* n_photons);
 /* photon_map::estimate_irradiance of map `kind` at n (point, normal) queries (6 floats) */
 int lrt_photon_estimate(lrt_scene* scene, int32_t kind, const float* queries6, uint32_t n, int max_depth, float* out3);
 /* photon_integrator::radiance (photon_integrator.h:39-123) for every (pixel,

@@ -4,6 +4,8 @@
 // (photon_gather.h:15-101, single-threaded order), the median kd photon map
 // and its kNN irradiance estimate (photon_map.h:26-165), and the eye pass
 // photon_integrator::li (photon_integrator.h:67-296).
+#include <algorithm>
+#include <cstring>
 #include <queue>
 #include <thread>
 
@@ -209,6 +211,80 @@ struct pmap {
         scale = s;
         nodes.clear();
         if (!ph.empty()) build_range(0, (int)ph.size());
+    }
+
+    // The library's device tree (no counterpart in the reference): the shape
+    // and axis rule of build_range, but a node's left child takes the
+    // floor(c / 2) photons first in (coordinate, input index) order on the
+    // axis, its split is the coordinate of the right child's first photon in
+    // that order (the median), and a leaf lists its photons in (x, input index)
+    // order.  Coordinates compare by their order-preserving key (-0 before +0).
+    static uint32_t fkey(float f) {
+        uint32_t u;
+        std::memcpy(&u, &f, 4);
+        return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    }
+    struct by_axis {
+        const std::vector<photon>* g;
+        int a;
+        bool operator()(uint32_t i, uint32_t j) const {
+            const uint32_t ki = fkey((*g)[i].pos[a]), kj = fkey((*g)[j].pos[a]);
+            return ki != kj ? ki < kj : i < j;
+        }
+    };
+    int median_range(const std::vector<photon>& g, const std::vector<uint32_t>& xs, std::vector<uint32_t>& order,
+                     std::vector<uint8_t>& mark) {
+        const int index = (int)nodes.size();
+        nodes.push_back(node{});
+        const int c = (int)xs.size();
+        if (c <= 8) {
+            nodes[(size_t)index].leaf = true;
+            nodes[(size_t)index].start = (int)order.size();
+            nodes[(size_t)index].count = c;
+            order.insert(order.end(), xs.begin(), xs.end());
+            return index;
+        }
+        box b;
+        for (uint32_t i : xs) b.grow(g[i].pos);
+        const v3 e = b.hi - b.lo;
+        int axis = 0;
+        if (e.y > e[axis]) axis = 1;
+        if (e.z > e[axis]) axis = 2;
+        std::vector<uint32_t> sorted = xs;
+        std::sort(sorted.begin(), sorted.end(), by_axis{&g, axis});
+        const int k = c / 2;
+        const float split = g[sorted[(size_t)k]].pos[axis];
+        for (int i = 0; i < k; ++i) mark[sorted[(size_t)i]] = 1;
+        std::vector<uint32_t> lx, rx;
+        lx.reserve((size_t)k);
+        rx.reserve((size_t)(c - k));
+        for (uint32_t i : xs) (mark[i] ? lx : rx).push_back(i);
+        for (int i = 0; i < k; ++i) mark[sorted[(size_t)i]] = 0;
+        std::vector<uint32_t>().swap(sorted);
+        const int l = median_range(g, lx, order, mark);
+        const int r = median_range(g, rx, order, mark);
+        nodes[(size_t)index].axis = axis;
+        nodes[(size_t)index].split = split;
+        nodes[(size_t)index].left = l;
+        nodes[(size_t)index].right = r;
+        return index;
+    }
+    // tree order: order[i] = input index of the photon at tree position i
+    std::vector<uint32_t> tree_order;
+    void build_median(std::vector<photon> g, float s) {
+        scale = s;
+        nodes.clear();
+        ph.clear();
+        tree_order.clear();
+        if (g.empty()) return;
+        std::vector<uint32_t> xs(g.size());
+        for (size_t i = 0; i < g.size(); ++i) xs[i] = (uint32_t)i;
+        std::sort(xs.begin(), xs.end(), by_axis{&g, 0});
+        std::vector<uint8_t> mark(g.size(), 0);
+        tree_order.reserve(g.size());
+        median_range(g, xs, tree_order, mark);
+        ph.resize(g.size());
+        for (size_t i = 0; i < g.size(); ++i) ph[i] = g[tree_order[i]];
     }
     using nb = std::pair<float, uint32_t>;
     void search(int index, const v3& p, float r2, size_t k, std::priority_queue<nb>& heap) const {
@@ -550,10 +626,12 @@ extern "C" {
 // particle) each; n = 0 leaves a map empty).  mode 0: Philox streams as on
 // the device; mode 1: the reference sampler.  per_sample (optional): float
 // radiance per (pixel, sample).
+// tree: 0 the reference's photon_map tree, 1 the library's device tree
+// (pmap::build_median).
 int orc_render_photon(void* h, int W, int H, int spp, int max_depth, uint64_t seed, int mode, const float* g_ph,
                       const uint32_t* g_dp, int64_t g_n, float g_scale, float g_radius, const float* c_ph,
                       const uint32_t* c_dp, int64_t c_n, float c_scale, float c_radius, int k, int threads,
-                      int row_begin, int row_end, double* sum, double* sumsq, float* per_sample) {
+                      int row_begin, int row_end, double* sum, double* sumsq, float* per_sample, int tree) {
     const scene& S = *(scene*)h;
     pmap maps[2];
     const float* ph[2] = {g_ph, c_ph};
@@ -567,7 +645,8 @@ int orc_render_photon(void* h, int W, int H, int spp, int max_depth, uint64_t se
             v[(size_t)i] = photon{load3(p), load3(p + 3), load3(p + 6), load3(p + 9), (uint16_t)dp[m][2 * i],
                                   dp[m][2 * i + 1]};
         }
-        maps[m].build(std::move(v), sc[m]);
+        if (tree) maps[m].build_median(std::move(v), sc[m]);
+        else maps[m].build(std::move(v), sc[m]);
     }
     eye_job J{&S, {&maps[0], &maps[1]}, {g_radius, c_radius}, max_depth, k};
     const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
@@ -674,7 +753,7 @@ int64_t orc_trace_particles(void* h, int kind, uint32_t first, uint32_t count, i
 // photon_map::build + estimate_irradiance for n queries (point, normal)
 int orc_photon_estimate(const float* photons12, const uint32_t* depth_particle, int64_t n_photons, float scale,
                         const float* queries6, uint32_t n_queries, float radius, int max_depth, int k, float* out3,
-                        int threads) {
+                        int threads, int tree) {
     pmap m;
     std::vector<photon> ph((size_t)n_photons);
     for (int64_t i = 0; i < n_photons; ++i) {
@@ -682,7 +761,8 @@ int orc_photon_estimate(const float* photons12, const uint32_t* depth_particle, 
         ph[(size_t)i] = photon{load3(p), load3(p + 3), load3(p + 6), load3(p + 9), (uint16_t)depth_particle[2 * i],
                                depth_particle[2 * i + 1]};
     }
-    m.build(std::move(ph), scale);
+    if (tree) m.build_median(std::move(ph), scale);
+    else m.build(std::move(ph), scale);
     const int nt = std::max(1, threads);
     std::vector<std::thread> pool;
     auto body = [&](int t) {
@@ -695,6 +775,58 @@ int orc_photon_estimate(const float* photons12, const uint32_t* depth_particle, 
     body(0);
     for (auto& th : pool) th.join();
     return 0;
+}
+
+// The library's device photon tree over n positions (3 floats each, input
+// order): nodes (4 words each, pre-order: interior {split bits, axis, left,
+// right}, leaf {0, 3, first, count}) and order (input index per tree
+// position).  Returns the node count; with null buffers only counts.
+int64_t orc_photon_tree(const float* xyz, int64_t n, uint32_t* nodes4, uint32_t* order) {
+    std::vector<photon> g((size_t)n);
+    for (int64_t i = 0; i < n; ++i) g[(size_t)i].pos = load3(xyz + 3 * i);
+    pmap m;
+    m.build_median(std::move(g), 1.0f);
+    if (nodes4) {
+        for (size_t i = 0; i < m.nodes.size(); ++i) {
+            const pmap::node& d = m.nodes[i];
+            uint32_t* o = nodes4 + 4 * i;
+            if (d.leaf) {
+                o[0] = 0u; o[1] = 3u; o[2] = (uint32_t)d.start; o[3] = (uint32_t)d.count;
+            } else {
+                std::memcpy(&o[0], &d.split, 4);
+                o[1] = (uint32_t)d.axis; o[2] = (uint32_t)d.left; o[3] = (uint32_t)d.right;
+            }
+        }
+    }
+    if (order) std::memcpy(order, m.tree_order.data(), m.tree_order.size() * 4);
+    return (int64_t)m.nodes.size();
+}
+
+// The reference's photon_map tree (pmap::build) over n positions, same
+// layout (the split read after both subtrees, photon_map.h:107-110).
+int64_t orc_photon_tree_reference(const float* xyz, int64_t n, uint32_t* nodes4, uint32_t* order) {
+    std::vector<photon> g((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        g[(size_t)i].pos = load3(xyz + 3 * i);
+        g[(size_t)i].particle = (uint32_t)i;
+    }
+    pmap m;
+    m.build(std::move(g), 1.0f);
+    if (nodes4) {
+        for (size_t i = 0; i < m.nodes.size(); ++i) {
+            const pmap::node& d = m.nodes[i];
+            uint32_t* o = nodes4 + 4 * i;
+            if (d.leaf) {
+                o[0] = 0u; o[1] = 3u; o[2] = (uint32_t)d.start; o[3] = (uint32_t)d.count;
+            } else {
+                std::memcpy(&o[0], &d.split, 4);
+                o[1] = (uint32_t)d.axis; o[2] = (uint32_t)d.left; o[3] = (uint32_t)d.right;
+            }
+        }
+    }
+    if (order)
+        for (int64_t i = 0; i < n; ++i) order[i] = m.ph[(size_t)i].particle;
+    return (int64_t)m.nodes.size();
 }
 
 }  // extern "C"

@@ -4,6 +4,7 @@
 #include <chrono>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -278,9 +279,9 @@ using namespace lrt;
 
 struct PhotonState {
     PhotonBuf buf;
-    DevBuf raw, per, sort, hist, perm, xyz, nodes, order, pos, pay;
+    DevBuf raw, per, sort, hist, perm, xyz, nodes, order, pos, pay, tree_work;
     PhotonMapDev map;
-    uint64_t n = 0, shot = 0;
+    uint64_t n = 0, shot = 0, n_nodes = 0;
     bool valid = false;
 };
 
@@ -1542,10 +1543,101 @@ static void radix_sort(lrt_scene& sc, DevBuf& hbuf, uint32_t* k0, uint32_t* v0, 
     out_v = v0;
 }
 
-// raw photons (S.buf, count n_raw) -> kept in particle order -> kd tree
-// (host build from the positions) -> photons packed in tree order
+// nodes of a median subtree over c photons (leaves of at most 8)
+static uint64_t pk_subtree_nodes(uint64_t c, std::map<uint64_t, uint64_t>& memo) {
+    if (c <= 8) return 1;
+    auto it = memo.find(c);
+    if (it != memo.end()) return it->second;
+    const uint64_t v = 1 + pk_subtree_nodes(c / 2, memo) + pk_subtree_nodes(c - c / 2, memo);
+    memo[c] = v;
+    return v;
+}
+
+// The device median tree over the n photon positions xyz (particle order):
+// S.nodes (pre-order), S.order (tree position -> particle-order index).
+// Level l holds 2^l segments of floor(n / 2^l) or that + 1 photons; levels
+// whose smallest segment has 9 or more photons split every segment, the first
+// level below that is finished per segment (leaves, or a 9 into 4 + 5).
+// Returns the depth (nodes on the longest root-to-leaf path).
+static int build_photon_tree_device(lrt_scene& sc, PhotonState& S, uint32_t n) {
+    std::map<uint64_t, uint64_t> memo;
+    const uint64_t n_nodes = pk_subtree_nodes(n, memo);
+    int L = 0;  // levels that split every segment
+    while ((n >> L) >= 9u) ++L;
+    const uint32_t m_max = 1u << L;
+    const size_t tiles = (3 * (size_t)n + kPkTile - 1) / kPkTile;
+    const size_t words = 3 * (size_t)n * 3 + 2 * (size_t)n + 2 * 4 * (size_t)m_max + tiles + (size_t)n / 4 + 64;
+    S.tree_work.alloc(words * 4 + 16 * 256);
+    char* c = S.tree_work.as<char>();
+    uint32_t* lists = alloc_carve<uint32_t>(c, 3 * (size_t)n);
+    uint32_t* lists2 = alloc_carve<uint32_t>(c, 3 * (size_t)n);
+    uint32_t* prefix = alloc_carve<uint32_t>(c, 3 * (size_t)n);
+    uint32_t* seg = alloc_carve<uint32_t>(c, n);
+    uint32_t* seg2 = alloc_carve<uint32_t>(c, n);
+    PkLevel lv[2];
+    for (PkLevel& l : lv) {
+        l.start = alloc_carve<uint32_t>(c, m_max);
+        l.count = alloc_carve<uint32_t>(c, m_max);
+        l.pre = alloc_carve<uint32_t>(c, m_max);
+        l.axis = alloc_carve<uint32_t>(c, m_max);
+    }
+    uint32_t* tile_sum = alloc_carve<uint32_t>(c, tiles);
+    uint8_t* side = alloc_carve<uint8_t>(c, n);
+    S.nodes.alloc(n_nodes * 16);
+    S.order.alloc((size_t)n * 4);
+    const float* xyz = S.xyz.as<float>();
+    const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>((n + 255) / 256, (uint32_t)sc.cus * 16));
+    // lists sorted by (coordinate key, index) per axis: stable radix sort of the keys
+    uint32_t* kbuf = prefix;  // scratch for the sorts (3n words: keys, and the sort's second buffers)
+    uint32_t* kb2 = lists2;
+    for (int a = 0; a < 3; ++a) {
+        uint32_t* keys = kbuf;
+        uint32_t* ids = kbuf + n;
+        hipLaunchKernelGGL(k_pk_keys, dim3(g), dim3(256), 0, sc.stream, xyz, n, a, keys, ids);
+        uint32_t *ok, *ov;
+        radix_sort(sc, S.hist, keys, ids, kb2, kb2 + n, n, 32, ok, ov);
+        HIPCHECK(hipMemcpyAsync(lists + (size_t)a * n, ov, (size_t)n * 4, hipMemcpyDeviceToDevice, sc.stream));
+    }
+    // level 0: the root segment
+    hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(256), 0, sc.stream, seg, n, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, sc.stream, lv[0].start, 1u, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, sc.stream, lv[0].count, 1u, n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, sc.stream, lv[0].pre, 1u, 0u);
+    uint4* nodes = S.nodes.as<uint4>();
+    const uint32_t g3 = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)((3 * (size_t)n + 255) / 256), (uint32_t)sc.cus * 16));
+    for (int l = 0; l < L; ++l) {
+        const uint32_t m = 1u << l;
+        const uint32_t a_next = n >> (l + 1);
+        const uint32_t s_lo = (uint32_t)pk_subtree_nodes(a_next, memo), s_hi = (uint32_t)pk_subtree_nodes(a_next + 1, memo);
+        PkLevel& cur = lv[l & 1];
+        PkLevel& nxt = lv[(l + 1) & 1];
+        hipLaunchKernelGGL(k_pk_choose, dim3((m + 255) / 256), dim3(256), 0, sc.stream, xyz, lists, n, cur, m, nxt,
+                           a_next, s_lo, s_hi, nodes);
+        hipLaunchKernelGGL(k_pk_side, dim3(g), dim3(256), 0, sc.stream, lists, n, seg, cur, side);
+        hipLaunchKernelGGL(k_pk_scan_tiles, dim3((uint32_t)tiles), dim3(kPkScanBlock), 0, sc.stream, lists, side,
+                           3 * (size_t)n, tile_sum);
+        hipLaunchKernelGGL(k_pk_scan_sums, dim3(1), dim3(1024), 0, sc.stream, tile_sum, (uint32_t)tiles);
+        hipLaunchKernelGGL(k_pk_scan_apply, dim3((uint32_t)tiles), dim3(kPkScanBlock), 0, sc.stream, lists, side,
+                           3 * (size_t)n, tile_sum, prefix);
+        hipLaunchKernelGGL(k_pk_scatter, dim3(g3), dim3(256), 0, sc.stream, lists, n, seg, cur, side, prefix, lists2,
+                           seg2);
+        std::swap(lists, lists2);
+        std::swap(seg, seg2);
+    }
+    const uint32_t m = 1u << L;
+    hipLaunchKernelGGL(k_pk_finish, dim3((m + 255) / 256), dim3(256), 0, sc.stream, xyz, lists, n, lv[L & 1], m, nodes,
+                       S.order.as<uint32_t>());
+    HIPCHECK(hipGetLastError());
+    S.n_nodes = n_nodes;
+    const uint64_t deepest = ((uint64_t)n + m - 1) / m;  // the right-most segment of the last level
+    return L + 1 + (deepest > 8 ? 1 : 0);
+}
+
+// raw photons (S.buf, count n_raw) -> kept in particle order -> kd tree (on
+// the device, or the reference's tree built on the host) -> photons packed in
+// tree order
 static void build_photon_map(lrt_scene& sc, PhotonState& S, uint32_t n_raw, uint32_t n_keep, float scale, float radius,
-                             uint32_t k) {
+                             uint32_t k, uint32_t tree) {
     const uint32_t g = std::max<uint32_t>(1, std::min<uint32_t>((n_raw + 255) / 256, (uint32_t)sc.cus * 16));
     S.sort.alloc((size_t)std::max<uint32_t>(n_raw, 1) * 4 * 4 + 4 * 256);
     char* c = S.sort.as<char>();
@@ -1569,22 +1661,30 @@ static void build_photon_map(lrt_scene& sc, PhotonState& S, uint32_t n_raw, uint
         hipLaunchKernelGGL(k_photon_positions, dim3(g), dim3(256), 0, sc.stream, S.buf, S.perm.as<uint32_t>(), n_keep,
                            S.xyz.as<float>());
     HIPCHECK(hipGetLastError());
-    std::vector<float> xyz((size_t)n_keep * 3);
-    if (n_keep) HIPCHECK(hipMemcpyAsync(xyz.data(), S.xyz.p, (size_t)n_keep * 12, hipMemcpyDeviceToHost, sc.stream));
-    HIPCHECK(hipStreamSynchronize(sc.stream));
-    PhotonKd kd;
-    photon_kd_build(xyz.data(), n_keep, kd);
-    require(kd.depth < kPkStack, "photon kd tree deeper than the search stack");
-    for (size_t i = 0; i < kd.nodes.size(); i += 4)  // the device search loads a leaf's photons 8 at a time
-        require(kd.nodes[i + 1] != 3u || kd.nodes[i + 3] <= 8u, "photon kd leaf with more than 8 photons");
+    if (tree == LRT_PHOTON_TREE_DEVICE) {
+        if (n_keep) require(build_photon_tree_device(sc, S, n_keep) < kPkStack, "photon kd tree deeper than the search stack");
+    } else {
+        std::vector<float> xyz((size_t)n_keep * 3);
+        if (n_keep) HIPCHECK(hipMemcpyAsync(xyz.data(), S.xyz.p, (size_t)n_keep * 12, hipMemcpyDeviceToHost, sc.stream));
+        HIPCHECK(hipStreamSynchronize(sc.stream));
+        PhotonKd kd;
+        photon_kd_build(xyz.data(), n_keep, kd);
+        require(kd.depth < kPkStack, "photon kd tree deeper than the search stack");
+        for (size_t i = 0; i < kd.nodes.size(); i += 4)  // the device search loads a leaf's photons 8 at a time
+            require(kd.nodes[i + 1] != 3u || kd.nodes[i + 3] <= 8u, "photon kd leaf with more than 8 photons");
+        S.nodes.alloc(std::max<size_t>(kd.nodes.size(), 4) * 4);
+        S.order.alloc((size_t)std::max<uint32_t>(n_keep, 1) * 4);
+        if (n_keep) {
+            HIPCHECK(hipMemcpyAsync(S.nodes.p, kd.nodes.data(), kd.nodes.size() * 4, hipMemcpyHostToDevice, sc.stream));
+            HIPCHECK(hipMemcpyAsync(S.order.p, kd.order.data(), (size_t)n_keep * 4, hipMemcpyHostToDevice, sc.stream));
+        }
+        S.n_nodes = kd.nodes.size() / 4;
+        HIPCHECK(hipStreamSynchronize(sc.stream));  // the host vectors are released on return
+    }
     // 3. photons in tree order
-    S.nodes.alloc(std::max<size_t>(kd.nodes.size(), 4) * 4);
-    S.order.alloc((size_t)std::max<uint32_t>(n_keep, 1) * 4);
     S.pos.alloc((size_t)std::max<uint32_t>(n_keep, 1) * 16);
     S.pay.alloc((size_t)std::max<uint32_t>(n_keep, 1) * 36);
     if (n_keep) {
-        HIPCHECK(hipMemcpyAsync(S.nodes.p, kd.nodes.data(), kd.nodes.size() * 4, hipMemcpyHostToDevice, sc.stream));
-        HIPCHECK(hipMemcpyAsync(S.order.p, kd.order.data(), (size_t)n_keep * 4, hipMemcpyHostToDevice, sc.stream));
         hipLaunchKernelGGL(k_photon_pack, dim3(g), dim3(256), 0, sc.stream, S.buf, S.perm.as<uint32_t>(),
                            S.order.as<uint32_t>(), n_keep, S.pos.as<float4>(), S.pay.as<float>());
     }
@@ -1623,6 +1723,7 @@ int lrt_photon_build(lrt_scene* sc, const lrt_photon_params* prm, lrt_photon_sta
         require(prm->k >= 1 && prm->k <= (uint32_t)kMaxK, "k must be in [1, 128]");
         require(prm->radius > 0.0f && prm->max_depth >= 1, "bad photon parameters");
         require(prm->max_depth < 256, "photon depth is stored in 8 bits: max_depth must be < 256");
+        require(prm->tree == LRT_PHOTON_TREE_DEVICE || prm->tree == LRT_PHOTON_TREE_REFERENCE, "unknown photon tree");
         PhotonState& S = photon_state(sc, prm->kind);
         HIPCHECK(hipSetDevice(sc->device));
         const auto t0 = std::chrono::steady_clock::now();
@@ -1692,7 +1793,8 @@ int lrt_photon_build(lrt_scene* sc, const lrt_photon_params* prm, lrt_photon_sta
         const bool full = !failed && returned >= prm->requested;
         const auto t1 = std::chrono::steady_clock::now();
         if (full) {
-            build_photon_map(*sc, S, (uint32_t)returned, prm->requested, 1.0f / (float)shot, prm->radius, prm->k);
+            build_photon_map(*sc, S, (uint32_t)returned, prm->requested, 1.0f / (float)shot, prm->radius, prm->k,
+                             prm->tree);
             S.shot = shot;
         }
         const auto t2 = std::chrono::steady_clock::now();
@@ -1735,8 +1837,25 @@ int lrt_photon_map_load(lrt_scene* sc, int32_t kind, const float* photons12, con
             keys[i] = ((uint32_t)i << 8) | depth_particle[2 * i];
         }
         HIPCHECK(hipMemcpy(S.buf.key, keys.data(), n * 4, hipMemcpyHostToDevice));
-        build_photon_map(*sc, S, (uint32_t)n, (uint32_t)n, scale, radius, k);
+        build_photon_map(*sc, S, (uint32_t)n, (uint32_t)n, scale, radius, k, LRT_PHOTON_TREE_REFERENCE);
         S.shot = (uint64_t)std::llround(1.0 / scale);
+    });
+}
+
+int lrt_photon_map_tree(lrt_scene* sc, int32_t kind, uint32_t* nodes4, uint64_t node_capacity, uint32_t* order,
+                        uint64_t order_capacity, uint64_t* n_nodes, uint64_t* n_photons) {
+    return guarded([&] {
+        require(sc && n_nodes && n_photons, "null pointer");
+        PhotonState& S = photon_state(sc, kind);
+        const bool have = S.valid && S.n > 0;
+        *n_nodes = have ? S.n_nodes : 0;
+        *n_photons = have ? S.n : 0;
+        if (!have || (!nodes4 && !order)) return;
+        require(nodes4 && order && node_capacity >= S.n_nodes && order_capacity >= S.n, "buffers too small");
+        HIPCHECK(hipSetDevice(sc->device));
+        HIPCHECK(hipMemcpyAsync(nodes4, S.nodes.p, S.n_nodes * 16, hipMemcpyDeviceToHost, sc->stream));
+        HIPCHECK(hipMemcpyAsync(order, S.order.p, S.n * 4, hipMemcpyDeviceToHost, sc->stream));
+        HIPCHECK(hipStreamSynchronize(sc->stream));
     });
 }
 

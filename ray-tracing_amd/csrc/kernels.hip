@@ -781,5 +781,8 @@ __global__ __launch_bounds__(kBlock) void k_accumulate(const PathState P, uint32
 // photon tracing, sorting and the kd photon map (uses surface() and wave_append)
 #include "device/photon.h"
 
+// the photon kd tree built on the device (uses the radix sort of photon.h)
+#include "device/photon_tree.h"
+
 // differentiable rendering (uses surface())
 #include "device/diff.h"

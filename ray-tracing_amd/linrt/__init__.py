@@ -107,7 +107,9 @@ class RenderStats(C.Structure):
 
 class PhotonParams(C.Structure):
     _fields_ = [("kind", C.c_int32), ("requested", C.c_uint32), ("max_depth", C.c_uint32), ("rr_depth", C.c_uint32),
-                ("seed", C.c_uint64), ("radius", C.c_float), ("k", C.c_uint32)]
+                ("seed", C.c_uint64), ("radius", C.c_float), ("k", C.c_uint32), ("tree", C.c_uint32)]
+
+PHOTON_TREE_DEVICE, PHOTON_TREE_REFERENCE = 0, 1
 
 
 class PhotonStats(C.Structure):
@@ -158,7 +160,7 @@ EXPORTED = [
     "lrt_diff_param_count", "lrt_diff_render", "lrt_diff_adjoint", "lrt_diff_render_held",
     "lrt_load_obj", "lrt_mesh_view", "lrt_mesh_free", "lrt_generate_obj_scene", "lrt_image_write",
     "lrt_image_to_bytes", "lrt_render_capture_rays", "lrt_trace_closest_noalloc", "lrt_sky_cook",
-    "lrt_sky_elevation", "lrt_accum_write", "lrt_accum_read",
+    "lrt_sky_elevation", "lrt_accum_write", "lrt_accum_read", "lrt_photon_map_tree",
 ]
 
 
@@ -612,10 +614,12 @@ class Scene:
         _check(LIB.lrt_render_aov(self._h, C.byref(prm), C.c_void_p(aov_ptr), 1))
 
     # ---- photon mapping (photon_integrator.h:15-123) ----
-    def photon_build(self, requested, max_depth, rr_depth, seed=1, radius=0.05, k=120, kind=PHOTON_SURFACE):
+    def photon_build(self, requested, max_depth, rr_depth, seed=1, radius=0.05, k=120, kind=PHOTON_SURFACE,
+                     tree=PHOTON_TREE_DEVICE):
         """Traces particles until `requested` photons are stored and builds map
-        `kind` (photon_gather.h:15-101, photon_map.h:26-114)."""
-        prm = PhotonParams(kind, requested, max_depth, rr_depth, seed, radius, k)
+        `kind` (photon_gather.h:15-101, photon_map.h:26-114); tree: the device
+        median tree (default) or the reference's nth_element tree (host)."""
+        prm = PhotonParams(kind, requested, max_depth, rr_depth, seed, radius, k, tree)
         st = PhotonStats()
         _check(LIB.lrt_photon_build(self._h, C.byref(prm), C.byref(st)))
         return st
@@ -630,6 +634,19 @@ class Scene:
         _check(LIB.lrt_photon_export(self._h, C.c_int32(kind), _p(ph), _p(dp, C.c_uint32), C.c_uint64(n.value),
                                      C.byref(n), C.byref(shot)))
         return ph, dp, shot.value
+
+    def photon_map_tree(self, kind=PHOTON_SURFACE):
+        """Map `kind`'s kd tree (lrt_photon_map_tree): nodes [m, 4] uint32
+        (interior {split bits, axis, left, right}, leaf {0, 3, first, count},
+        pre-order) and order [n] (particle-order index of each tree position)."""
+        m, n = C.c_uint64(), C.c_uint64()
+        _check(LIB.lrt_photon_map_tree(self._h, C.c_int32(kind), None, C.c_uint64(0), None, C.c_uint64(0),
+                                       C.byref(m), C.byref(n)))
+        nodes = np.zeros((m.value, 4), np.uint32)
+        order = np.zeros(n.value, np.uint32)
+        _check(LIB.lrt_photon_map_tree(self._h, C.c_int32(kind), _p(nodes, C.c_uint32), C.c_uint64(m.value),
+                                       _p(order, C.c_uint32), C.c_uint64(n.value), C.byref(m), C.byref(n)))
+        return nodes, order
 
     def photon_map_load(self, photons, depth_particle, scale, radius, k=120, kind=PHOTON_SURFACE):
         ph = np.ascontiguousarray(photons, np.float32).reshape(-1, 12)

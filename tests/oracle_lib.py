@@ -208,10 +208,11 @@ class _SceneHandle:
           _p(ph), _p(dp, C.c_uint32), C.byref(shot))
         return ph[:n], dp[:n], shot.value
 
-    def render_photon(self, W, H, spp, maps, max_depth=128, k=120, seed=1, mode=0, threads=8, rows=None):
+    def render_photon(self, W, H, spp, maps, max_depth=128, k=120, seed=1, mode=0, threads=8, rows=None, tree=0):
         """photon_integrator::radiance per (pixel, sample) over maps built from
         maps = [(photons, depth_particle, scale, radius) or None] * 2 (global,
-        caustic).  Returns per-sample radiance [H, W, spp, 3] (float32)."""
+        caustic); tree (oracle only): 0 the reference's tree, 1 the library's
+        device tree.  Returns per-sample radiance [H, W, spp, 3] (float32)."""
         rb, re = rows if rows is not None else (0, 0)
         ps = np.zeros(H * W * spp * 3, np.float32)
         args = []
@@ -231,7 +232,7 @@ class _SceneHandle:
         else:
             self.impl.f("render_photon")(self.h, C.c_int(W), C.c_int(H), C.c_int(spp), C.c_int(max_depth),
                                          C.c_uint64(seed), C.c_int(mode), *args, C.c_int(k), C.c_int(threads),
-                                         C.c_int(rb), C.c_int(re), None, None, _p(ps))
+                                         C.c_int(rb), C.c_int(re), None, None, _p(ps), C.c_int(tree))
         return ps.reshape(H, W, spp, 3)
 
     def bounds(self):
@@ -241,8 +242,9 @@ class _SceneHandle:
         return r.value, bool(d.value)
 
 
-def photon_estimate(impl, photons, depth_particle, scale, queries, radius, max_depth, k, threads=8):
-    """photon_map build + estimate_irradiance through the oracle or the reference."""
+def photon_estimate(impl, photons, depth_particle, scale, queries, radius, max_depth, k, threads=8, tree=0):
+    """photon_map build + estimate_irradiance through the oracle or the reference;
+    tree (oracle only): 0 the reference's tree, 1 the library's device tree."""
     ph = np.ascontiguousarray(photons, np.float32).reshape(-1, 12)
     dp = np.ascontiguousarray(depth_particle, np.uint32).reshape(-1, 2)
     q = np.ascontiguousarray(queries, np.float32).reshape(-1, 6)
@@ -254,9 +256,23 @@ def photon_estimate(impl, photons, depth_particle, scale, queries, radius, max_d
     else:
         rc = impl.f("photon_estimate")(_p(ph), _p(dp, C.c_uint32), C.c_int64(ph.shape[0]), C.c_float(scale), _p(q),
                                        C.c_uint32(q.shape[0]), C.c_float(radius), C.c_int(max_depth), C.c_int(k),
-                                       _p(out), C.c_int(threads))
+                                       _p(out), C.c_int(threads), C.c_int(tree))
     assert rc == 0
     return out
+
+
+def photon_tree(xyz, rule="device"):
+    """The oracle's photon kd tree over positions [n, 3] (input order): rule
+    "device" (the library's median tree) or "reference" (photon_map's):
+    (nodes [m, 4] uint32, order [n] uint32)."""
+    xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
+    f = ORC.orc_photon_tree if rule == "device" else ORC.orc_photon_tree_reference
+    f.restype = C.c_int64
+    m = f(_p(xyz), C.c_int64(xyz.shape[0]), None, None)
+    nodes = np.zeros((m, 4), np.uint32)
+    order = np.zeros(xyz.shape[0], np.uint32)
+    f(_p(xyz), C.c_int64(xyz.shape[0]), _p(nodes, C.c_uint32), _p(order, C.c_uint32))
+    return nodes, order
 
 
 oracle = _Impl(ORC, "orc_") if ORC is not None else None

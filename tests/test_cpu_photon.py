@@ -138,3 +138,78 @@ def test_eye_pass_philox_streams_differ_but_agree():
     se = np.sqrt(a.var(2) / 64 + b.var(2) / 64) + 1e-6
     z = np.abs(ma - mb) / se
     assert np.mean(z < 3) > 0.97
+
+
+def _tree_sets(nodes, order, node=0, out=None):
+    """(axis, left, right, leaf set) of every node, pre-order."""
+    out = [] if out is None else out
+    w = nodes[node]
+    if w[1] == 3:
+        out.append(("leaf", frozenset(order[w[2]:w[2] + w[3]].tolist())))
+    else:
+        out.append((int(w[1]), int(w[2]), int(w[3])))
+    return out
+
+
+def test_device_tree_rule_against_the_reference_tree():
+    """The library's device photon tree (oracle restatement, pmap::build_median)
+    has the reference's photon_map tree's shape on photons without equal
+    coordinates: the same pre-order nodes, children, axes and leaf sets.  Its
+    split is the median's own coordinate, so it separates the children
+    exactly (every left photon <= split <= every right photon on the axis);
+    the reference's split, read after the right subtree was built, lies at or
+    above it and can exceed right-child photons."""
+    rng = np.random.default_rng(4)
+    xyz = (rng.random((50_000, 3)) * np.array([2.0, 1.0, 0.5])).astype(np.float32)
+    dn, do = OL.photon_tree(xyz, "device")
+    rn, ro = OL.photon_tree(xyz, "reference")
+    assert dn.shape == rn.shape and sorted(do.tolist()) == list(range(len(xyz)))
+    assert np.array_equal(dn[:, 1:][dn[:, 1] != 3], rn[:, 1:][rn[:, 1] != 3])  # axes and children
+    leaf = dn[:, 1] == 3
+    assert np.array_equal(leaf, rn[:, 1] == 3) and np.array_equal(dn[leaf], rn[leaf])  # leaf ranges
+    for w in dn[leaf]:
+        assert set(do[w[2]:w[2] + w[3]].tolist()) == set(ro[w[2]:w[2] + w[3]].tolist())
+    above = 0
+    for i in np.nonzero(~leaf)[0]:
+        ax, split = dn[i, 1], dn[i, 0:1].view(np.float32)[0]
+
+        def span(node):  # tree positions under node
+            w = dn[node]
+            if w[1] == 3:
+                return w[2], w[2] + w[3]
+            return span(w[2])[0], span(w[3])[1]
+        lo, mid = span(dn[i, 2])
+        _, hi = span(dn[i, 3])
+        c = xyz[do[lo:hi], ax]
+        assert c[:mid - lo].max() <= split <= c[mid - lo:].min() and split == c[mid - lo:].min()
+        ref_split = rn[i, 0:1].view(np.float32)[0]
+        assert ref_split >= split
+        above += ref_split > split
+        if i > 2000:
+            break
+    assert above > 0  # the reference's split read differs from the median at some nodes
+
+
+def test_device_tree_search_is_exact():
+    """Over the device tree the k-nearest search finds the exact k nearest
+    photons within the radius (the brute-force estimate); over the reference's
+    tree it misses neighbours at some queries."""
+    rng = np.random.default_rng(8)
+    n = 20_000
+    ph = np.zeros((n, 12), np.float32)
+    ph[:, 0:3] = rng.random((n, 3)) * np.array([2.0, 0.02, 2.0])
+    ph[:, 3:6] = [0.0, -1.0, 0.0]
+    ph[:, 6:9] = [0.0, 1.0, 0.0]
+    ph[:, 9:12] = rng.random((n, 3))
+    dp = np.stack([np.full(n, 2), np.arange(n)], 1).astype(np.uint32)
+    q = np.zeros((1500, 6), np.float32)
+    q[:, 0:3] = rng.random((1500, 3)) * np.array([2.0, 0.02, 2.0])
+    q[:, 3:6] = [0.0, 1.0, 0.0]
+    b = brute_estimate(ph, dp, 1e-3, q, 0.08, 100, 32)
+    dev = OL.photon_estimate(OL.oracle, ph, dp, 1e-3, q, 0.08, 100, 32, tree=1)
+    ref = OL.photon_estimate(OL.oracle, ph, dp, 1e-3, q, 0.08, 100, 32, tree=0)
+    ok_dev = np.all(np.isclose(dev, b, rtol=2e-5, atol=1e-9), axis=1)
+    ok_ref = np.all(np.isclose(ref, b, rtol=2e-5, atol=1e-9), axis=1)
+    print(f"queries equal to the brute-force k nearest: device tree {ok_dev.mean():.4f}, reference tree {ok_ref.mean():.4f}")
+    assert ok_dev.mean() >= 0.998
+    assert ok_ref.mean() < ok_dev.mean()

@@ -115,8 +115,9 @@ def _density(ph, q, r=0.1):
 
 
 def test_estimate_large_map_vs_oracle(pbox):
-    """20,000 queries with k = 120 over the device-built global map, against
-    the oracle's photon_map on the exported photons."""
+    """20,000 queries with k = 120 over the device-built global map (the
+    device median tree), against the oracle's photon_map with the same tree
+    rule on the exported photons."""
     g, sc, orc = pbox
     sc.photon_build(60000, 8, 5, seed=7, radius=0.05, k=120)
     ph, dp, shot = sc.photon_export()
@@ -126,11 +127,50 @@ def test_estimate_large_map_vs_oracle(pbox):
                         ph[rng.permutation(pick), 6:9]], 1).astype(np.float32)
     for md in (100, 4):
         e = sc.photon_estimate(q, md)
-        o = OL.photon_estimate(OL.oracle, ph, dp, 1.0 / shot, q, 0.05, md, 120)
+        o = OL.photon_estimate(OL.oracle, ph, dp, 1.0 / shot, q, 0.05, md, 120, tree=1)
         exact = np.mean(np.all(e == o, axis=1))
         print(f"depth<={md}: {exact:.4%} bit-exact, {np.mean(o.any(1)):.2%} nonzero")
         assert np.allclose(e, o, rtol=1e-5, atol=1e-7 * np.abs(o).max())
         assert exact >= 0.99
+
+
+@pytest.mark.parametrize("kind,req", ((linrt.PHOTON_SURFACE, 60000), (linrt.PHOTON_CAUSTIC, 20000),
+                                      (linrt.PHOTON_SURFACE, 1 << 20)))
+def test_device_tree_equals_oracle(pbox, kind, req):
+    """The kd tree lrt_photon_build makes on the device (nodes in pre-order,
+    tree order of the photons) is node for node the oracle's restatement of
+    the same rule (pmap::build_median) over the exported photons, and has the
+    reference tree's shape."""
+    g, sc, orc = pbox
+    st = sc.photon_build(req, 8, 5, seed=7, radius=0.05, k=64, kind=kind)
+    nodes, order = sc.photon_map_tree(kind)
+    ph, dp, shot = sc.photon_export(kind)
+    on, oo = OL.photon_tree(ph[:, :3], "device")
+    print(f"kind {kind}: {len(ph)} photons, {len(nodes)} nodes, map {st.sort_ms:.1f} ms, trace {st.trace_ms:.1f} ms")
+    assert np.array_equal(nodes, on)
+    assert np.array_equal(order, oo)
+    # the reference tree's shape (it depends on the photon count only): the
+    # same pre-order nodes, children and leaf ranges; the axes agree except
+    # below medians that tie (photons on the room's axis-aligned walls share
+    # coordinates exactly, and the two rules send different tied photons left;
+    # without ties the trees are the same, test_cpu_photon.py)
+    rn, _ = OL.photon_tree(ph[:, :3], "reference")
+    assert rn.shape == nodes.shape
+    leaf = nodes[:, 1] == 3
+    assert np.array_equal(leaf, rn[:, 1] == 3)
+    assert np.array_equal(nodes[leaf], rn[leaf]) and np.array_equal(nodes[~leaf, 2:], rn[~leaf, 2:])
+    print(f"  interior nodes with the reference tree's axis: {np.mean(nodes[~leaf, 1] == rn[~leaf, 1]):.5f}")
+
+
+def test_reference_tree_mode(pbox):
+    """tree = reference: the reference's std::nth_element tree, built on the
+    host, node for node the oracle's photon_map (both follow photon_map.h:83-114)."""
+    g, sc, orc = pbox
+    sc.photon_build(60000, 8, 5, seed=7, radius=0.05, k=64, tree=linrt.PHOTON_TREE_REFERENCE)
+    nodes, order = sc.photon_map_tree()
+    ph, _, _ = sc.photon_export()
+    rn, ro = OL.photon_tree(ph[:, :3], "reference")
+    assert np.array_equal(nodes, rn) and np.array_equal(order, ro)
 
 
 def compare_images(a_sum, a_sq, b_ps, label, rtol, frac):
@@ -167,7 +207,8 @@ def test_eye_pass_sphere_room_vs_oracle():
 
 def test_eye_pass_photon_box_vs_oracle(pbox):
     """The photon box (seven region materials incl. rough coatings) with
-    device-built global and caustic maps, exported to the oracle."""
+    device-built global and caustic maps (device trees), exported to the
+    oracle, which builds the same trees."""
     g, sc, orc = pbox
     R = sc.info.bounding_radius
     sc.photon_build(40000, 127, 5, seed=5, radius=0.05 * R, k=120, kind=linrt.PHOTON_SURFACE)
@@ -177,7 +218,7 @@ def test_eye_pass_photon_box_vs_oracle(pbox):
     W, H, spp = 24, 24, 2
     s, q, st = sc.render_photon(W, H, spp, 128, seed=3)
     maps = [(gp, gd, 1.0 / gs, 0.05 * R), (cp, cd, 1.0 / cs, 0.0125 * R)]
-    ps = orc.render_photon(W, H, spp, maps, mode=0, seed=3)
+    ps = orc.render_photon(W, H, spp, maps, mode=0, seed=3, tree=1)
     compare_images(s, q, ps, "photon box", 1e-4, 0.99)
 
 
