@@ -237,8 +237,6 @@ DevLight make_light(const lrt_light_desc& d, const std::vector<lrt_object_desc>&
         std::memcpy(L.rad, d.channel_radiance, sizeof(L.rad));
     } else {
         require(d.object >= 0 && (size_t)d.object < objs.size(), "area light object out of range");
-        require(objs[(size_t)d.object].shape == LRT_SHAPE_RECTANGLE,
-                "area lights are supported on rectangle shapes only", LRT_ERR_UNSUPPORTED);
     }
     return L;
 }
@@ -294,6 +292,8 @@ struct lrt_scene {
     Mats M;
     CameraDev cam;
     DevBuf nodes, refs, tris, cnormals, trimat, mats, tables, obj_box;
+    DevBuf light_cdf, light_tris;  // mesh lights: running triangle areas, triangle vertices
+    bool mesh_lights = false;      // some area light is a mesh: shading kernels with its pdf ray cast
     lrt_scene_info info;
     std::vector<lrt_object_desc> objects;  // shapes only (pointers not kept)
     // path-tracer work buffers
@@ -332,6 +332,9 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
     std::vector<uint32_t> words, refs;
     std::vector<float> tris, cn;
     std::vector<uint16_t> tmat;
+    // mesh emitters: running triangle areas and vertices (triangle_mesh.h:16-25)
+    std::vector<float> lcdf, ltris;
+    std::vector<uint32_t> obj_cdf_base(kMaxObjects, 0), obj_cdf_n(kMaxObjects, 0);
     bool any_tmat = false;
     int max_depth = 0;
     // per object: min xyz, max xyz of a mesh's triangles (empty box otherwise)
@@ -359,6 +362,9 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
             ob.n_tris = nt;
             ob.per_tri_material = od.triangle_material ? 1 : 0;
             std::vector<float> boxes(6 * (size_t)nt);
+            const bool emissive = od.emitter >= 0;
+            if (emissive) obj_cdf_base[o] = (uint32_t)lcdf.size();
+            float area = 0.0f;
             for (uint32_t t = 0; t < nt; ++t) {
                 const uint32_t i0 = od.indices[3 * t], i1 = od.indices[3 * t + 1], i2 = od.indices[3 * t + 2];
                 require(i0 < od.n_vertices && i1 < od.n_vertices && i2 < od.n_vertices,
@@ -366,6 +372,15 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
                 const f3 p0 = ld3(od.positions + 3 * (size_t)i0), p1 = ld3(od.positions + 3 * (size_t)i1),
                          p2 = ld3(od.positions + 3 * (size_t)i2);
                 f3 e1 = p1 - p0, e2 = p2 - p0;
+                {  // triangle_mesh.h:19-25: area += 0.5 * |cross(p1 - p0, p2 - p0)|
+                    const f3 g = cross(e1, e2);
+                    area += 0.5f * std::sqrt(g.x * g.x + g.y * g.y + g.z * g.z);
+                    if (emissive) {
+                        lcdf.push_back(area);
+                        const float v9[9] = {p0.x, p0.y, p0.z, p1.x, p1.y, p1.z, p2.x, p2.y, p2.z};
+                        ltris.insert(ltris.end(), v9, v9 + 9);
+                    }
+                }
                 if (len2(cross(e1, e2)) <= 0.0f) e1 = e2 = splat(0.0f);  // triangle_mesh.h:52-54
                 const float rec[9] = {p0.x, p0.y, p0.z, e1.x, e1.y, e1.z, e2.x, e2.y, e2.z};
                 tris.insert(tris.end(), rec, rec + 9);
@@ -391,6 +406,8 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
                     tmat.push_back((uint16_t)std::max(0, ob.material));
                 }
             }
+            ob.area = area;
+            if (emissive) obj_cdf_n[o] = nt;
             KdTree kt;
             kd_build(boxes.data(), nt, kt);
             FlatTree ft;
@@ -476,7 +493,24 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
     sc.M.tables = sc.tables.as<float>();
     std::memset(&sc.Ls, 0, sizeof(sc.Ls));
     sc.Ls.n = (int32_t)d.n_lights;
-    for (uint32_t i = 0; i < d.n_lights; ++i) sc.Ls.l[i] = make_light(d.lights[i], sc.objects);
+    sc.S.nonconvex_emitters = 0u;
+    sc.mesh_lights = false;
+    for (uint32_t i = 0; i < d.n_lights; ++i) {
+        sc.Ls.l[i] = make_light(d.lights[i], sc.objects);
+        if (d.lights[i].kind == LRT_LIGHT_AREA && d.objects[d.lights[i].object].shape == LRT_SHAPE_MESH) {
+            const int o = d.lights[i].object;
+            sc.Ls.l[i].cdf_base = obj_cdf_base[(size_t)o];
+            sc.Ls.l[i].n_tris = obj_cdf_n[(size_t)o];
+            sc.S.nonconvex_emitters |= 1u << i;
+            sc.mesh_lights = true;
+        }
+    }
+    if (lcdf.empty()) lcdf.push_back(0.0f);
+    if (ltris.empty()) ltris.assign(9, 0.0f);
+    upload(sc.light_cdf, lcdf.data(), lcdf.size() * 4, sc.stream);
+    upload(sc.light_tris, ltris.data(), ltris.size() * 4, sc.stream);
+    sc.Ls.cdf = sc.light_cdf.as<float>();
+    sc.Ls.tri9 = sc.light_tris.as<float>();
     for (uint32_t o = 0; o < d.n_objects; ++o)
         if (d.objects[o].emitter >= 0) {
             const lrt_light_desc& L = d.lights[d.objects[o].emitter];
@@ -1116,7 +1150,8 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
                 hipLaunchKernelGGL(k_shade_bins, dim3(g_bins), dim3(kBlock), 0, sc->stream, sc->S, Hb, counters);
                 hipLaunchKernelGGL(k_shade_scan, dim3(1), dim3(1024), 0, sc->stream, counters);
                 hipLaunchKernelGGL(k_shade_order, dim3(g_bins), dim3(kBlock), 0, sc->stream, sc->S, Hb, counters, order);
-                hipLaunchKernelGGL(k_shade, dim3(g_shade), dim3(kBlock), 0, sc->stream, sc->S, sc->Ls, sc->M, P, q, qn,
+                hipLaunchKernelGGL(sc->mesh_lights ? k_shade<true> : k_shade<false>, dim3(g_shade), dim3(kBlock), 0,
+                                   sc->stream, sc->S, sc->Ls, sc->M, P, q, qn,
                                    counters, Hb, order, Sb, sp);
                 HIPCHECK(hipEventRecord(e2.second, sc->stream));
                 if (capture)
@@ -1443,6 +1478,8 @@ int lrt_diff_render(lrt_scene* sc, const lrt_diff_params* dp, const float* theta
                     float* image, int device_ptrs) {
     return guarded([&] {
         require(sc && dp && theta && prm && image, "null pointer");
+        require(!sc->mesh_lights, "the differentiable renderer takes rectangle, sphere and sky lights (no mesh lights)",
+                LRT_ERR_UNSUPPORTED);
         HIPCHECK(hipSetDevice(sc->device));
         float sv[3];
         const float* dth = diff_theta(*sc, *dp, theta, device_ptrs, sv);
@@ -1466,6 +1503,8 @@ int lrt_diff_render_held(lrt_scene* sc, const lrt_diff_params* dp, const float* 
                          const lrt_render_params* prm, float* image) {
     return guarded([&] {
         require(sc && dp && theta && theta_sampling && prm && image, "null pointer");
+        require(!sc->mesh_lights, "the differentiable renderer takes rectangle, sphere and sky lights (no mesh lights)",
+                LRT_ERR_UNSUPPORTED);
         HIPCHECK(hipSetDevice(sc->device));
         const uint32_t np = diff_count(*dp);
         // BSDF values at theta (texture and scalars), sampling at theta_sampling's scalars
@@ -1493,6 +1532,8 @@ int lrt_diff_adjoint(lrt_scene* sc, const lrt_diff_params* dp, const float* thet
                      const float* d_image, float* grad, int device_ptrs) {
     return guarded([&] {
         require(sc && dp && theta && prm && d_image && grad, "null pointer");
+        require(!sc->mesh_lights, "the differentiable renderer takes rectangle, sphere and sky lights (no mesh lights)",
+                LRT_ERR_UNSUPPORTED);
         HIPCHECK(hipSetDevice(sc->device));
         float sv[3];
         const float* dth = diff_theta(*sc, *dp, theta, device_ptrs, sv);
@@ -2066,7 +2107,8 @@ static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, doub
                 HIPCHECK(hipEventRecord(e1.second, sc->stream));
                 auto e2 = ts.next();
                 HIPCHECK(hipEventRecord(e2.first, sc->stream));
-                hipLaunchKernelGGL(k_eye_shade, dim3(g), dim3(kBlock), 0, sc->stream, sc->S, sc->Ls, sc->M, A, n, Hb, B, Sh,
+                hipLaunchKernelGGL(sc->mesh_lights ? k_eye_shade<true> : k_eye_shade<false>, dim3(g), dim3(kBlock), 0,
+                                   sc->stream, sc->S, sc->Ls, sc->M, A, n, Hb, B, Sh,
                                    Q, counters, ep, acc);
                 HIPCHECK(hipEventRecord(e2.second, sc->stream));
                 HIPCHECK(hipGetLastError());

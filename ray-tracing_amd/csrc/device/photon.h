@@ -31,7 +31,7 @@ struct EmitCdf {
     float total;
 };
 
-// scene::sample_emitter_ray + area_emitter::sample_ray (rectangles)
+// scene::sample_emitter_ray + area_emitter::sample_ray (area_emitter.h:93-122)
 LRT_D bool emitter_ray(const DevScene& S, const DevLights& Ls, const EmitCdf& E, float sx, float sy, float dx, float dy,
                        Ray& r, f3& weight) {
     if (E.total <= 0.0f || E.n <= 0) return false;
@@ -47,8 +47,9 @@ LRT_D bool emitter_ray(const DevScene& S, const DevLights& Ls, const EmitCdf& E,
     const DevLight& L = Ls.l[E.index[bin]];
     const DevObject& ob = S.obj[L.object];
     if (ob.area <= 0.0f) return false;
-    const f3 p = ld3(ob.corner) + u * ld3(ob.eu) + sy * ld3(ob.ev);
-    const f3 n = ld3(ob.rn);
+    f3 p, n;
+    float pdf_area;
+    if (!shape_sample(S, Ls, L, u, sy, p, n, pdf_area) || !(pdf_area > 0.0f)) return false;
     const float ct = sqrtf(dx), st = sqrtf(1.0f - dx);
     const float phi = 2.0f * kPiF * dy;
     float sp, cp;

@@ -35,8 +35,18 @@ struct DevScene {
     const float* obj_box;      // per object 6 floats: enlarged AABB of a mesh (min xyz, max xyz)
     int32_t n_objects;
     int32_t stack_depth;       // deepest root-to-leaf path over all trees
+    uint32_t nonconvex_emitters;  // bit e: emitter e's shape is a mesh (its own surface can hide a blocker)
+    uint32_t pad;
     DevObject obj[kMaxObjects];
 };
+
+// An emitter whose surface may lie between a shading point and the point
+// sampled on it: the shadow test must be the reference's closest-hit form
+// (blocked iff the nearest surface in the window is not the emitter's,
+// path_integrator.h:113-116) rather than an any-hit that skips the emitter.
+LRT_D bool emitter_nonconvex(const DevScene& S, int e) {
+    return e >= 0 && e < 32 && ((S.nonconvex_emitters >> e) & 1u) != 0u;
+}
 
 #ifndef LRT_LDS_STACK
 #define LRT_LDS_STACK 8
@@ -199,11 +209,14 @@ LRT_D bool mesh_box_clip(const DevScene& S, int o, const Ray& r, const float inv
 // leaf code once per node step.  The visit order is the reference's: the
 // descent neither changes mint nor closest, so the reference's per-node test
 // closest < mint can only change after a pop.
-template <bool ANY, bool COUNT>
+// USE_LDS false: the whole stack in private memory (lds unused), for callers
+// without an LDS stack (the emitter pdf's ray cast of a mesh light).
+template <bool ANY, bool COUNT, bool USE_LDS = true>
 LRT_D bool kd_traverse(const DevScene& S, const DevObject& ob, const Ray& r, float inv[3], float tmin,
                        float& closest, int32_t& prim, float& hu, float& hv, uint2* lds, Counters& cnt) {
     const float maxt0 = closest;
-    uint2 spill[kMaxStack - kLdsStack];
+    constexpr int kLds = USE_LDS ? kLdsStack : 0;
+    uint2 spill[kMaxStack - kLds];
     int top = 0;
     float mint = tmin, maxt = closest;
     uint32_t node = ob.node_base;
@@ -230,8 +243,12 @@ LRT_D bool kd_traverse(const DevScene& S, const DevObject& ob, const Ray& r, flo
                 node = second;
             } else {
                 const uint2 e = make_uint2(second, __float_as_uint(t_plane));
-                if (top < kLdsStack) lds[top * stride] = e;
-                else spill[top - kLdsStack] = e;
+                if constexpr (USE_LDS) {
+                    if (top < kLds) lds[top * stride] = e;
+                    else spill[top - kLds] = e;
+                } else {
+                    spill[top] = e;
+                }
                 ++top;
                 node = first;
                 maxt = t_plane;
@@ -280,16 +297,42 @@ LRT_D bool kd_traverse(const DevScene& S, const DevObject& ob, const Ray& r, flo
         }
         if (top == 0) break;
         --top;
-        const uint2 e = top < kLdsStack ? lds[top * stride] : spill[top - kLdsStack];
+        uint2 e;
+        if constexpr (USE_LDS) e = top < kLds ? lds[top * stride] : spill[top - kLds];
+        else e = spill[top];
         node = e.x;
         mint = __uint_as_float(e.y);
         if (top == 0) maxt = maxt0;
         else {
-            const uint2 below = (top - 1) < kLdsStack ? lds[(top - 1) * stride] : spill[top - 1 - kLdsStack];
+            uint2 below;
+            if constexpr (USE_LDS) below = (top - 1) < kLds ? lds[(top - 1) * stride] : spill[top - 1 - kLds];
+            else below = spill[top - 1];
             maxt = __uint_as_float(below.y);
         }
     }
     return found;
+}
+
+// The normal triangle_mesh::hit records for a hit on triangle prim of mesh ob
+// at barycentrics (u, v) by a ray of direction d (triangle_mesh.h:81-96):
+// front face from the geometric normal; the normal is the interpolated corner
+// normal, normalised and turned toward the ray's origin, or without corner
+// normals the face-forwarded geometric normal.
+LRT_D void mesh_hit_normal(const DevScene& S, const DevObject& ob, int prim, float u, float v, f3 d, f3& n, bool& front) {
+    const uint32_t g = ob.tri_base + (uint32_t)prim;
+    const float* tri = S.tris + 9 * (size_t)g;
+    const f3 gn = cross(ld3(tri + 3), ld3(tri + 6));
+    const f3 outward = gn / len(gn);
+    front = dot(d, outward) < 0;
+    n = front ? outward : -outward;
+    const float* cn = S.cnormals + 9 * (size_t)g;
+    const float b0 = 1.0f - u - v;
+    f3 sh = b0 * ld3(cn) + u * ld3(cn + 3) + v * ld3(cn + 6);
+    if (len2(sh) > 0.0f) {
+        sh = sh / len(sh);
+        if (dot(d, sh) > 0.0f) sh = -sh;
+        n = sh;
+    }
 }
 
 // hittable_list::hit over the object list (hittable_list.h:19-34); a later
@@ -331,10 +374,15 @@ LRT_D bool trace_closest(const DevScene& S, const Ray& r, float tmin, float tmax
 // Shadow test of path_integrator.h:113-116: blocked iff a surface that does
 // not belong to emitter `ignore` is hit in [tmin, tmax].  Exact when the
 // ignored emitter is convex (rectangle, sphere): its own surface cannot be
-// nearer than the sampled point on it.
+// nearer than the sampled point on it; a mesh emitter takes the reference's
+// closest-hit form.
 LRT_D bool trace_occluded(const DevScene& S, const Ray& r, float tmin, float tmax, int ignore, uint2* lds) {
     float inv[3] = {1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
     Counters cnt{0, 0, 0};
+    if (emitter_nonconvex(S, ignore)) {  // the reference's closest-hit form
+        HitInfo h;
+        return trace_closest<false>(S, r, tmin, tmax, h, lds, cnt) && S.obj[h.obj].emitter != ignore;
+    }
     for (int o = 0; o < S.n_objects; ++o) {
         const DevObject& ob = S.obj[o];
         if (ob.emitter >= 0 && ob.emitter == ignore) continue;
@@ -403,6 +451,17 @@ LRT_D void lane_trace_start(LaneTrace& T, const Ray& r, float tmin, float tmax, 
     T.in_mesh = false;
     T.hit = false;
     T.any = false;
+}
+
+// Shadow rays (path_integrator.h:104-116): an any-hit that skips the sampled
+// emitter's surfaces, or for a non-convex emitter the reference's closest-hit
+// form; shadow_blocked decides when the ray is finished.
+LRT_D void lane_shadow_start(const DevScene& S, LaneTrace& T, const Ray& r, float tmin, float tmax, int ignore) {
+    lane_trace_start(T, r, tmin, tmax, ignore);
+    T.any = !emitter_nonconvex(S, ignore);
+}
+LRT_D bool shadow_blocked(const DevScene& S, const LaneTrace& T, int ignore) {
+    return T.hit && (T.any || S.obj[T.obj()].emitter != ignore);
 }
 
 // One round; returns true when the ray is finished (T.hit / T.obj(), T.prim,

@@ -292,16 +292,16 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_diff_shadow(const Dev
         const uint32_t got = wave_refill(!active, &counters[5], n, more);
         if (got != 0xffffffffu) {
             i = got;
-            lane_trace_start(T, Ray{f3{Sb.ox[i], Sb.oy[i], Sb.oz[i]}, f3{Sb.dx[i], Sb.dy[i], Sb.dz[i]}}, 0.001f,
-                             Sb.tmax[i], Sb.ignore[i]);
+            lane_shadow_start(S, T, Ray{f3{Sb.ox[i], Sb.oy[i], Sb.oz[i]}, f3{Sb.dx[i], Sb.dy[i], Sb.dz[i]}}, 0.001f,
+                              Sb.tmax[i], Sb.ignore[i]);
             active = true;
         }
         if (!__ballot(active)) {
             if (!more) break;
             continue;
         }
-        if (active && lane_trace_round<true>(S, T, lds, spill)) {
-            if (!T.hit) {
+        if (active && lane_trace_round<false>(S, T, lds, spill)) {
+            if (!shadow_blocked(S, T, Sb.ignore[i])) {
                 float4* w = path_words(P, Sb.path[i]);
                 w[3] = word(xyz(w[3]) + f3{Sb.cr[i], Sb.cg[i], Sb.cb[i]}, 0.0f);
                 const int t = DS.tex[i];

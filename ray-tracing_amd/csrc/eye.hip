@@ -122,6 +122,7 @@ LRT_D void eye_push(EyeItems& O, uint32_t s, uint32_t sid, uint32_t path, uint32
 }
 
 // counters: [0] children, [1] shadow rays, [2] photon queries
+template <bool MESH_LIGHTS>  // as k_shade
 __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_eye_shade(const DevScene S, const DevLights Ls, const Mats M,
                                                       const EyeItems It, uint32_t n, const HitBuf Hb, EyeItems Out,
                                                       EyeShadow Sh, EyeQuery Q, uint32_t* counters, EyeParams prm,
@@ -155,10 +156,10 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_eye_shade(const DevSc
             if (hit && sh.emitter >= 0 && sh.front) {
                 const f3 out = -normalize(r.d);
                 emission = dot(sh.n, out) <= 0.0f ? splat(0.0f) : ld3(Ls.l[sh.emitter].radiance);
-                lp = pdf_emitter_direction(S, Ls, r.o, r.d);
+                lp = pdf_emitter_direction<MESH_LIGHTS>(S, Ls, r.o, r.d);
             } else if (!hit) {
                 emission = env_eval(Ls, r.d);
-                lp = pdf_emitter_direction(S, Ls, r.o, r.d);
+                lp = pdf_emitter_direction<MESH_LIGHTS>(S, Ls, r.o, r.d);
             }
             if (!is_zero(emission)) {
                 const bool first = depth == 2u;
@@ -267,7 +268,7 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_eye_shade(const DevSc
                         sr.wo = to_local(fr, sd);
                         const EP e = bsdf_ep(M, sh.material, ctx, sr, M_SOLID);
                         if (!is_zero(e.f)) {
-                            const float lp = pdf_emitter_direction(S, Ls, sh.p, sd);
+                            const float lp = pdf_emitter_direction<MESH_LIGHTS>(S, Ls, sh.p, sd);
                             if (lp > 0.0f) {
                                 const float w = power_heuristic(lp * (float)ce, e.pdf * (float)cbn) * ew;
                                 contrib = T * (e.f * ds.radiance * (w / lp));
@@ -332,16 +333,16 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_eye_shadow(const DevS
         const uint32_t got = wave_refill(!active, next, n, more);
         if (got != 0xffffffffu) {
             i = got;
-            lane_trace_start(T, Ray{f3{Sh.ox[i], Sh.oy[i], Sh.oz[i]}, f3{Sh.dx[i], Sh.dy[i], Sh.dz[i]}}, 0.001f,
-                             Sh.tmax[i], Sh.ignore[i]);
+            lane_shadow_start(S, T, Ray{f3{Sh.ox[i], Sh.oy[i], Sh.oz[i]}, f3{Sh.dx[i], Sh.dy[i], Sh.dz[i]}}, 0.001f,
+                              Sh.tmax[i], Sh.ignore[i]);
             active = true;
         }
         if (!__ballot(active)) {
             if (!more) break;
             continue;
         }
-        if (active && lane_trace_round<true>(S, T, lds, spill)) {
-            if (!T.hit) eye_add(acc, Sh.sid[i], f3{Sh.cr[i], Sh.cg[i], Sh.cb[i]});
+        if (active && lane_trace_round<false>(S, T, lds, spill)) {
+            if (!shadow_blocked(S, T, Sh.ignore[i])) eye_add(acc, Sh.sid[i], f3{Sh.cr[i], Sh.cg[i], Sh.cb[i]});
             active = false;
         }
     }

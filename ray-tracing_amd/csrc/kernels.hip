@@ -101,17 +101,17 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_trace_occluded(const 
         const uint32_t got = wave_refill(!active, next, n, more);
         if (got != 0xffffffffu) {
             i = got;
-            lane_trace_start(T, Ray{f3{rays[i], rays[n + i], rays[2 * (size_t)n + i]},
-                                    f3{rays[3 * (size_t)n + i], rays[4 * (size_t)n + i], rays[5 * (size_t)n + i]}},
-                             rays[6 * (size_t)n + i], rays[7 * (size_t)n + i], ignore ? ignore[i] : -1);
+            lane_shadow_start(S, T, Ray{f3{rays[i], rays[n + i], rays[2 * (size_t)n + i]},
+                                       f3{rays[3 * (size_t)n + i], rays[4 * (size_t)n + i], rays[5 * (size_t)n + i]}},
+                              rays[6 * (size_t)n + i], rays[7 * (size_t)n + i], ignore ? ignore[i] : -1);
             active = true;
         }
         if (!__ballot(active)) {
             if (!more) break;
             continue;
         }
-        if (active && lane_trace_round<true>(S, T, lds, spill)) {
-            out[i] = T.hit ? 1 : 0;
+        if (active && lane_trace_round<false>(S, T, lds, spill)) {
+            out[i] = shadow_blocked(S, T, ignore ? ignore[i] : -1) ? 1 : 0;
             active = false;
         }
     }
@@ -299,19 +299,7 @@ LRT_D SurfaceHit surface(const DevScene& S, const Ray& r, int obj, int prim, flo
     s.emitter = ob.emitter;
     if (ob.shape == kShapeMesh) {
         const uint32_t g = ob.tri_base + (uint32_t)prim;
-        const float* tri = S.tris + 9 * (size_t)g;
-        const f3 gn = cross(ld3(tri + 3), ld3(tri + 6));
-        const f3 outward = gn / len(gn);
-        s.front = dot(r.d, outward) < 0;
-        s.n = s.front ? outward : -outward;
-        const float* cn = S.cnormals + 9 * (size_t)g;
-        const float b0 = 1.0f - u - v;
-        f3 sh = b0 * ld3(cn) + u * ld3(cn + 3) + v * ld3(cn + 6);
-        if (len2(sh) > 0.0f) {
-            sh = sh / len(sh);
-            if (dot(r.d, sh) > 0.0f) sh = -sh;
-            s.n = sh;
-        }
+        mesh_hit_normal(S, ob, prim, u, v, r.d, s.n, s.front);
         s.material = ob.per_tri_material ? (int)S.tri_mat[g] : ob.material;
     } else if (ob.shape == kShapeRect) {
         const f3 rn = ld3(ob.rn);
@@ -397,6 +385,9 @@ __global__ __launch_bounds__(kBlock) void k_shade_order(const DevScene S, const 
 }
 
 // one bounce of path_integrator::radiance for every queued path
+// MESH_LIGHTS: compiled with the mesh lights' pdf ray cast (launched only for
+// scenes that have one, so the other scenes' shading keeps its registers)
+template <bool MESH_LIGHTS>
 __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_shade(const DevScene S, const DevLights Ls, const Mats M, PathState P,
                                                   const uint32_t* __restrict__ queue, uint32_t* __restrict__ next_queue,
                                                   uint32_t* counters, const HitBuf Hb, const uint32_t* __restrict__ order,
@@ -429,7 +420,7 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_shade(const DevScene 
                 if (!have) {
                     L = L + T * Le;
                 } else {
-                    const float lp = pdf_emitter_direction(S, Ls, splat(0.0f), r.d);
+                    const float lp = pdf_emitter_direction<MESH_LIGHTS>(S, Ls, splat(0.0f), r.d);
                     L = L + T * power_heuristic(sa_pdf, lp) * Le;
                 }
             } else {
@@ -441,7 +432,7 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_shade(const DevScene 
                         if (!have) {
                             L = L + T * Le;
                         } else {
-                            const float lp = pdf_emitter_direction(S, Ls, r.o, r.d);
+                            const float lp = pdf_emitter_direction<MESH_LIGHTS>(S, Ls, r.o, r.d);
                             L = L + T * power_heuristic(sa_pdf, lp) * Le;
                         }
                     }
@@ -495,7 +486,7 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_shade(const DevScene 
                                 sr.wo = to_local(fr, sd);
                                 const EP e = bsdf_ep(M, sh.material, ctx, sr, M_SOLID);
                                 if (!is_zero(e.f)) {
-                                    const float lp = pdf_emitter_direction(S, Ls, sh.p, sd);
+                                    const float lp = pdf_emitter_direction<MESH_LIGHTS>(S, Ls, sh.p, sd);
                                     if (lp > 0.0f) {
                                         const float w = power_heuristic(lp, e.pdf);
                                         contrib = T * e.f * ds.radiance * (w / lp);
@@ -624,16 +615,16 @@ __global__ __launch_bounds__(kBlock) LRT_PATH_TRACE_ATTR void k_shadow(const Dev
         const uint32_t got = wave_refill(!active, &counters[5], n, more);
         if (got != 0xffffffffu) {
             i = got;
-            lane_trace_start(T, Ray{f3{Sb.ox[i], Sb.oy[i], Sb.oz[i]}, f3{Sb.dx[i], Sb.dy[i], Sb.dz[i]}}, 0.001f,
-                             Sb.tmax[i], Sb.ignore[i]);
+            lane_shadow_start(S, T, Ray{f3{Sb.ox[i], Sb.oy[i], Sb.oz[i]}, f3{Sb.dx[i], Sb.dy[i], Sb.dz[i]}}, 0.001f,
+                              Sb.tmax[i], Sb.ignore[i]);
             active = true;
         }
         if (!__ballot(active)) {
             if (!more) break;
             continue;
         }
-        if (active && lane_trace_round<true>(S, T, lds, spill)) {
-            if (!T.hit) {
+        if (active && lane_trace_round<false>(S, T, lds, spill)) {
+            if (!shadow_blocked(S, T, Sb.ignore[i])) {
                 float4* w = path_words(P, Sb.path[i]);
                 w[3] = word(xyz(w[3]) + f3{Sb.cr[i], Sb.cg[i], Sb.cb[i]}, 0.0f);
             }
@@ -665,9 +656,8 @@ __global__ __launch_bounds__(kBlock) LRT_PATH_TRACE_ATTR void k_trace_queues(con
         if (got != 0xffffffffu) {
             i = got;
             if (i < ns) {
-                lane_trace_start(T, Ray{f3{Sb.ox[i], Sb.oy[i], Sb.oz[i]}, f3{Sb.dx[i], Sb.dy[i], Sb.dz[i]}}, 0.001f,
-                                 Sb.tmax[i], Sb.ignore[i]);
-                T.any = true;
+                lane_shadow_start(S, T, Ray{f3{Sb.ox[i], Sb.oy[i], Sb.oz[i]}, f3{Sb.dx[i], Sb.dy[i], Sb.dz[i]}},
+                                  0.001f, Sb.tmax[i], Sb.ignore[i]);
             } else {
                 const float4* w = path_words(P, next_queue[i - ns]);
                 lane_trace_start(T, Ray{xyz(w[0]), xyz(w[1])}, 0.001f, kInfF, -1);
@@ -680,7 +670,7 @@ __global__ __launch_bounds__(kBlock) LRT_PATH_TRACE_ATTR void k_trace_queues(con
         }
         if (active && lane_trace_round<false>(S, T, lds, spill)) {
             if (i < ns) {
-                if (!T.hit) {
+                if (!shadow_blocked(S, T, Sb.ignore[i])) {
                     float4* w = path_words(P, Sb.path[i]);
                     w[3] = word(xyz(w[3]) + f3{Sb.cr[i], Sb.cg[i], Sb.cb[i]}, 0.0f);
                 }

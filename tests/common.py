@@ -167,3 +167,70 @@ class SphereRoom:
         self.desc = linrt.SceneDesc(len(objs), self.objects, len(mats), self.materials, 1, self.lights, 0, None,
                                     d.camera)
         self.desc_ptr = C_.pointer(self.desc)
+
+
+class EmitterRoom:
+    """The box room of the 256x256 workload lit by three area lights of
+    different shapes (area_emitter.h over triangle_mesh / sphere / rectangle):
+    light 0 a non-convex emissive mesh (an icosphere with radial bumps, its
+    corner normals interpolated, so a ray toward one part of it can meet
+    another part first), light 1 an emissive analytic sphere, light 2 the
+    ceiling rectangle; a diffuse sphere and a plastic sphere receive the
+    light.  Every emitter also carries a diffuse BSDF, as a reference shape
+    may."""
+
+    def __init__(self, seed=1, bumps=0.25):
+        import ctypes as C_
+        self.gen = linrt.box_icosphere(seed=seed, subdivisions=1)
+        d = self.gen.desc
+        m = linrt.material
+        mats = [m(linrt.BSDF_DIFFUSE, diffuse=(0.62, 0.55, 0.45)),
+                m(linrt.BSDF_DIFFUSE, diffuse=(0.3, 0.6, 0.3)),
+                m(linrt.BSDF_PLASTIC, int_ior=1.49, ext_ior=1.000277, diffuse=(0.2, 0.5, 0.8))]
+        mats[2].fdr_int = linrt.fresnel_diffuse_reflectance(np.float32(1.0) / (np.float32(1.49) / np.float32(1.000277)))
+        self.materials = (linrt.Material * len(mats))(*mats)
+        room = linrt.ObjectDesc()
+        C_.pointer(room)[0] = d.objects[0]
+        room.material = 0
+        # the emissive mesh: icosphere (level 2) with seeded radial bumps
+        pos, idx = linrt.box_icosphere(seed=seed, subdivisions=2).mesh(1)
+        pos = pos.astype(np.float64)
+        rng = np.random.default_rng(seed)
+        rad = np.linalg.norm(pos, axis=1, keepdims=True)
+        unit = pos / rad
+        r = 0.16 * (1.0 + bumps * rng.uniform(-1.0, 1.0, (len(pos), 1)))
+        self.mesh_pos = np.ascontiguousarray((unit * r + np.array([0.35, 0.35, -0.1])).astype(np.float32))
+        self.mesh_idx = np.ascontiguousarray(idx.astype(np.uint32))
+        self.mesh_cn = np.ascontiguousarray(unit[self.mesh_idx.reshape(-1)].astype(np.float32).reshape(-1, 9))
+        mesh = linrt.ObjectDesc()
+        mesh.shape, mesh.material, mesh.emitter = linrt.SHAPE_MESH, 0, 0
+        mesh.n_vertices, mesh.n_triangles = len(self.mesh_pos), len(self.mesh_idx)
+        mesh.positions = self.mesh_pos.ctypes.data_as(C_.POINTER(C_.c_float))
+        mesh.indices = self.mesh_idx.ctypes.data_as(C_.POINTER(C_.c_uint32))
+        mesh.corner_normals = self.mesh_cn.ctypes.data_as(C_.POINTER(C_.c_float))
+        objs = [room, mesh]
+        for c, rr, mi, em in (((-0.4, 0.45, 0.2), 0.12, 0, 1), ((-0.3, -0.65, -0.2), 0.3, 1, -1),
+                              ((0.35, -0.7, 0.25), 0.28, 2, -1)):
+            o = linrt.ObjectDesc()
+            o.shape, o.material, o.emitter, o.radius = linrt.SHAPE_SPHERE, mi, em, rr
+            for i in range(3):
+                o.center[i] = c[i]
+            objs.append(o)
+        rect = linrt.ObjectDesc()
+        for i in range(d.n_objects):
+            if d.objects[i].shape == linrt.SHAPE_RECTANGLE:
+                C_.pointer(rect)[0] = d.objects[i]
+        rect.emitter = 2
+        objs.append(rect)
+        self.objects = (linrt.ObjectDesc * len(objs))(*objs)
+        lights = []
+        for obj, rgb in ((1, (6.0, 5.0, 4.0)), (2, (2.0, 3.0, 5.0)), (len(objs) - 1, (5.0, 5.0, 5.0))):
+            ld = linrt.LightDesc()
+            ld.kind, ld.object = linrt.LIGHT_AREA, obj
+            for i in range(3):
+                ld.radiance[i] = rgb[i]
+            lights.append(ld)
+        self.lights = (linrt.LightDesc * len(lights))(*lights)
+        self.desc = linrt.SceneDesc(len(objs), self.objects, len(mats), self.materials, len(lights), self.lights, 0,
+                                    None, d.camera)
+        self.desc_ptr = C_.pointer(self.desc)
