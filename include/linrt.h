@@ -217,6 +217,15 @@ int lrt_bsdf_eval_pdf(int hip_device, const lrt_material* materials, uint32_t n_
                       const uint8_t* front_face, uint32_t n,
                       int measure, int mode, int component, uint32_t type_mask,
                       float* f_cos, float* pdf);
+/* d (f*cos) / d theta of BSDF tuples, as the differentiable renderer's adjoint
+ * uses it (no counterpart in the reference): which 0 = GGX alpha, 1 = the
+ * interior IOR; exact (dual numbers) for the rough conductor and the rough
+ * dielectric, central differences (step 1e-3) for other types.  Solid-angle
+ * measure, all components, df: 3 floats per tuple. */
+int lrt_bsdf_param_grad(int hip_device, const lrt_material* materials, uint32_t n_materials,
+                        const float* tables, uint32_t n_tables, const int32_t* mat,
+                        const float* wi, const float* wo, const uint8_t* front_face, uint32_t n,
+                        int mode, int which, float* df);
 /* bsdf::sample for n tuples with sample u (2 floats) and lobe sample
  * (ctx.lobe_sample); outputs weight (3), wo (3), pdf, eta, sampled_type,
  * sampled_component. */
@@ -380,7 +389,9 @@ enum {
     LRT_GEN_BOX_ICOSPHERE = 0,   /* closed box room + icosphere L5 + ceiling light */
     LRT_GEN_DISPLACED_SPHERE = 1, /* displaced sphere, 7 material regions, ground, sky */
     LRT_GEN_PHOTON_BOX = 2,       /* box room + displaced sphere (7 regions, radius 0.5) + 0.5x0.5 light */
-    LRT_GEN_DISPLACED_SPHERE_VARIANT = 3 /* kind 1 with seeded material colours, roughness and glass index */
+    LRT_GEN_DISPLACED_SPHERE_VARIANT = 3, /* kind 1 with seeded material colours, roughness and glass index */
+    LRT_GEN_INVERSE_SPHERE = 4    /* displaced sphere in one diffuse material, rough copper ground, rough
+                                     glass ball, sky: the differentiable renderer's scene (no counterpart) */
 };
 typedef struct lrt_gen_params {
     int32_t kind;
@@ -469,7 +480,9 @@ int lrt_render_aov(lrt_scene* scene, const lrt_render_params* params, float* aov
 /* ---- differentiable rendering (no counterpart in the reference) ---------
  * The path tracer of lrt_render as one kernel thread per pixel (identical
  * radiance per sample), with parameters theta: an albedo texture on one
- * diffuse material (spherical mapping about the origin, nearest texel), the
+ * diffuse material (nearest texel; a point's direction from the origin
+ * projected along z onto the unit disk, points behind z = 0 mirrored, and the
+ * disk mapped onto the texture square by the inverse concentric map), the
  * GGX alpha of up to two materials and the interior IOR of one.  theta holds
  * tex_height x tex_width RGB texels (row-major), then alpha[0], alpha[1] and
  * the IOR (3 slots always; unused ones are ignored).  lrt_diff_adjoint

@@ -310,7 +310,7 @@ struct lrt_scene {
     float bounding_radius = 0.0f;
     float box_lo[3] = {0.0f, 0.0f, 0.0f}, box_hi[3] = {0.0f, 0.0f, 0.0f};  // scene bounds
     bool has_delta = false;
-    DevBuf diff_mats, diff_tex, diff_theta, diff_img, diff_dimg, diff_grad, diff_part, diff_paths, diff_shadow;
+    DevBuf diff_mats, diff_tex, diff_theta, diff_img, diff_dimg, diff_grad, diff_fix, diff_part, diff_paths, diff_shadow;
     std::vector<lrt_material> host_mats;
     int max_spec_components = 0;
     Timer timers[LRT_KERNEL_COUNT_];
@@ -920,6 +920,31 @@ int lrt_bsdf_eval_pdf(int hip_device, const lrt_material* materials, uint32_t n_
     });
 }
 
+int lrt_bsdf_param_grad(int hip_device, const lrt_material* materials, uint32_t n_materials, const float* tables,
+                        uint32_t n_tables, const int32_t* mat, const float* wi, const float* wo,
+                        const uint8_t* front_face, uint32_t n, int mode, int which, float* df) {
+    return guarded([&] {
+        require(materials && mat && wi && wo && front_face && df, "null pointer");
+        require(n_tables == 0 || tables, "tables is null");
+        require(which == 0 || which == 1, "which must be 0 (alpha) or 1 (interior IOR)");
+        check_materials(materials, n_materials, n_tables);
+        for (uint32_t i = 0; i < n; ++i) require(mat[i] >= 0 && (uint32_t)mat[i] < n_materials, "material index out of range");
+        if (n == 0) return;
+        BatchDev B(hip_device);
+        Mats M{B.put(materials, n_materials), B.put(tables, (size_t)n_tables * 101)};
+        const int32_t* dm = B.put(mat, n);
+        const float* dwi = B.put(wi, 3 * (size_t)n);
+        const float* dwo = B.put(wo, 3 * (size_t)n);
+        const uint8_t* dfr = B.put(front_face, n);
+        float* dd = B.put<float>(nullptr, 3 * (size_t)n);
+        hipLaunchKernelGGL(k_bsdf_param_grad, dim3((n + 255) / 256), dim3(256), 0, B.s, M, dm, dwi, dwo, dfr, n, mode,
+                           which, dd);
+        HIPCHECK(hipGetLastError());
+        B.get(df, dd, 3 * (size_t)n);
+        HIPCHECK(hipStreamSynchronize(B.s));
+    });
+}
+
 int lrt_bsdf_sample(int hip_device, const lrt_material* materials, uint32_t n_materials, const float* tables,
                     uint32_t n_tables, const int32_t* mat, const float* wi, const uint8_t* front_face, const float* u,
                     const float* lobe_u, uint32_t n, int mode, int component, uint32_t type_mask, float* weight, float* wo,
@@ -1307,10 +1332,10 @@ static void diff_launch_render(lrt_scene& sc, const Mats& M, const DiffParams& D
 
 // The wavefront forward (img != null: mean radiance per pixel, W x H x 3,
 // device) or adjoint (dimg != null: gradient of sum(dimg * image) added into
-// grad, device) of the differentiable renderer, over passes of up to
-// LRT_PASS_PATHS paths like lrt_render.
+// the fixed-point sums gfix at scale *gscale, device) of the differentiable
+// renderer, over passes of up to LRT_PASS_PATHS paths like lrt_render.
 static void diff_wave_run(lrt_scene& sc, const Mats& M, const DiffParams& D, const lrt_render_params& prm, float* img,
-                          const float* dimg, float* grad, uint32_t n_params) {
+                          const float* dimg, unsigned long long* gfix, const float* gscale, uint32_t n_params) {
     const uint32_t W = prm.width, H = prm.height, npix = W * H;
     const uint32_t slots = std::max<uint32_t>(1, std::min<uint32_t>(prm.spp, pass_paths() / npix));
     const uint32_t n = npix * slots;
@@ -1366,7 +1391,7 @@ static void diff_wave_run(lrt_scene& sc, const Mats& M, const DiffParams& D, con
     const uint32_t g_bins = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc.cus * 4);
     if (img) hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(sc, (uint64_t)npix * 3, 4)), dim3(kBlock), 0, sc.stream, img,
                                 (long long)npix * 3);
-    float* grad_s = adj ? grad + (n_params - 3) : nullptr;
+    unsigned long long* grad_s = adj ? gfix + (n_params - 3) : nullptr;
     const float inv_spp = 1.0f / (float)prm.spp;
     auto pass = [&](uint32_t smp, uint32_t np, bool replay) {
         hipLaunchKernelGGL(k_raygen, dim3((std::max(np, 1024u) + kBlock - 1) / kBlock), dim3(kBlock), 0, sc.stream, P, sc.cam, W, H, 0u, np,
@@ -1378,15 +1403,15 @@ static void diff_wave_run(lrt_scene& sc, const Mats& M, const DiffParams& D, con
             hipLaunchKernelGGL(k_shade_bins, dim3(g_bins), dim3(kBlock), 0, sc.stream, sc.S, Hb, counters);
             hipLaunchKernelGGL(k_shade_scan, dim3(1), dim3(1024), 0, sc.stream, counters);
             hipLaunchKernelGGL(k_shade_order, dim3(g_bins), dim3(kBlock), 0, sc.stream, sc.S, Hb, counters, order);
-            const DiffWave dw{depth, prm.max_depth, prm.rr_depth, smp, D.key, replay ? dimg : nullptr, inv_spp};
+            const DiffWave dw{depth, prm.max_depth, prm.rr_depth, smp, D.key, replay ? dimg : nullptr, inv_spp, gscale};
             if (replay) {
                 hipLaunchKernelGGL(k_diff_shade<true>, dim3(g_shade), dim3(kBlock), 0, sc.stream, sc.S, sc.Ls, M, D, P, q, qn,
-                                   counters, Hb, order, Sb, DS, DP, dw, grad, grad_s);
-                hipLaunchKernelGGL(k_diff_shadow, dim3(g_trace), dim3(kBlock), 0, sc.stream, sc.S, P, counters, Sb, DS, grad,
-                                   grad_s);
+                                   counters, Hb, order, Sb, DS, DP, dw, gfix, grad_s);
+                hipLaunchKernelGGL(k_diff_shadow, dim3(g_trace), dim3(kBlock), 0, sc.stream, sc.S, P, counters, Sb, DS, gfix,
+                                   grad_s, gscale);
             } else {
                 hipLaunchKernelGGL(k_diff_shade<false>, dim3(g_shade), dim3(kBlock), 0, sc.stream, sc.S, sc.Ls, M, D, P, q,
-                                   qn, counters, Hb, order, Sb, DS, DP, dw, grad, grad_s);
+                                   qn, counters, Hb, order, Sb, DS, DP, dw, gfix, grad_s);
                 hipLaunchKernelGGL(k_shadow, dim3(g_trace), dim3(kBlock), 0, sc.stream, sc.S, P, counters, Sb);
             }
             hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, sc.stream, counters, dstats);
@@ -1493,7 +1518,7 @@ int lrt_diff_render(lrt_scene* sc, const lrt_diff_params* dp, const float* theta
             img = sc->diff_img.as<float>();
         }
         if (prm->flags & LRT_DIFF_PER_SAMPLE) diff_launch_render(*sc, M, D, *prm, img);
-        else diff_wave_run(*sc, M, D, *prm, img, nullptr, nullptr, diff_count(*dp));
+        else diff_wave_run(*sc, M, D, *prm, img, nullptr, nullptr, nullptr, diff_count(*dp));
         if (!device_ptrs) HIPCHECK(hipMemcpyAsync(image, img, bytes, hipMemcpyDeviceToHost, sc->stream));
         HIPCHECK(hipStreamSynchronize(sc->stream));
     });
@@ -1550,16 +1575,24 @@ int lrt_diff_adjoint(lrt_scene* sc, const lrt_diff_params* dp, const float* thet
             sc->diff_grad.alloc((size_t)np * 4);
             g = sc->diff_grad.as<float>();
         }
-        hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(*sc, np, 4)), dim3(kBlock), 0, sc->stream, g, (long long)np);
+        // fixed-point sums (np words) and their scale
+        sc->diff_fix.alloc((size_t)np * 8 + 256);
+        unsigned long long* gfix = sc->diff_fix.as<unsigned long long>();
+        float* gscale = reinterpret_cast<float*>(gfix + np);
+        hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(*sc, 2 * (uint64_t)np, 4)), dim3(kBlock), 0, sc->stream,
+                           reinterpret_cast<float*>(gfix), 2 * (long long)np);
+        hipLaunchKernelGGL(k_grad_scale, dim3(1), dim3(1024), 0, sc->stream, dimg, prm->width * prm->height * 3,
+                           1.0f / (float)prm->spp, gscale);
         if (prm->flags & LRT_DIFF_PER_SAMPLE) {
             uint32_t chunk, nchunks;
             diff_chunks(*prm, chunk, nchunks);
             const uint64_t n = (uint64_t)prm->width * prm->height * nchunks;
             hipLaunchKernelGGL(k_diff_adjoint, dim3(grid_for(*sc, n, 16)), dim3(kBlock), 0, sc->stream, sc->S, sc->Ls, M, D,
-                               sc->cam, chunk, nchunks, dimg, g, g + (np - 3));
+                               sc->cam, chunk, nchunks, dimg, gfix, gfix + (np - 3), gscale);
         } else {
-            diff_wave_run(*sc, M, D, *prm, nullptr, dimg, g, np);
+            diff_wave_run(*sc, M, D, *prm, nullptr, dimg, gfix, gscale, np);
         }
+        hipLaunchKernelGGL(k_grad_out, dim3(grid_for(*sc, np, 4)), dim3(kBlock), 0, sc->stream, gfix, np, gscale, g);
         HIPCHECK(hipGetLastError());
         if (!device_ptrs) HIPCHECK(hipMemcpyAsync(grad, g, (size_t)np * 4, hipMemcpyDeviceToHost, sc->stream));
         HIPCHECK(hipStreamSynchronize(sc->stream));

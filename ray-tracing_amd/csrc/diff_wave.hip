@@ -45,14 +45,18 @@ struct DiffWave {
     uint2 key;
     const float* d_img;            // replay: gradient image (W x H x 3); null: forward
     float inv_spp;
+    const float* gscale;           // replay: fixed-point scale of the gradient sums
 };
 
-LRT_D void grad_reduce(const float (&s)[3], float* __restrict__ grad_s) {
+// the wave's sums of per-lane fixed-point values (lanes take paths in queue
+// order, which varies run to run: integer sums keep the result independent of
+// it), one atomic per wave and parameter; all lanes active
+LRT_D void grad_reduce(const long long (&q)[3], unsigned long long* __restrict__ grad_s) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        float v = s[k];
+        long long v = q[k];
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if ((threadIdx.x & 63) == 0 && v != 0.0f) atomicAdd(grad_s + k, v);
+        if ((threadIdx.x & 63) == 0 && v != 0) atomicAdd(grad_s + k, (unsigned long long)v);
     }
 }
 
@@ -64,11 +68,13 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_diff_shade(const DevS
                                                                        uint32_t* __restrict__ next_queue, uint32_t* counters,
                                                                        const HitBuf Hb, const uint32_t* __restrict__ order,
                                                                        ShadowBuf Sb, DiffShadow DS, DiffPaths DP, DiffWave prm,
-                                                                       float* __restrict__ grad_tex, float* __restrict__ grad_s) {
+                                                                       unsigned long long* __restrict__ grad_tex,
+                                                                       unsigned long long* __restrict__ grad_s) {
     const uint32_t n = counters[0];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t n_round = (n + stride - 1) / stride * stride;
-    float gs[3] = {0.0f, 0.0f, 0.0f};
+    const float gsc = ADJ ? *prm.gscale : 0.0f;
+    long long gs[3] = {0, 0, 0};
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_round; j += stride) {
         bool cont = false, shadow = false;
         uint32_t p = 0;
@@ -105,16 +111,16 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_diff_shade(const DevS
                         const float rho[3] = {DP.rr[p], DP.rg[p], DP.rb[p]};
                         const float rv[3] = {R.x, R.y, R.z}, dv[3] = {delta.x, delta.y, delta.z};
                         for (int c = 0; c < 3; ++c)
-                            if (rho[c] > 1e-20f && rv[c] != 0.0f && isfinite(dv[c] * rv[c] / rho[c]))
-                                atomicAdd(grad_tex + 3 * (size_t)pt + c, dv[c] * rv[c] / rho[c]);
+                            if (rho[c] > 1e-20f && rv[c] != 0.0f)
+                                grad_add(grad_tex + 3 * (size_t)pt + c, dv[c] * rv[c] / rho[c], gsc);
                     }
                     if (pf & 1) {
                         const float gv = dot(delta, R * f3{DP.far[p], DP.fag[p], DP.fab[p]});
-                        if (isfinite(gv)) gs[(pf >> 1) & 1] += gv;
+                        gs[(pf >> 1) & 1] += grad_quant(gv, gsc);
                     }
                     if (pf & 4) {
                         const float gv = dot(delta, R * f3{DP.fer[p], DP.feg[p], DP.feb[p]});
-                        if (isfinite(gv)) gs[2] += gv;
+                        gs[2] += grad_quant(gv, gsc);
                     }
                 }
                 if (pt >= 0 || pf != 0) {
@@ -278,13 +284,16 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_diff_shade(const DevS
 // replay shadow rays: unoccluded ones add their radiance and gradient terms
 __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_diff_shadow(const DevScene S, PathState P,
                                                                        uint32_t* __restrict__ counters, const ShadowBuf Sb,
-                                                                       const DiffShadow DS, float* __restrict__ grad_tex,
-                                                                       float* __restrict__ grad_s) {
+                                                                       const DiffShadow DS,
+                                                                       unsigned long long* __restrict__ grad_tex,
+                                                                       unsigned long long* __restrict__ grad_s,
+                                                                       const float* __restrict__ gscale) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;
     uint2 spill[kMaxStack - kLdsStack];
     const uint32_t n = counters[2];
-    float gs[3] = {0.0f, 0.0f, 0.0f};
+    const float gsc = *gscale;
+    long long gs[3] = {0, 0, 0};
     LaneTrace T;
     uint32_t i = 0;
     bool active = false, more = true;
@@ -308,10 +317,10 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_diff_shadow(const Dev
                 if (t >= 0) {
                     const float g3[3] = {DS.gr[i], DS.gg[i], DS.gb[i]};
                     for (int c = 0; c < 3; ++c)
-                        if (g3[c] != 0.0f) atomicAdd(grad_tex + 3 * (size_t)t + c, g3[c]);
+                        if (g3[c] != 0.0f) grad_add(grad_tex + 3 * (size_t)t + c, g3[c], gsc);
                 }
-                if (DS.slot[i] >= 0) gs[DS.slot[i]] += DS.ga[i];
-                gs[2] += DS.ge[i];
+                if (DS.slot[i] >= 0) gs[DS.slot[i]] += grad_quant(DS.ga[i], gsc);
+                gs[2] += grad_quant(DS.ge[i], gsc);
             }
             active = false;
         }

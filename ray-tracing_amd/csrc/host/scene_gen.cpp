@@ -395,6 +395,47 @@ Generated* generate(const lrt_gen_params& p, const float* tables, uint32_t n_tab
         std::memcpy(cam.up, up, 12);
         cam.fov_deg = 40.0f;
         cam.aspect = p.aspect > 0 ? p.aspect : 16.0f / 9.0f;
+    } else if (p.kind == LRT_GEN_INVERSE_SPHERE) {
+        // the differentiable renderer's scene: the displaced sphere in one
+        // diffuse material (0, the textured one), a rough copper ground (2)
+        // that mirrors its underside, and a rough glass ball (1) in front and
+        // to the right, under the Hosek sky
+        if (!sky) throw std::runtime_error("inverse-rendering scene needs a sky light");
+        const float cu_eta[3] = {0.200438f, 0.924033f, 1.102212f}, cu_k[3] = {3.912948f, 2.452847f, 2.142188f};
+        lrt_material m0 = blank();
+        m0.type = LRT_BSDF_DIFFUSE;
+        m0.diffuse[0] = m0.diffuse[1] = m0.diffuse[2] = 0.5f;
+        lrt_material m1 = blank();
+        m1.type = LRT_BSDF_ROUGH_DIELECTRIC; m1.int_ior = 1.5f; m1.ext_ior = 1.0f; m1.alpha = 0.2f;
+        lrt_material m2 = blank();
+        m2.type = LRT_BSDF_CONDUCTOR; m2.alpha = 0.1f;
+        std::memcpy(m2.cond_eta, cu_eta, 12); std::memcpy(m2.cond_k, cu_k, 12);
+        for (const auto& m : {m0, m1, m2}) g->materials.push_back(m);
+        float sites[1][3] = {{0.0f, 1.0f, 0.0f}};
+        g->meshes.resize(2);
+        displaced_sphere(g->meshes[0], p.segments ? p.segments : 512, p.rings ? p.rings : 256, p.seed, sites, 1);
+        g->meshes[0].tmat.clear();
+        Mesh& ground = g->meshes[1];
+        const float y = -1.06f, e = 20.0f, n[3] = {0.0f, 1.0f, 0.0f};
+        const float a[3] = {-e, y, -e}, b[3] = {-e, y, e}, c[3] = {e, y, e}, d[3] = {e, y, -e};
+        add_quad(ground, a, b, c, d, n);
+        g->objects.push_back(obj_for_mesh(g->meshes[0], 0));
+        g->objects.push_back(obj_for_mesh(g->meshes[1], 2));
+        lrt_object_desc ball;
+        std::memset(&ball, 0, sizeof(ball));
+        ball.shape = LRT_SHAPE_SPHERE;
+        ball.material = 1;
+        ball.emitter = -1;
+        ball.center[0] = 1.45f; ball.center[1] = -0.70f; ball.center[2] = 0.9f;
+        ball.radius = 0.35f;
+        g->objects.push_back(ball);
+        g->lights.push_back(*sky);
+        const float o[3] = {0.3f, 0.55f, 4.9f}, la[3] = {0.3f, -0.3f, 0.0f}, up[3] = {0.0f, 1.0f, 0.0f};
+        std::memcpy(cam.origin, o, 12);
+        std::memcpy(cam.look_at, la, 12);
+        std::memcpy(cam.up, up, 12);
+        cam.fov_deg = 40.0f;
+        cam.aspect = p.aspect > 0 ? p.aspect : 1.0f;
     } else if (p.kind == LRT_GEN_PHOTON_BOX) {
         // the box room around the displaced sphere (radius 0.5, 7 regions),
         // lit by a 0.5 x 0.5 ceiling light

@@ -40,7 +40,7 @@ MODE_RADIANCE, MODE_IMPORTANCE = 0, 1
  BSDF_ROUGH_PLASTIC, BSDF_COATING, BSDF_ROUGH_COATING) = range(9)
 SHAPE_MESH, SHAPE_RECTANGLE, SHAPE_SPHERE = 0, 1, 2
 LIGHT_AREA, LIGHT_SKY = 0, 1
-GEN_BOX_ICOSPHERE, GEN_DISPLACED_SPHERE, GEN_PHOTON_BOX, GEN_DISPLACED_SPHERE_VARIANT = 0, 1, 2, 3
+GEN_BOX_ICOSPHERE, GEN_DISPLACED_SPHERE, GEN_PHOTON_BOX, GEN_DISPLACED_SPHERE_VARIANT, GEN_INVERSE_SPHERE = range(5)
 PHOTON_SURFACE, PHOTON_CAUSTIC = 0, 1
 KERNEL_CLOSEST, KERNEL_OCCLUDED, KERNEL_SHADE, KERNEL_GATHER = 0, 1, 2, 3
 
@@ -160,7 +160,7 @@ EXPORTED = [
     "lrt_diff_param_count", "lrt_diff_render", "lrt_diff_adjoint", "lrt_diff_render_held",
     "lrt_load_obj", "lrt_mesh_view", "lrt_mesh_free", "lrt_generate_obj_scene", "lrt_image_write",
     "lrt_image_to_bytes", "lrt_render_capture_rays", "lrt_trace_closest_noalloc", "lrt_sky_cook",
-    "lrt_sky_elevation", "lrt_accum_write", "lrt_accum_read", "lrt_photon_map_tree",
+    "lrt_sky_elevation", "lrt_accum_write", "lrt_accum_read", "lrt_photon_map_tree", "lrt_bsdf_param_grad",
 ]
 
 
@@ -493,6 +493,13 @@ def photon_box(seed=1, segments=512, rings=256, aspect=1.0):
     return Generated(GEN_PHOTON_BOX, seed=seed, segments=segments, rings=rings, aspect=aspect, tables=tables)
 
 
+def inverse_sphere(seed=1, segments=512, rings=256, aspect=1.0):
+    """The differentiable renderer's scene: the displaced sphere (262,144
+    triangles by default) in one diffuse material (0, the one a texture covers),
+    a rough copper ground (2) and a rough glass ball (1), under the Hosek sky."""
+    return Generated(GEN_INVERSE_SPHERE, seed=seed, segments=segments, rings=rings, aspect=aspect, sky=load_sky())
+
+
 class Scene:
     """Scene uploaded to one GPU (lrt_scene_create)."""
 
@@ -821,6 +828,23 @@ def bsdf_eval_pdf(materials, tables, mat, wi, wo, front, measure=MEASURE_SOLID_A
                                  C.c_int(measure), C.c_int(mode), C.c_int(component), C.c_uint32(type_mask),
                                  _p(f), _p(pdf)))
     return f, pdf
+
+
+def bsdf_param_grad(materials, tables, mat, wi, wo, front, which, mode=MODE_RADIANCE, device=0):
+    """lrt_bsdf_param_grad: d (f*cos) / d alpha (which 0) or d interior IOR
+    (which 1) of BSDF tuples, [n, 3]."""
+    marr, nm = _mats_arg(materials)
+    tb = np.ascontiguousarray(tables if tables is not None else np.zeros((0, 101)), np.float32)
+    mat = np.ascontiguousarray(mat, np.int32)
+    wi = np.ascontiguousarray(wi, np.float32)
+    wo = np.ascontiguousarray(wo, np.float32)
+    front = np.ascontiguousarray(front, np.uint8)
+    n = mat.size
+    df = np.zeros((n, 3), np.float32)
+    _check(LIB.lrt_bsdf_param_grad(C.c_int(device), marr, C.c_uint32(nm), _p(tb), C.c_uint32(tb.size // 101),
+                                   _p(mat, C.c_int32), _p(wi), _p(wo), _p(front, C.c_uint8), C.c_uint32(n),
+                                   C.c_int(mode), C.c_int(which), _p(df)))
+    return df
 
 
 def bsdf_sample(materials, tables, mat, wi, front, u, lobe_u, mode=MODE_RADIANCE, component=-1,

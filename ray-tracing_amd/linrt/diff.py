@@ -1,8 +1,9 @@
-"""Differentiable rendering on the GPU (lrt_diff_*): a torch.autograd.Function
-whose forward is the library's path-tracing kernel and whose backward is its
-path-replay adjoint kernel, and the inverse-rendering workload built on it:
-recover a 64x64 RGB albedo texture on the displaced sphere, two GGX
-roughnesses and one IOR (12,291 scalars) from a target image, by Adam.
+"""Differentiable rendering on the GPU (lrt_diff_*): a C++
+torch::autograd::Function (csrc/torch/diff_function.cpp, module linrt_torch)
+whose forward is the library's path tracer and whose backward is its
+path-replay adjoint, and the inverse-rendering workload built on it: recover
+a 64x64 RGB albedo texture covering the displaced sphere, two GGX roughnesses
+and one IOR (12,291 scalars) from a target image, by Adam.
 
   python -m linrt.diff [--iters 200] [--res 512] [--spp 32]
 """
@@ -10,8 +11,12 @@ from __future__ import annotations
 
 import argparse
 import ctypes as C
+import importlib.machinery
+import importlib.util
+import shutil
 import sys
 import time
+from pathlib import Path
 
 import numpy as np
 
@@ -19,6 +24,38 @@ import linrt
 
 TEX = 64
 DIFF_PER_SAMPLE = 4  # lrt_render_params.flags: LRT_DIFF_PER_SAMPLE
+EXT_NAME = "linrt_torch"
+EXT_DIR = Path(__file__).resolve().parent / "_build"
+EXT_SRC = linrt.PKG_ROOT / "csrc" / "torch" / "diff_function.cpp"
+INCLUDE = linrt.PKG_ROOT.parent / "include"
+_ext = None
+
+
+def build_torch_ext(verbose=False):
+    """Compiles the C++ autograd Function module into EXT_DIR (always anew)."""
+    from torch.utils import cpp_extension
+    shutil.rmtree(EXT_DIR, ignore_errors=True)
+    EXT_DIR.mkdir(parents=True)
+    cpp_extension.load(name=EXT_NAME, sources=[str(EXT_SRC)], extra_include_paths=[str(INCLUDE)],
+                       extra_cflags=["-O2"], build_directory=str(EXT_DIR), verbose=verbose)
+
+
+def torch_ext():
+    """The built linrt_torch module, bound to the loaded liblinrt_hip.so; raises
+    when it is not built."""
+    global _ext
+    if _ext is None:
+        import torch  # noqa: F401  (the module links against torch's libraries)
+        path = EXT_DIR / f"{EXT_NAME}.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} is not built: run linrt.diff.build_torch_ext()")
+        loader = importlib.machinery.ExtensionFileLoader(EXT_NAME, str(path))
+        spec = importlib.util.spec_from_file_location(EXT_NAME, str(path), loader=loader)
+        mod = importlib.util.module_from_spec(spec)
+        loader.exec_module(mod)
+        mod.bind(str(linrt.LIB_PATH))
+        _ext = mod
+    return _ext
 
 
 class DiffParams(C.Structure):
@@ -27,8 +64,9 @@ class DiffParams(C.Structure):
 
 
 def params(tex_material=0, tex=TEX, alpha_materials=(1, 2), ior_material=1):
-    """Displaced-sphere regions: 0 diffuse (textured), 1 rough dielectric
-    (alpha and IOR), 2 rough conductor (alpha)."""
+    """Materials of the inverse-rendering scene (linrt.inverse_sphere): 0 the
+    sphere's diffuse (textured), 1 the rough glass ball (alpha and IOR), 2 the
+    rough copper ground (alpha)."""
     a = (C.c_int32 * 2)(*alpha_materials)
     return DiffParams(tex_material, tex, tex, a, ior_material)
 
@@ -90,28 +128,21 @@ class Renderer:
         return g
 
 
-def render_fn():
-    """torch.autograd.Function: image = render(theta); backward = adjoint."""
-    import torch
-
-    class Render(torch.autograd.Function):
-        @staticmethod
-        def forward(ctx, theta, renderer):
-            ctx.renderer = renderer
-            ctx.save_for_backward(theta)
-            return renderer.render(theta)
-
-        @staticmethod
-        def backward(ctx, grad_image):
-            (theta,) = ctx.saved_tensors
-            return ctx.renderer.adjoint(theta, grad_image, decorrelate=True), None
-
-    return Render
+def render(theta, R):
+    """R's image of theta, differentiable in theta: the C++ autograd Function
+    (forward lrt_diff_render, backward lrt_diff_adjoint replaying other samples
+    than the forward's)."""
+    dp = [R.dp.tex_material, R.dp.tex_width, R.dp.tex_height, R.dp.alpha_material[0], R.dp.alpha_material[1],
+          R.dp.ior_material]
+    p = R.prm
+    prm = [p.width, p.height, p.spp, p.spp_offset, p.max_depth, p.rr_depth, p.seed, p.row_begin, p.row_end, p.flags]
+    return torch_ext().render(theta, R.scene._h.value, dp, prm)
 
 
 def scene(res=512, segments=512, rings=256):
-    """The displaced sphere (262,144 tris by default) under the Hosek sky, square camera."""
-    gen = linrt.displaced_sphere(seed=1, segments=segments, rings=rings, aspect=1.0)
+    """The inverse-rendering scene (displaced sphere of 262,144 tris by
+    default, copper ground, glass ball, Hosek sky), square camera."""
+    gen = linrt.inverse_sphere(seed=1, segments=segments, rings=rings, aspect=1.0)
     return gen, linrt.Scene(gen.desc_ptr)
 
 
@@ -148,13 +179,12 @@ def optimize(iters=200, res=512, spp=32, max_depth=8, lr=0.02, seed=7, log=print
                                                            rng.uniform(1.2, 1.8)]]).astype(np.float32)
     theta = torch.from_numpy(theta0).cuda().requires_grad_()
     opt = torch.optim.Adam([theta], lr=lr)
-    Render = render_fn()
     hist = []
     t0 = time.perf_counter()
     for it in range(iters):
         R.prm.seed = 1000 + it  # fresh samples each iteration
         opt.zero_grad()
-        img = Render.apply(theta, R)
+        img = render(theta, R)
         loss = ((img - target) ** 2).mean()
         loss.backward()
         bad = int((~torch.isfinite(theta.grad)).sum())

@@ -187,3 +187,16 @@ def test_mesh_and_sphere_lights_oracle_is_the_reference():
     po, do = OL.oracle.scene(er.desc_ptr).trace_particles(0, 800, 8, 5, mode=1)
     pr, dr = OL.reference.scene(er.desc_ptr).trace_particles(0, 800, 8, 5)
     assert len(po) > 200 and np.array_equal(po, pr) and np.array_equal(do, dr)
+
+
+def test_torch_function_module_loads():
+    """The C++ autograd Function module (linrt_torch) loads, binds the
+    library's differentiable-rendering entry points and refuses a CPU tensor
+    before any GPU call."""
+    import torch
+    from linrt import diff
+    ext = diff.torch_ext()
+    with pytest.raises(RuntimeError, match="GPU tensor"):
+        ext.render(torch.zeros(12291), 1, [0, 64, 64, 1, 2, 1], [8, 8, 1, 0, 8, 1 << 20, 1, 0, 0, 0])
+    with pytest.raises(RuntimeError, match="6 integers"):
+        ext.render(torch.zeros(12291), 1, [0, 64], [8, 8, 1, 0, 8, 1 << 20, 1, 0, 0, 0])
