@@ -44,7 +44,7 @@ def cpu_baseline(gen, budget_s=15.0):
     import oracle_lib  # noqa: E402
     if oracle_lib.oracle is None:
         return None
-    threads = os.cpu_count() or 1
+    threads = len(os.sched_getaffinity(0)) or 1
     t0 = time.perf_counter()
     sc = oracle_lib.oracle.scene(gen.desc_ptr)
     build_s = time.perf_counter() - t0
@@ -57,10 +57,27 @@ def cpu_baseline(gen, budget_s=15.0):
     r = sc.render(W, H, SPP, MAX_DEPTH, RR_DEPTH, seed=SEED, threads=threads, rows=(mid, mid + rows))
     dt = time.perf_counter() - t0
     samples = W * rows * SPP
-    return {"value": samples / dt / 1e6, "unit": "Msamples/s", "cores": threads, "kind": "port",
-            "sample": f"rows {mid}..{mid + rows - 1} of the 1920x1080 frame at 64 spp ({samples} samples, "
-                      f"{dt:.1f} s render loop; kd build {build_s:.1f} s not timed)",
-            "extension_rays": int(r["ext"]), "shadow_rays": int(r["shadow"])}
+    value = samples / dt / 1e6
+    model = "unknown"
+    try:
+        model = next(ln.split(":", 1)[1].strip() for ln in Path("/proc/cpuinfo").read_text().splitlines()
+                     if ln.startswith("model name"))
+    except (OSError, StopIteration):
+        pass
+    out = {"value": value, "unit": "Msamples/s", "cores": threads, "kind": "port",
+           "sample": f"rows {mid}..{mid + rows - 1} of the 1920x1080 frame at 64 spp ({samples} samples, "
+                     f"{dt:.1f} s render loop; kd build {build_s:.1f} s not timed)",
+           "cpu_model": model, "logical_cpus": os.cpu_count(), "threads": threads,
+           "extension_rays": int(r["ext"]), "shadow_rays": int(r["shadow"])}
+    # the restatement's speed relative to the reference's own code, measured
+    # on one host with the same scenes and sample streams (oracle/calibrate_cpu.py)
+    cal = REPO / "oracle" / "cpu_calibration.json"
+    if cal.exists():
+        c = json.loads(cal.read_text())
+        out["calibration"] = {"ratio_port_over_reference": c["ratio"], "measured_on": c["host"],
+                              "cases": [{k: v[k] for k in ("workload", "ratio")} for v in c["cases"]],
+                              "reference_equivalent_value": round(value / c["ratio"], 4)}
+    return out
 
 
 def ray_bench(args, torch, linrt):
@@ -191,6 +208,10 @@ def main():
     ap.add_argument("--no-denoise", action="store_true", help="skip the time-to-relMSE leg")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only copy this many MiB device-to-device 5 times (HBM counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--gather-calibration", type=int, default=0,
+                    help="only gather this many MiB of rows at random indices 5 times (counter calibration for "
+                         "scattered reads, row size --gather-row bytes)")
+    ap.add_argument("--gather-row", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the path shards by pixel/sample ranges "
@@ -209,6 +230,19 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_mib": args.copy_calibration, "launches": 5, "bytes_read_per_launch": n * 4,
                           "bytes_written_per_launch": n * 4}))
+        return
+    if args.gather_calibration:
+        row = args.gather_row // 4
+        n = args.gather_calibration * 2**20 // args.gather_row
+        src = torch.ones((n, row), dtype=torch.float32, device="cuda:0")
+        idx = torch.randperm(n, generator=torch.Generator().manual_seed(1)).to("cuda:0")
+        out = torch.empty_like(src)
+        for _ in range(5):
+            torch.index_select(src, 0, idx, out=out)
+        torch.cuda.synchronize()
+        print(json.dumps({"gather_mib": args.gather_calibration, "row_bytes": args.gather_row, "rows": n,
+                          "launches": 5, "bytes_read_per_launch": n * args.gather_row + n * 8,
+                          "bytes_written_per_launch": n * args.gather_row}))
         return
     if args.rays:
         return ray_bench(args, torch, linrt)
@@ -264,19 +298,33 @@ def main():
     # roofline of the dominant kernels, the traversal launches (k_extend for
     # camera rays, then per bounce k_trace_queues over the shadow rays and the
     # next extension rays): B_ray = 32 (ray) + 8 n_node + 4 n_ref + 36 n_tri
-    # + 16 (result) bytes, n_* from the counting pass (the reference's
-    # closest-hit traversal, also for shadow rays)
-    per_frame_bytes = (32 + 16) * (st_c.extension_rays + st_c.shadow_rays) + \
+    # + 16 (result) bytes, n_* from the counting pass.  `frac` prices the
+    # reference's traversal work (full kd window, closest-hit shadow rays);
+    # `own_work` prices what the timed kernels do (windows clipped to the mesh
+    # box, shadow rays ended by their first hit), counted by the same rules,
+    # with B = 32 + 8 n_node + 36 n_tri + 16: the kernels read each leaf's
+    # triangle records in reference order and no reference index per test.
+    rays = st_c.extension_rays + st_c.shadow_rays
+    per_frame_bytes = (32 + 16) * rays + \
         8 * (st_c.nodes_visited + st_c.shadow_nodes_visited) + 4 * (st_c.refs_read + st_c.shadow_refs_read) + \
         36 * (st_c.tris_tested + st_c.shadow_tris_tested)
+    own_frame_bytes = (32 + 16) * rays + \
+        8 * (st_c.own_nodes_visited + st_c.own_shadow_nodes_visited) + \
+        36 * (st_c.own_tris_tested + st_c.own_shadow_tris_tested)
     launches_per_frame = n_c / max(args.steps, 1)
     bytes_per_launch = per_frame_bytes / launches_per_frame
+    own_bytes_per_launch = own_frame_bytes / launches_per_frame
     avg_ms = ms_c / max(n_c, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    traffic = None
+    own_achieved = own_bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic, traffic_src = None, None
     tf = REPO / "profiles" / "traffic_trace_displaced_sphere.json"
     if tf.exists():
-        traffic = json.loads(tf.read_text()).get("hbm_bytes_per_launch")
+        tj = json.loads(tf.read_text())
+        traffic = tj.get("hbm_bytes_per_launch")
+        traffic_src = (f"{tf.relative_to(REPO)}: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of this workload, measured "
+                       f"apart on build {tj.get('build', '?')} and not re-measured by this run (L2-to-fabric "
+                       f"bytes; the 256 MB Infinity Cache sits between them and HBM)")
 
     cpu = None if args.no_cpu else cpu_baseline(gen)
     ttq = None
@@ -311,8 +359,22 @@ def main():
                      "kernel": "k_extend + k_trace_queues (kd traversal: extension rays closest hit, shadow rays any hit)",
                      "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(avg_ms, 4),
-                     "launches": int(n_c)},
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(avg_ms, 4),
+                     "launches": int(n_c),
+                     "frac_basis": "the reference's traversal work: full kd window, closest-hit shadow rays",
+                     "own_work": {"what": "the timed kernels' own traversal: box-clipped windows, any-hit shadow rays, "
+                                          "leaf triangles read in reference order",
+                                  "bytes_per_launch": int(own_bytes_per_launch), "achieved": round(own_achieved, 1),
+                                  "frac": round(own_achieved / HBM_PEAK_GBS, 4),
+                                  "per_ray": {"nodes": round((st_c.own_nodes_visited + st_c.own_shadow_nodes_visited)
+                                                             / max(rays, 1), 2),
+                                              "tris": round((st_c.own_tris_tested + st_c.own_shadow_tris_tested)
+                                                            / max(rays, 1), 2)}},
+                     "reference_work_per_ray": {"nodes": round((st_c.nodes_visited + st_c.shadow_nodes_visited)
+                                                               / max(rays, 1), 2),
+                                                "tris": round((st_c.tris_tested + st_c.shadow_tris_tested)
+                                                              / max(rays, 1), 2)}},
         "cpu_baseline": cpu,
         "time_to_relmse": ttq,
     }

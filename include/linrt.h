@@ -301,6 +301,13 @@ typedef struct lrt_render_stats {
        reference's closest-hit shadow query over [0.001, tmax]
        (path_integrator.h:113-116); lrt_render only */
     uint64_t shadow_nodes_visited, shadow_refs_read, shadow_tris_tested;
+    /* with LRT_RENDER_COUNT, lrt_render only: the work the timed traversal
+       kernels do themselves, counted by the same rules: extension rays
+       traversed over the window clipped to each mesh's box, shadow rays clipped
+       too and ended by their first blocking hit (closest-hit form toward a
+       non-convex emitter) */
+    uint64_t own_nodes_visited, own_refs_read, own_tris_tested;
+    uint64_t own_shadow_nodes_visited, own_shadow_refs_read, own_shadow_tris_tested;
 } lrt_render_stats;
 
 /* Renders and ADDS the per-pixel sum of sample radiance into sum_rgb

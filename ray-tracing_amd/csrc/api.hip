@@ -291,7 +291,7 @@ struct lrt_scene {
     DevLights Ls;
     Mats M;
     CameraDev cam;
-    DevBuf nodes, refs, tris, cnormals, trimat, mats, tables, obj_box;
+    DevBuf nodes, refs, tris, leaf_tris, cnormals, trimat, mats, tables, obj_box;
     DevBuf light_cdf, light_tris;  // mesh lights: running triangle areas, triangle vertices
     bool mesh_lights = false;      // some area light is a mesh: shading kernels with its pdf ray cast
     lrt_scene_info info;
@@ -457,6 +457,14 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
     upload(sc.nodes, words.data(), words.size() * 4, sc.stream);
     upload(sc.refs, refs.data(), refs.size() * 4, sc.stream);
     upload(sc.tris, tris.data(), tris.size() * 4, sc.stream);
+    {
+        // the triangle records again in leaf-reference order
+        std::vector<float> leaf(refs.size() * 9);
+        for (size_t i = 0; i < refs.size(); ++i)
+            std::memcpy(&leaf[9 * i], &tris[9 * (size_t)refs[i]], 9 * sizeof(float));
+        upload(sc.leaf_tris, leaf.data(), leaf.size() * 4, sc.stream);
+        HIPCHECK(hipStreamSynchronize(sc.stream));
+    }
     upload(sc.cnormals, cn.data(), cn.size() * 4, sc.stream);
     upload(sc.trimat, tmat.data(), tmat.size() * 2, sc.stream);
     std::vector<lrt_material> mats(d.materials, d.materials + d.n_materials);
@@ -485,6 +493,7 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
     sc.S.nodes = sc.nodes.as<uint2>();
     sc.S.refs = sc.refs.as<uint32_t>();
     sc.S.tris = sc.tris.as<float>();
+    sc.S.leaf_tris = sc.leaf_tris.as<float>();
     sc.S.cnormals = sc.cnormals.as<float>();
     sc.S.tri_mat = sc.trimat.as<uint16_t>();
     sc.S.n_objects = (int32_t)d.n_objects;
@@ -592,7 +601,7 @@ void build_scene(lrt_scene& sc, const lrt_scene_desc& d) {
     sc.info.n_triangles = tris.size() / 9;
     sc.info.n_nodes = n_nodes;
     sc.info.n_refs = refs.size();
-    sc.info.device_bytes = sc.nodes.bytes + sc.refs.bytes + sc.tris.bytes + sc.cnormals.bytes + sc.trimat.bytes;
+    sc.info.device_bytes = sc.nodes.bytes + sc.refs.bytes + sc.tris.bytes + sc.leaf_tris.bytes + sc.cnormals.bytes + sc.trimat.bytes;
     sc.info.max_tree_depth = max_depth;
     sc.info.build_seconds = std::chrono::duration<double>(t1 - t0).count();
     sc.info.upload_seconds = std::chrono::duration<double>(t2 - t1).count();
@@ -1132,8 +1141,9 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         HIPCHECK(hipMemcpyAsync(dstats, zero, sizeof(zero), hipMemcpyHostToDevice, sc->stream));
         const bool counting = (prm->flags & LRT_RENDER_COUNT) != 0;
         sc->totals.alloc(256);
-        const unsigned long long zero8[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // extension, shadow work
-        HIPCHECK(hipMemcpyAsync(sc->totals.p, zero8, sizeof(zero8), hipMemcpyHostToDevice, sc->stream));
+        // traversal work: reference extension, reference shadow, own extension, own shadow
+        const unsigned long long zero12[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        HIPCHECK(hipMemcpyAsync(sc->totals.p, zero12, sizeof(zero12), hipMemcpyHostToDevice, sc->stream));
         const uint2 key = make_uint2((uint32_t)prm->seed, (uint32_t)(prm->seed >> 32));
         const uint32_t* sobol = nullptr;
         if (prm->flags & LRT_RENDER_REFERENCE_SAMPLER) {
@@ -1200,7 +1210,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
             hipLaunchKernelGGL(k_accumulate, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, P, ns, dsum, dsq);
             HIPCHECK(hipGetLastError());
         }
-        unsigned long long hst[4] = {0, 0, 0, 0}, work[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        unsigned long long hst[4] = {0, 0, 0, 0}, work[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         HIPCHECK(hipMemcpyAsync(hst, dstats, sizeof(hst), hipMemcpyDeviceToHost, sc->stream));
         HIPCHECK(hipMemcpyAsync(work, sc->totals.p, sizeof(work), hipMemcpyDeviceToHost, sc->stream));
         if (!device_ptrs) {
@@ -1233,6 +1243,12 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
             stats->shadow_nodes_visited = work[3];
             stats->shadow_refs_read = work[4];
             stats->shadow_tris_tested = work[5];
+            stats->own_nodes_visited = work[6];
+            stats->own_refs_read = work[7];
+            stats->own_tris_tested = work[8];
+            stats->own_shadow_nodes_visited = work[9];
+            stats->own_shadow_refs_read = work[10];
+            stats->own_shadow_tris_tested = work[11];
         }
     }
 }

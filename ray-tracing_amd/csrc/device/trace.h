@@ -30,6 +30,8 @@ struct DevScene {
     const uint2* nodes;        // flattened kd nodes of every mesh object
     const uint32_t* refs;      // global triangle index per leaf reference
     const float* tris;         // 9 floats per triangle: p0, p1-p0, p2-p0 (degenerate: edges zeroed)
+    const float* leaf_tris;    // the same 9 floats per leaf reference, in reference order: a leaf's
+                               // triangles lie together, so its tests read a few whole cache lines
     const float* cnormals;     // 9 floats per triangle: corner normals
     const uint16_t* tri_mat;   // per triangle material (or null)
     const float* obj_box;      // per object 6 floats: enlarged AABB of a mesh (min xyz, max xyz)
@@ -270,14 +272,10 @@ LRT_D bool kd_traverse(const DevScene& S, const DevObject& ob, const Ray& r, flo
         for (uint32_t k = 0; k < count; k += kLeafBatch) {
             // issue the reference and vertex loads of a batch together,
             // then test in reference order (ties go to the later triangle)
-            uint32_t tri[kLeafBatch];
             TriRec rec[kLeafBatch];
 #pragma unroll
             for (int j = 0; j < kLeafBatch; ++j)
-                if (k + j < count) tri[j] = S.refs[first + k + j];
-#pragma unroll
-            for (int j = 0; j < kLeafBatch; ++j)
-                if (k + j < count) rec[j] = load_tri(S.tris + 9 * (size_t)tri[j]);
+                if (k + j < count) rec[j] = load_tri(S.leaf_tris + 9 * (size_t)(first + k + j));
 #pragma unroll
             for (int j = 0; j < kLeafBatch; ++j) {
                 if (k + j < count) {
@@ -285,7 +283,7 @@ LRT_D bool kd_traverse(const DevScene& S, const DevObject& ob, const Ray& r, flo
                     float t, u, v;
                     if (tri_test(rec[j], r, mint, limit, t, u, v)) {
                         closest = t;
-                        prim = (int32_t)(tri[j] - ob.tri_base);
+                        prim = (int32_t)(S.refs[first + k + j] - ob.tri_base);
                         hu = u;
                         hv = v;
                         found = true;
@@ -337,7 +335,9 @@ LRT_D void mesh_hit_normal(const DevScene& S, const DevObject& ob, int prim, flo
 
 // hittable_list::hit over the object list (hittable_list.h:19-34); a later
 // object wins an equal t.  `lds` points at this lane's first stack slot.
-template <bool COUNT>
+// FULL: traverse each mesh over the reference's full window instead of the
+// window clipped to the mesh's box (the counting pass counts both).
+template <bool COUNT, bool FULL = COUNT>
 LRT_D bool trace_closest(const DevScene& S, const Ray& r, float tmin, float tmax, HitInfo& h, uint2* lds,
                          Counters& cnt) {
     float inv[3] = {1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
@@ -348,9 +348,8 @@ LRT_D bool trace_closest(const DevScene& S, const Ray& r, float tmin, float tmax
     for (int o = 0; o < S.n_objects; ++o) {
         const DevObject& ob = S.obj[o];
         if (ob.shape == kShapeMesh) {
-            // the counting pass traverses the reference's full window
             float t0 = tmin, c = closest, u = 0.0f, v = 0.0f;
-            if (!COUNT && !mesh_box_clip(S, o, r, inv, t0, c)) continue;
+            if (!FULL && !mesh_box_clip(S, o, r, inv, t0, c)) continue;
             int32_t p = -1;
             if (kd_traverse<false, COUNT>(S, ob, r, inv, t0, c, p, u, v, lds, cnt)) {
                 closest = c; h.obj = o; h.prim = p; h.u = u; h.v = v; any = true;
@@ -376,12 +375,15 @@ LRT_D bool trace_closest(const DevScene& S, const Ray& r, float tmin, float tmax
 // ignored emitter is convex (rectangle, sphere): its own surface cannot be
 // nearer than the sampled point on it; a mesh emitter takes the reference's
 // closest-hit form.
-LRT_D bool trace_occluded(const DevScene& S, const Ray& r, float tmin, float tmax, int ignore, uint2* lds) {
+// COUNT: also count the traversal work into cnt (box-clipped windows, the
+// first hit ends the query: the work the persistent kernels do).
+template <bool COUNT>
+LRT_D bool trace_occluded(const DevScene& S, const Ray& r, float tmin, float tmax, int ignore, uint2* lds,
+                          Counters& cnt) {
     float inv[3] = {1.0f / r.d.x, 1.0f / r.d.y, 1.0f / r.d.z};
-    Counters cnt{0, 0, 0};
     if (emitter_nonconvex(S, ignore)) {  // the reference's closest-hit form
         HitInfo h;
-        return trace_closest<false>(S, r, tmin, tmax, h, lds, cnt) && S.obj[h.obj].emitter != ignore;
+        return trace_closest<COUNT, false>(S, r, tmin, tmax, h, lds, cnt) && S.obj[h.obj].emitter != ignore;
     }
     for (int o = 0; o < S.n_objects; ++o) {
         const DevObject& ob = S.obj[o];
@@ -390,7 +392,7 @@ LRT_D bool trace_occluded(const DevScene& S, const Ray& r, float tmin, float tma
             float t0 = tmin, c = tmax, u, v;
             if (!mesh_box_clip(S, o, r, inv, t0, c)) continue;
             int32_t p;
-            if (kd_traverse<true, false>(S, ob, r, inv, t0, c, p, u, v, lds, cnt)) return true;
+            if (kd_traverse<true, COUNT>(S, ob, r, inv, t0, c, p, u, v, lds, cnt)) return true;
         } else if (ob.shape == kShapeRect) {
             float t, a, b;
             if (rect_test(ob, r, tmin, tmax, t, a, b)) return true;
@@ -400,6 +402,11 @@ LRT_D bool trace_occluded(const DevScene& S, const Ray& r, float tmin, float tma
         }
     }
     return false;
+}
+
+LRT_D bool trace_occluded(const DevScene& S, const Ray& r, float tmin, float tmax, int ignore, uint2* lds) {
+    Counters cnt{0, 0, 0};
+    return trace_occluded<false>(S, r, tmin, tmax, ignore, lds, cnt);
 }
 
 // ---- resumable traversal for persistent kernels ----------------------------
@@ -545,14 +552,10 @@ LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* 
         const uint32_t firstref = nd.x;
         float limit = fminf(maxt, T.closest);
         for (uint32_t k = 0; k < count; k += kLeafBatch) {
-            uint32_t tri[kLeafBatch];
             TriRec rec[kLeafBatch];
 #pragma unroll
             for (int j = 0; j < kLeafBatch; ++j)
-                if (k + j < count) tri[j] = S.refs[firstref + k + j];
-#pragma unroll
-            for (int j = 0; j < kLeafBatch; ++j)
-                if (k + j < count) rec[j] = load_tri(S.tris + 9 * (size_t)tri[j]);
+                if (k + j < count) rec[j] = load_tri(S.leaf_tris + 9 * (size_t)(firstref + k + j));
 #pragma unroll
             for (int j = 0; j < kLeafBatch; ++j) {
                 if (k + j < count) {
@@ -562,7 +565,7 @@ LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* 
                         if (any) return true;
                         T.closest = t;
                         T.set_obj(o);
-                        T.prim = (int32_t)(tri[j] - ob.tri_base);
+                        T.prim = (int32_t)(S.refs[firstref + k + j] - ob.tri_base);
                         T.u = u;
                         T.v = v;
                     }

@@ -245,45 +245,57 @@ __global__ __launch_bounds__(kBlock) LRT_PATH_TRACE_ATTR void k_extend(const Dev
     }
 }
 
-// same rays with algorithmic-work counters (node visits, references, tests)
-// into totals[0..2]; *count is the queue's length
+// same rays with algorithmic-work counters (node visits, references, tests):
+// the reference's full-window traversal into totals[0..2], the product's own
+// box-clipped traversal into totals[6..8]; *count is the queue's length
 __global__ __launch_bounds__(kBlock) void k_extend_count(const DevScene S, const PathState P, const uint32_t* __restrict__ queue,
                                                          const uint32_t* __restrict__ count, unsigned long long* totals) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;
     const uint32_t n = *count;
-    uint64_t a = 0, b = 0, c3 = 0;
+    uint64_t a = 0, b = 0, c3 = 0, oa = 0, ob = 0, oc = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float4* w = path_words(P, queue[i]);
         const Ray r{xyz(w[0]), xyz(w[1])};
         HitInfo h;
-        Counters c{0, 0, 0};
-        trace_closest<true>(S, r, 0.001f, kInfF, h, lds, c);
+        Counters c{0, 0, 0}, own{0, 0, 0};
+        trace_closest<true, true>(S, r, 0.001f, kInfF, h, lds, c);
+        trace_closest<true, false>(S, r, 0.001f, kInfF, h, lds, own);
         a += c.nodes; b += c.refs; c3 += c.tris;
+        oa += own.nodes; ob += own.refs; oc += own.tris;
     }
     atomicAdd(&totals[0], (unsigned long long)a);
     atomicAdd(&totals[1], (unsigned long long)b);
     atomicAdd(&totals[2], (unsigned long long)c3);
+    atomicAdd(&totals[6], (unsigned long long)oa);
+    atomicAdd(&totals[7], (unsigned long long)ob);
+    atomicAdd(&totals[8], (unsigned long long)oc);
 }
 
 // the shadow queue's algorithmic work into totals[3..5]: the reference's
-// shadow test is a closest-hit query over [0.001, tmax] (path_integrator.h:113-116)
+// shadow test is a closest-hit query over [0.001, tmax] (path_integrator.h:113-116);
+// the product's own work (box-clipped, ended by the first hit) into totals[9..11]
 __global__ __launch_bounds__(kBlock) void k_shadow_count(const DevScene S, const ShadowBuf Sb,
                                                          const uint32_t* __restrict__ counters, unsigned long long* totals) {
     __shared__ uint2 stack[kLdsStack * kBlock];
     uint2* lds = stack + threadIdx.x;
     const uint32_t n = counters[2];
-    uint64_t a = 0, b = 0, c3 = 0;
+    uint64_t a = 0, b = 0, c3 = 0, oa = 0, ob = 0, oc = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const Ray r{f3{Sb.ox[i], Sb.oy[i], Sb.oz[i]}, f3{Sb.dx[i], Sb.dy[i], Sb.dz[i]}};
         HitInfo h;
-        Counters c{0, 0, 0};
-        trace_closest<true>(S, r, 0.001f, Sb.tmax[i], h, lds, c);
+        Counters c{0, 0, 0}, own{0, 0, 0};
+        trace_closest<true, true>(S, r, 0.001f, Sb.tmax[i], h, lds, c);
+        trace_occluded<true>(S, r, 0.001f, Sb.tmax[i], Sb.ignore[i], lds, own);
         a += c.nodes; b += c.refs; c3 += c.tris;
+        oa += own.nodes; ob += own.refs; oc += own.tris;
     }
     atomicAdd(&totals[3], (unsigned long long)a);
     atomicAdd(&totals[4], (unsigned long long)b);
     atomicAdd(&totals[5], (unsigned long long)c3);
+    atomicAdd(&totals[9], (unsigned long long)oa);
+    atomicAdd(&totals[10], (unsigned long long)ob);
+    atomicAdd(&totals[11], (unsigned long long)oc);
 }
 
 struct SurfaceHit {

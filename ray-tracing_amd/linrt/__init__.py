@@ -102,7 +102,10 @@ class RenderStats(C.Structure):
                 ("trace_launches", C.c_uint64), ("occluded_launches", C.c_uint64),
                 ("nodes_visited", C.c_uint64), ("refs_read", C.c_uint64), ("tris_tested", C.c_uint64),
                 ("shadow_nodes_visited", C.c_uint64), ("shadow_refs_read", C.c_uint64),
-                ("shadow_tris_tested", C.c_uint64)]
+                ("shadow_tris_tested", C.c_uint64),
+                ("own_nodes_visited", C.c_uint64), ("own_refs_read", C.c_uint64), ("own_tris_tested", C.c_uint64),
+                ("own_shadow_nodes_visited", C.c_uint64), ("own_shadow_refs_read", C.c_uint64),
+                ("own_shadow_tris_tested", C.c_uint64)]
 
 
 class PhotonParams(C.Structure):
