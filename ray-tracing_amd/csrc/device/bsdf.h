@@ -186,7 +186,7 @@ LRT_D f3 cos_hemi(float ux, float uy, float& pdf) {
 // rough-transmittance slice lookup (rough_transmittance.h:17-29)
 LRT_D float trans_eval(const float* tb, float ct) {
     if (!(ct >= 0.0f)) return 0.0f;
-    const float x = pow_rn(ct, 0.25f);
+    const float x = qrt_rn(ct);
     if (!(x >= 0.0f && x <= 1.0f)) return 0.0f;
     float t = ((x - 0.0f) * 99.0f) / (1.0f - 0.0f);
     int k = (int)t;

@@ -36,7 +36,7 @@ LRT_HD bool is_zero(f3 a) { return a.x == 0.0f && a.y == 0.0f && a.z == 0.0f; }
 LRT_HD float comp(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
 LRT_HD f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
 
-// sin, cos, pow and exp of a float as the reference's std::sin(float) etc.
+// sin, cos, x^(1/4) and exp of a float as the reference's std::sin(float) etc.
 // compute them on its CPU: in double precision, rounded once to float.  That
 // is the correctly rounded float except where the double result lies within a
 // double ulp of a float rounding boundary.  The host C library's float
@@ -44,13 +44,52 @@ LRT_HD f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
 // on 98.7% of sin/cos and 99.9% of pow/exp arguments); the device library's
 // float versions are not correctly rounded and differ from the host more
 // often, which forks sampled paths.
-LRT_D void sincos_rn(float x, float* s, float* c) {
-    double sd, cd;
-    sincos((double)x, &sd, &cd);
-    *s = (float)sd;
-    *c = (float)cd;
+//
+// sincos_rn serves the samplers, whose arguments are 2 pi u with u in [0, 1):
+// Cody-Waite reduction by pi/2 (pi/2 split in three parts, the first two with
+// 30 significant bits, so k * part is exact for |k| < 2^23) and the Taylor
+// series of sin and cos on [-pi/4, pi/4] through the r^21 and r^22 terms
+// (truncation below 2e-25), evaluated with fused multiply-adds: within 2
+// double ulps of the exact values for |x| < 2^22, and a fraction of the
+// registers of a general double sincos, which also handles huge arguments.
+LRT_D void sincos_rn(float xf, float* s, float* c) {
+    constexpr double kP1 = 0x1.921fb548p+0, kP2 = -0x1.de973dc8p-31, kP3 = -0x1.9d9cceba3f91fp-62;
+    const double x = (double)xf;
+    const double k = rint(x * 0.6366197723675814);  // nearest multiple of pi/2
+    const double r = ((x - k * kP1) - k * kP2) - k * kP3;
+    const double r2 = r * r;
+    double ps = 1.9572941063391263e-20;  // (-1)^n / (2n + 1)!, n = 10 .. 1
+    ps = fma(ps, r2, -8.22063524662433e-18);
+    ps = fma(ps, r2, 2.8114572543455206e-15);
+    ps = fma(ps, r2, -7.647163731819816e-13);
+    ps = fma(ps, r2, 1.6059043836821613e-10);
+    ps = fma(ps, r2, -2.505210838544172e-08);
+    ps = fma(ps, r2, 2.7557319223985893e-06);
+    ps = fma(ps, r2, -0.0001984126984126984);
+    ps = fma(ps, r2, 0.008333333333333333);
+    ps = fma(ps, r2, -0.16666666666666666);
+    const double sr = fma(r * r2, ps, r);
+    double pc = -8.896791392450574e-22;  // (-1)^n / (2n)!, n = 11 .. 1
+    pc = fma(pc, r2, 4.110317623312165e-19);
+    pc = fma(pc, r2, -1.5619206968586225e-16);
+    pc = fma(pc, r2, 4.779477332387385e-14);
+    pc = fma(pc, r2, -1.1470745597729725e-11);
+    pc = fma(pc, r2, 2.08767569878681e-09);
+    pc = fma(pc, r2, -2.755731922398589e-07);
+    pc = fma(pc, r2, 2.48015873015873e-05);
+    pc = fma(pc, r2, -0.001388888888888889);
+    pc = fma(pc, r2, 0.041666666666666664);
+    pc = fma(pc, r2, -0.5);
+    const double cr = fma(r2, pc, 1.0);
+    const int q = (int)k & 3;  // sin(r + q pi/2), cos(r + q pi/2)
+    double sv = (q & 1) ? cr : sr, cv = (q & 1) ? sr : cr;
+    if (q & 2) sv = -sv;
+    if ((q + 1) & 2) cv = -cv;
+    *s = (float)sv;
+    *c = (float)cv;
 }
-LRT_D float pow_rn(float x, float y) { return (float)pow((double)x, (double)y); }
+// x^(1/4) as two correctly rounded double square roots (within 0.75 double ulp)
+LRT_D float qrt_rn(float x) { return (float)sqrt(sqrt((double)x)); }
 LRT_D float exp_rn(float x) { return (float)exp((double)x); }
 
 // orthonormal basis around a normal (onb.h:8-23)
