@@ -344,6 +344,18 @@ def main():
                "denoised_best": None if r["denoised_best"] is None else rnd(r["denoised_best"]),
                "speedup": None if r["speedup"] is None else round(r["speedup"], 2),
                "speedup_4spp": None if r["speedup_4spp"] is None else round(r["speedup_4spp"], 2)}
+        # the denoiser's own roofline: the U-Net's 10 fused convolutions at
+        # 1080p, flops 2*9*cin*cout per output pixel over the summed layer
+        # times (HIP events on the denoiser's stream), against the dense bf16
+        # MFMA peak and against each layer's own bound
+        nw = r["network"]
+        ttq["network_roofline"] = {
+            "bound": "mfma", "kernel": "k_conv_fwd (U-Net forward, 10 layers, 1920x1080)",
+            "achieved": round(nw["tflops"], 1), "peak": dn.BF16_PEAK / 1e12, "unit": "TFLOP/s",
+            "frac": round(nw["tflops"] * 1e12 / dn.BF16_PEAK, 4), "traffic": None,
+            "flop_per_forward": round(nw["tflop"] * 1e12), "conv_ms": round(nw["conv_ms"], 4),
+            "forward_ms": round(nw["forward_ms"], 4),
+            "per_layer": nw["layers"]}
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "Msamples/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 2), "higher_is_better": True,

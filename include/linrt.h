@@ -532,6 +532,12 @@ int lrt_denoiser_forward(lrt_denoiser* d, const float* radiance, const float* au
 int lrt_denoiser_loss_grad(lrt_denoiser* d, const float* radiance, const float* aux, const float* target, uint32_t B,
                            uint32_t H, uint32_t W, float* loss, float* grads, int device_ptrs);
 int lrt_denoiser_adam_step(lrt_denoiser* d, float lr);
+/* per-layer timing of lrt_denoiser_forward: returns the milliseconds each of
+ * the 10 convolution layers took (HIP events on the denoiser's stream,
+ * summed over the forwards since the previous call; layer_ms may be NULL)
+ * and the number of those forwards, clears the sums, and turns timing on or
+ * off for the following forwards */
+int lrt_denoiser_timing(lrt_denoiser* d, int enable, double* layer_ms, uint64_t* calls);
 /* one 3x3 convolution layer on host FP32 arrays (rounded to BF16 on the
  * device): op 0 out = act(conv(x, w) + b) [B,H,W,cout]; op 1 the data
  * gradient out = conv^T(g, w) [B,H,W,cin]; op 2 the weight and bias gradient

@@ -10,7 +10,13 @@
 //   d1b 64->64 | up, concat skip 0 | d0a 96->32 | out 32->3 (linear)
 //
 // Skips are written by their producer straight into the channel slice of the
-// concat buffer the decoder reads, so no concat copies exist.
+// concat buffer the decoder reads.  The forward convolutions read pooled,
+// upsampled and concatenated inputs and the network's input features through
+// their staging (k_conv_fwd), so inference writes none of those tensors; the
+// training step materialises the ones its weight gradients read.
+#include <map>
+#include <mutex>
+
 #include "device/nn.h"
 
 namespace lrt {
@@ -51,6 +57,90 @@ static void conv(hipStream_t s, int coutp, const uint16_t* x, int B, int H, int 
         case 192: conv_launch<192>(s, x, B, H, W, cin, xcs, xoff, frag, bias, cout_real, relu, y, ycs, yoff, yf, yfcs, yfoff, yf_acc); break;
         default: throw Error(LRT_ERR_UNSUPPORTED, "conv: unsupported output channel count " + std::to_string(coutp));
     }
+    HIPCHECK(hipGetLastError());
+}
+
+// a direct bf16 source / a bf16 destination
+static ConvSrc src(const uint16_t* p, int cs, int off, int c, int mode = kSrcDirect) { return ConvSrc{p, cs, off, c, mode}; }
+static ConvIn in1(ConvSrc a) { return ConvIn{a, ConvSrc{nullptr, 0, 0, 0, kSrcDirect}, nullptr, nullptr}; }
+static ConvIn in2(ConvSrc a, ConvSrc b) { return ConvIn{a, b, nullptr, nullptr}; }
+static ConvOut out_bf16(uint16_t* y, int ycs, int yoff) {
+    return ConvOut{y, ycs, yoff, nullptr, 0, 0, 0, nullptr, nullptr, kOutBf16};
+}
+static ConvOut out_f32(float* yf, int yfcs, int yfoff, int acc) {
+    return ConvOut{nullptr, 0, 0, yf, yfcs, yfoff, acc, nullptr, nullptr, kOutF32};
+}
+
+// blocks of a 256-thread kernel the current device holds at once (all CUs),
+// per kernel
+static int conv_resident_blocks(const void* kern) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, int> known;
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = known.find({kern, dev});
+    if (it != known.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    HIPCHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0));
+    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int n = std::max(per_cu, 1) * std::max(cus, 1);
+    known[{kern, dev}] = n;
+    return n;
+}
+
+template <int M0, int M1, int OUT>
+static void conv_fwd_nt(hipStream_t s, int coutp, const ConvIn& in, int B, int H, int W, int cin, const void* frag,
+                        const float* bias, int cout_real, int relu, const ConvOut& out) {
+    const int nkc = cin / 32;
+    const bf16x8* f = (const bf16x8*)frag;
+    auto launch = [&](auto kern, int ntw) {
+        const int groups = coutp / 16 / ntw;
+        const long long tiles = (long long)B * ((W + kFwdTW - 1) / kFwdTW) * ((H + 3) / 4);
+        // one chunk: persistent blocks (as many as the CUs hold at once), the
+        // weights staged once per block and each tile's loads overlapping the
+        // previous tile's MFMAs; several chunks: a block per tile, which the
+        // hardware balances better over the few tiles of the coarse levels
+        long long nb = tiles;
+        if (nkc == 1) nb = std::min<long long>(tiles, std::max<long long>(1, (long long)conv_resident_blocks((const void*)kern) / groups));
+        hipLaunchKernelGGL(kern, dim3((uint32_t)nb, groups), dim3(256), 0, s, in, B, H, W, nkc, groups, f, bias, cout_real,
+                           relu, out);
+    };
+    switch (coutp) {
+        case 16: launch(k_conv_fwd<1, M0, M1, OUT>, 1); break;
+        case 32: launch(k_conv_fwd<2, M0, M1, OUT>, 2); break;
+        case 64: launch(k_conv_fwd<4, M0, M1, OUT>, 4); break;
+        case 128: launch(k_conv_fwd<4, M0, M1, OUT>, 4); break;
+        default: throw Error(LRT_ERR_UNSUPPORTED, "conv: unsupported output channel count " + std::to_string(coutp));
+    }
+}
+
+// the fused forward convolution over a virtual input of cin channels (cin a
+// multiple of 32, the sources' channel counts adding up to at most cin)
+static void conv_fwd(hipStream_t s, int coutp, const ConvIn& in, int B, int H, int W, int cin, const void* frag,
+                     const float* bias, int cout_real, int relu, const ConvOut& out) {
+    require(cin % 32 == 0 && in.s0.c % 8 == 0 && in.s1.c % 8 == 0 && in.s0.c + in.s1.c <= cin,
+            "conv: input channels must be padded to 32");
+    for (const ConvSrc* c : {&in.s0, &in.s1})
+        if (c->c && c->mode != kSrcFeatures)
+            require(c->p != nullptr && c->cs % 8 == 0 && c->off % 8 == 0 && c->off + c->c <= c->cs, "conv: bad source");
+    if (in.s0.mode == kSrcUp2 || in.s1.mode == kSrcUp2) require(H % 2 == 0 && W % 2 == 0, "conv: upsampled source needs even H, W");
+    const int m0 = in.s0.mode, m1 = in.s1.c ? in.s1.mode : m0;
+    // the combinations the network and lrt_conv3x3 use
+    if (m0 == kSrcFeatures && out.mode == kOutBf16)
+        conv_fwd_nt<kSrcFeatures, kSrcFeatures, kOutBf16>(s, coutp, in, B, H, W, cin, frag, bias, cout_real, relu, out);
+    else if (m0 == kSrcDirect && m1 == kSrcDirect && out.mode == kOutBf16)
+        conv_fwd_nt<kSrcDirect, kSrcDirect, kOutBf16>(s, coutp, in, B, H, W, cin, frag, bias, cout_real, relu, out);
+    else if (m0 == kSrcDirect && m1 == kSrcDirect && out.mode == kOutF32)
+        conv_fwd_nt<kSrcDirect, kSrcDirect, kOutF32>(s, coutp, in, B, H, W, cin, frag, bias, cout_real, relu, out);
+    else if (m0 == kSrcDirect && m1 == kSrcDirect && out.mode == kOutImage)
+        conv_fwd_nt<kSrcDirect, kSrcDirect, kOutImage>(s, coutp, in, B, H, W, cin, frag, bias, cout_real, relu, out);
+    else if (m0 == kSrcPool2 && m1 == kSrcPool2 && out.mode == kOutBf16)
+        conv_fwd_nt<kSrcPool2, kSrcPool2, kOutBf16>(s, coutp, in, B, H, W, cin, frag, bias, cout_real, relu, out);
+    else if (m0 == kSrcUp2 && m1 == kSrcDirect && out.mode == kOutBf16)
+        conv_fwd_nt<kSrcUp2, kSrcDirect, kOutBf16>(s, coutp, in, B, H, W, cin, frag, bias, cout_real, relu, out);
+    else
+        throw Error(LRT_ERR_UNSUPPORTED, "conv: unsupported combination of input sources and output");
     HIPCHECK(hipGetLastError());
 }
 
@@ -98,7 +188,15 @@ struct lrt_denoiser {
     DevBuf x0, a0, cat0, p1, a1, cat1, p2, a2, b2, d1a, d1b, d0a, out;
     DevBuf g, d_out, d_d0a, d_cat0, d_d1b, d_d1a, d_cat1, d_b2, d_a2, d_p2, d_a1, d_p1, d_a0;
     DevBuf in_rad, in_aux, in_tgt, res;
+    // per-layer timing of lrt_denoiser_forward (lrt_denoiser_timing): events
+    // around each convolution, their sums since the last read
+    bool timing = false;
+    hipEvent_t ev[11] = {};
+    double layer_ms[10] = {};
+    uint64_t timed_calls = 0;
     ~lrt_denoiser() {
+        for (hipEvent_t e : ev)
+            if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -162,33 +260,65 @@ static void shape_grad(lrt_denoiser& D) {
     D.d_a0.alloc(P * 32 * 4);
 }
 
-// forward pass from device radiance (3 / px) and aux (7 / px) into D.out (fp32, 16 / px)
-static void forward(lrt_denoiser& D, const float* rad, const float* aux) {
+// forward pass from device radiance (3 / px) and aux (7 / px): the denoised
+// radiance into img (3 floats / px), or with `train` the last layer's fp32
+// output into D.out (16 / px) and the tensors the weight gradients read (x0,
+// p1, p2 and the upsampled halves of cat0 / cat1), which the fused
+// convolutions themselves never write
+static void forward(lrt_denoiser& D, const float* rad, const float* aux, bool train, float* img) {
     hipStream_t s = D.stream;
     const int B = D.B, H = D.H, W = D.W, H1 = H / 2, W1 = W / 2, H2 = H / 4, W2 = W / 4;
     const long long P = (long long)B * H * W;
     const float* p = D.params.as<float>();
     auto wf = [&](int i) { return (const void*)D.L[i].fwd.p; };
     auto bias = [&](int i) { return p + D.L[i].boff; };
-    hipLaunchKernelGGL(k_nn_input, dim3(grid1(P, D.cus)), dim3(256), 0, s, rad, aux, P, D.x0.as<uint16_t>(), 32);
-    conv(s, 32, D.x0.as<uint16_t>(), B, H, W, 32, 32, 0, wf(0), bias(0), 32, 1, D.a0.as<uint16_t>(), 32, 0, nullptr, 0, 0, 0);
-    conv(s, 32, D.a0.as<uint16_t>(), B, H, W, 32, 32, 0, wf(1), bias(1), 32, 1, D.cat0.as<uint16_t>(), 96, 64, nullptr, 0, 0, 0);
-    hipLaunchKernelGGL(k_pool2, dim3(grid1(P / 4 * 32, D.cus)), dim3(256), 0, s, D.cat0.as<uint16_t>(), B, H, W, 32, 96, 64,
-                       D.p1.as<uint16_t>(), 32, 0);
-    conv(s, 64, D.p1.as<uint16_t>(), B, H1, W1, 32, 32, 0, wf(2), bias(2), 64, 1, D.a1.as<uint16_t>(), 64, 0, nullptr, 0, 0, 0);
-    conv(s, 64, D.a1.as<uint16_t>(), B, H1, W1, 64, 64, 0, wf(3), bias(3), 64, 1, D.cat1.as<uint16_t>(), 192, 128, nullptr, 0, 0, 0);
-    hipLaunchKernelGGL(k_pool2, dim3(grid1(P / 16 * 64, D.cus)), dim3(256), 0, s, D.cat1.as<uint16_t>(), B, H1, W1, 64, 192, 128,
-                       D.p2.as<uint16_t>(), 64, 0);
-    conv(s, 128, D.p2.as<uint16_t>(), B, H2, W2, 64, 64, 0, wf(4), bias(4), 128, 1, D.a2.as<uint16_t>(), 128, 0, nullptr, 0, 0, 0);
-    conv(s, 128, D.a2.as<uint16_t>(), B, H2, W2, 128, 128, 0, wf(5), bias(5), 128, 1, D.b2.as<uint16_t>(), 128, 0, nullptr, 0, 0, 0);
-    hipLaunchKernelGGL(k_up2, dim3(grid1(P / 4 * 128, D.cus)), dim3(256), 0, s, D.b2.as<uint16_t>(), B, H1, W1, 128, 128, 0,
-                       D.cat1.as<uint16_t>(), 192, 0);
-    conv(s, 64, D.cat1.as<uint16_t>(), B, H1, W1, 192, 192, 0, wf(6), bias(6), 64, 1, D.d1a.as<uint16_t>(), 64, 0, nullptr, 0, 0, 0);
-    conv(s, 64, D.d1a.as<uint16_t>(), B, H1, W1, 64, 64, 0, wf(7), bias(7), 64, 1, D.d1b.as<uint16_t>(), 64, 0, nullptr, 0, 0, 0);
-    hipLaunchKernelGGL(k_up2, dim3(grid1(P * 64, D.cus)), dim3(256), 0, s, D.d1b.as<uint16_t>(), B, H, W, 64, 64, 0,
-                       D.cat0.as<uint16_t>(), 96, 0);
-    conv(s, 32, D.cat0.as<uint16_t>(), B, H, W, 96, 96, 0, wf(8), bias(8), 32, 1, D.d0a.as<uint16_t>(), 32, 0, nullptr, 0, 0, 0);
-    conv(s, 16, D.d0a.as<uint16_t>(), B, H, W, 32, 32, 0, wf(9), bias(9), 3, 0, nullptr, 0, 0, D.out.as<float>(), 16, 0, 0);
+    auto mark = [&](int k) {
+        if (D.timing) HIPCHECK(hipEventRecord(D.ev[k], s));
+    };
+    uint16_t *a0 = D.a0.as<uint16_t>(), *cat0 = D.cat0.as<uint16_t>(), *a1 = D.a1.as<uint16_t>(),
+             *cat1 = D.cat1.as<uint16_t>(), *a2 = D.a2.as<uint16_t>(), *b2 = D.b2.as<uint16_t>(),
+             *d1a = D.d1a.as<uint16_t>(), *d1b = D.d1b.as<uint16_t>(), *d0a = D.d0a.as<uint16_t>();
+    ConvIn feat = in1(ConvSrc{nullptr, 0, 0, 32, kSrcFeatures});
+    feat.rad = rad;
+    feat.aux = aux;
+    mark(0);
+    conv_fwd(s, 32, feat, B, H, W, 32, wf(0), bias(0), 32, 1, out_bf16(a0, 32, 0));
+    mark(1);
+    conv_fwd(s, 32, in1(src(a0, 32, 0, 32)), B, H, W, 32, wf(1), bias(1), 32, 1, out_bf16(cat0, 96, 64));
+    mark(2);
+    conv_fwd(s, 64, in1(src(cat0, 96, 64, 32, kSrcPool2)), B, H1, W1, 32, wf(2), bias(2), 64, 1, out_bf16(a1, 64, 0));
+    mark(3);
+    conv_fwd(s, 64, in1(src(a1, 64, 0, 64)), B, H1, W1, 64, wf(3), bias(3), 64, 1, out_bf16(cat1, 192, 128));
+    mark(4);
+    conv_fwd(s, 128, in1(src(cat1, 192, 128, 64, kSrcPool2)), B, H2, W2, 64, wf(4), bias(4), 128, 1, out_bf16(a2, 128, 0));
+    mark(5);
+    conv_fwd(s, 128, in1(src(a2, 128, 0, 128)), B, H2, W2, 128, wf(5), bias(5), 128, 1, out_bf16(b2, 128, 0));
+    mark(6);
+    conv_fwd(s, 64, in2(src(b2, 128, 0, 128, kSrcUp2), src(cat1, 192, 128, 64)), B, H1, W1, 192, wf(6), bias(6), 64, 1,
+             out_bf16(d1a, 64, 0));
+    mark(7);
+    conv_fwd(s, 64, in1(src(d1a, 64, 0, 64)), B, H1, W1, 64, wf(7), bias(7), 64, 1, out_bf16(d1b, 64, 0));
+    mark(8);
+    conv_fwd(s, 32, in2(src(d1b, 64, 0, 64, kSrcUp2), src(cat0, 96, 64, 32)), B, H, W, 96, wf(8), bias(8), 32, 1,
+             out_bf16(d0a, 32, 0));
+    mark(9);
+    ConvOut o = out_f32(D.out.as<float>(), 16, 0, 0);
+    if (!train) {
+        o.mode = kOutImage;
+        o.img = img;
+        o.rad = rad;
+    }
+    conv_fwd(s, 16, in1(src(d0a, 32, 0, 32)), B, H, W, 32, wf(9), bias(9), 3, 0, o);
+    mark(10);
+    if (train) {
+        hipLaunchKernelGGL(k_nn_input, dim3(grid1(P, D.cus)), dim3(256), 0, s, rad, aux, P, D.x0.as<uint16_t>(), 32);
+        hipLaunchKernelGGL(k_pool2, dim3(grid1(P / 4 * 32, D.cus)), dim3(256), 0, s, cat0, B, H, W, 32, 96, 64,
+                           D.p1.as<uint16_t>(), 32, 0);
+        hipLaunchKernelGGL(k_pool2, dim3(grid1(P / 16 * 64, D.cus)), dim3(256), 0, s, cat1, B, H1, W1, 64, 192, 128,
+                           D.p2.as<uint16_t>(), 64, 0);
+        hipLaunchKernelGGL(k_up2, dim3(grid1(P / 4 * 128, D.cus)), dim3(256), 0, s, b2, B, H1, W1, 128, 128, 0, cat1, 192, 0);
+        hipLaunchKernelGGL(k_up2, dim3(grid1(P * 64, D.cus)), dim3(256), 0, s, d1b, B, H, W, 64, 64, 0, cat0, 96, 0);
+    }
     HIPCHECK(hipGetLastError());
 }
 
@@ -387,17 +517,22 @@ int lrt_denoiser_forward(lrt_denoiser* d, const float* rad, const float* aux, ui
         const size_t P = (size_t)B * H * W;
         const float *drad, *daux, *dtgt;
         stage_inputs(*d, rad, aux, nullptr, P, device_ptrs, drad, daux, dtgt);
-        forward(*d, drad, daux);
         float* dout = out;
         if (!device_ptrs) {
             d->res.alloc(P * 12);
             dout = d->res.as<float>();
         }
-        hipLaunchKernelGGL(k_nn_output, dim3(grid1((long long)P, d->cus)), dim3(256), 0, d->stream, d->out.as<float>(), 16,
-                           drad, (long long)P, dout);
-        HIPCHECK(hipGetLastError());
+        forward(*d, drad, daux, false, dout);
         if (!device_ptrs) HIPCHECK(hipMemcpyAsync(out, dout, P * 12, hipMemcpyDeviceToHost, d->stream));
         HIPCHECK(hipStreamSynchronize(d->stream));
+        if (d->timing) {
+            for (int k = 0; k < 10; ++k) {
+                float ms = 0.0f;
+                HIPCHECK(hipEventElapsedTime(&ms, d->ev[k], d->ev[k + 1]));
+                d->layer_ms[k] += ms;
+            }
+            d->timed_calls++;
+        }
     });
 }
 
@@ -411,7 +546,7 @@ int lrt_denoiser_loss_grad(lrt_denoiser* d, const float* rad, const float* aux, 
         const size_t P = (size_t)B * H * W;
         const float *drad, *daux, *dtgt;
         stage_inputs(*d, rad, aux, target, P, device_ptrs, drad, daux, dtgt);
-        forward(*d, drad, daux);
+        forward(*d, drad, daux, true, nullptr);
         const long long n = (long long)d->n_params;
         hipLaunchKernelGGL(k_zero_f32, dim3(grid1(n, d->cus)), dim3(256), 0, d->stream, d->grads.as<float>(), n);
         hipLaunchKernelGGL(k_zero_f32, dim3(grid1((long long)P * 16, d->cus)), dim3(256), 0, d->stream, d->d_out.as<float>(),
@@ -426,6 +561,21 @@ int lrt_denoiser_loss_grad(lrt_denoiser* d, const float* rad, const float* aux, 
         if (grads) HIPCHECK(hipMemcpyAsync(grads, d->grads.p, d->n_params * 4, hipMemcpyDeviceToHost, d->stream));
         HIPCHECK(hipStreamSynchronize(d->stream));
         if (loss) *loss = l;
+    });
+}
+
+int lrt_denoiser_timing(lrt_denoiser* d, int enable, double* layer_ms, uint64_t* calls) {
+    return guarded([&] {
+        require(d != nullptr, "null pointer");
+        HIPCHECK(hipSetDevice(d->device));
+        if (layer_ms)
+            for (int k = 0; k < 10; ++k) layer_ms[k] = d->layer_ms[k];
+        if (calls) *calls = d->timed_calls;
+        for (double& v : d->layer_ms) v = 0.0;
+        d->timed_calls = 0;
+        if (enable && !d->ev[0])
+            for (hipEvent_t& e : d->ev) HIPCHECK(hipEventCreate(&e));
+        d->timing = enable != 0;
     });
 }
 
@@ -479,8 +629,8 @@ int lrt_conv3x3(int device, int op, const float* x, const float* w, const float*
             hipLaunchKernelGGL(k_pack_weights, dim3(64), dim3(256), 0, s, dw.as<float>(), (int)cin, (int)cout, 0, cinp, coutp,
                                dfrag.as<bf16x8>());
             dout.alloc(P * cout * 4);
-            conv(s, coutp, dx.as<uint16_t>(), (int)B, (int)H, (int)W, cinp, cinp, 0, dfrag.p, b ? db.as<float>() : nullptr,
-                 (int)cout, relu, nullptr, 0, 0, dout.as<float>(), (int)cout, 0, 0);
+            conv_fwd(s, coutp, in1(src(dx.as<uint16_t>(), cinp, 0, cinp)), (int)B, (int)H, (int)W, cinp, dfrag.p,
+                     b ? db.as<float>() : nullptr, (int)cout, relu, out_f32(dout.as<float>(), (int)cout, 0, 0));
             HIPCHECK(hipMemcpy(out, dout.p, P * cout * 4, hipMemcpyDeviceToHost));
         } else if (op == 1) {
             require(g != nullptr, "null pointer");

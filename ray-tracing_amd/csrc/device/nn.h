@@ -102,6 +102,344 @@ __global__ __launch_bounds__(256) void k_conv3x3(const uint16_t* __restrict__ x,
     }
 }
 
+// ---- fused forward convolution -------------------------------------------
+//
+// The forward pass reads its input through up to two sources that together
+// form the conv's input channels (source 0: channels [0, s0.c), source 1:
+// [s0.c, s0.c + s1.c), zero beyond), each read in one of four ways while the
+// tile is staged, so no concatenated, upsampled, pooled or feature tensor is
+// ever written:
+//   kSrcDirect   a tensor at the conv's resolution;
+//   kSrcUp2      a tensor at half the resolution, nearest upsampling;
+//   kSrcPool2    a tensor at twice the resolution, 2x2 average (the four
+//                values summed in float in raster order, times 0.25, rounded
+//                to bf16: the arithmetic of k_pool2);
+//   kSrcFeatures the network input computed from radiance (3 floats / px) and
+//                the first-hit features (7 floats / px): log1p radiance,
+//                albedo, normal * 0.5 + 0.5, log1p depth (k_nn_input's
+//                arithmetic), channels 10..31 zero.
+// The output is bf16 activations, fp32 values, or (kOutImage) the denoised
+// radiance expm1(max(log1p(max(rad, 0)) + v, 0)) of channels 0..2.
+enum : int { kSrcDirect = 0, kSrcUp2 = 1, kSrcPool2 = 2, kSrcFeatures = 3 };
+enum : int { kOutBf16 = 0, kOutF32 = 1, kOutImage = 2 };
+
+struct ConvSrc {
+    const uint16_t* p;  // bf16 NHWC tensor (unused for kSrcFeatures)
+    int cs, off, c;     // channel stride, channel offset, channel count (multiples of 8)
+    int mode;
+};
+struct ConvIn {
+    ConvSrc s0, s1;
+    const float* rad;  // kSrcFeatures inputs
+    const float* aux;
+};
+struct ConvOut {
+    uint16_t* y;
+    int ycs, yoff;
+    float* yf;
+    int yfcs, yfoff, yf_acc;
+    float* img;
+    const float* rad;
+    int mode;
+};
+
+constexpr int kFwdTW = 64;          // output pixels per row of a block tile (four 16-pixel tiles per wave)
+constexpr int kFwdSW = kFwdTW + 2;  // staged columns
+constexpr int kFwdPitch = 32;       // bf16 per staged pixel: one 32-channel chunk, four 16-byte units
+
+// LDS offset (bf16 elements) of 16-byte unit c (channels 8c .. 8c + 7) of
+// staged pixel p.  The units of pixels p and p + 8 are XOR-swizzled apart
+// (c ^ 2 when bit 3 of p is set): a fragment read has 16 lanes on 16
+// consecutive pixels and two units each per 32-lane group, and with the
+// swizzle those 32 reads of 16 bytes fall on 32 different 16-byte bank
+// groups.
+LRT_D int fwd_lds(int p, int c) { return p * kFwdPitch + ((c ^ ((p >> 2) & 2)) << 3); }
+
+// block shape of k_conv_fwd<NTW>: four waves, one output row each, 4 x 16
+// pixels per wave, NTW 16-channel output tiles per block
+template <int NTW>
+struct FwdShape {
+    static constexpr int TR = 4;                             // output rows per tile
+    static constexpr int SR = TR + 2;                        // staged rows
+    static constexpr int PIX = SR * kFwdSW;                  // staged pixels
+    static constexpr int IN_UNITS = PIX * 4;                 // 16-byte input units per 32-channel chunk
+    static constexpr int W_UNITS = 9 * NTW * 64;             // 16-byte weight units per chunk (9 taps x NTW fragments)
+    static constexpr int PER_IN = (IN_UNITS + 255) / 256;    // per thread
+    static constexpr int PER_PIX = (PIX + 255) / 256;
+    static constexpr int PER_W = (W_UNITS + 255) / 256;
+    static constexpr int IN_ELEMS = PIX * kFwdPitch;         // bf16 elements of the staged input
+};
+
+LRT_D uint32_t bf2x(float a, float b) { return (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16); }
+
+// 8 channels [c, c + 8) of source s at pixel (b, yy, xx) of the conv's grid
+template <int MODE>
+LRT_D uint4 conv_src_unit(const ConvIn& in, const ConvSrc& s, int b, int H, int W, int yy, int xx, int c) {
+    if (MODE == kSrcDirect)
+        return *(const uint4*)(s.p + ((size_t)((size_t)b * H + yy) * W + xx) * s.cs + s.off + c);
+    if (MODE == kSrcUp2) {
+        const int h = H >> 1, w = W >> 1;
+        return *(const uint4*)(s.p + ((size_t)((size_t)b * h + (yy >> 1)) * w + (xx >> 1)) * s.cs + s.off + c);
+    }
+    if (MODE == kSrcPool2) {
+        const int W2 = 2 * W;
+        const size_t base = ((size_t)((size_t)b * 2 * H + 2 * yy)) * W2 + 2 * xx;
+        union U { uint4 v; uint16_t h[8]; };
+        U a, bb, cc, d, o;
+        a.v = *(const uint4*)(s.p + base * s.cs + s.off + c);
+        bb.v = *(const uint4*)(s.p + (base + 1) * s.cs + s.off + c);
+        cc.v = *(const uint4*)(s.p + (base + W2) * s.cs + s.off + c);
+        d.v = *(const uint4*)(s.p + (base + W2 + 1) * s.cs + s.off + c);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float sum = bf2f(a.h[k]) + bf2f(bb.h[k]) + bf2f(cc.h[k]) + bf2f(d.h[k]);
+            o.h[k] = f2bf(0.25f * sum);
+        }
+        return o.v;
+    }
+    // kSrcFeatures
+    const size_t q = ((size_t)b * H + yy) * W + xx;
+    if (c == 0) {
+        const float* r = in.rad + 3 * q;
+        const float* a = in.aux + 7 * q;
+        return make_uint4(bf2x(log1pf(fmaxf(r[0], 0.0f)), log1pf(fmaxf(r[1], 0.0f))),
+                          bf2x(log1pf(fmaxf(r[2], 0.0f)), a[0]), bf2x(a[1], a[2]),
+                          bf2x(a[3] * 0.5f + 0.5f, a[4] * 0.5f + 0.5f));
+    }
+    if (c == 8) {
+        const float* a = in.aux + 7 * q;
+        return make_uint4(bf2x(a[5] * 0.5f + 0.5f, log1pf(fmaxf(a[6], 0.0f))), 0u, 0u, 0u);
+    }
+    return make_uint4(0u, 0u, 0u, 0u);
+}
+
+// 8 input channels [ch, ch + 8) of pixel (b, yy, xx) of the conv's virtual
+// input; M0 / M1: the modes of sources 0 / 1 (fixed per kernel instance, so a
+// layer's registers are sized for its own reads)
+template <int M0, int M1>
+LRT_D uint4 conv_in_unit(const ConvIn& in, int b, int H, int W, int yy, int xx, int ch) {
+    uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    if (yy < 0 || yy >= H || xx < 0 || xx >= W) return z;
+    const bool second = ch >= in.s0.c;
+    if (second && M1 != M0) {
+        const int c = ch - in.s0.c;
+        return c < in.s1.c ? conv_src_unit<M1>(in, in.s1, b, H, W, yy, xx, c) : z;
+    }
+    const ConvSrc s = second ? in.s1 : in.s0;
+    const int c = second ? ch - in.s0.c : ch;
+    return c < s.c ? conv_src_unit<M0>(in, s, b, H, W, yy, xx, c) : z;
+}
+
+// the two 16-byte units (channels 0..7, 8..15) of the network input at pixel
+// (b, yy, xx), zero outside the image (k_nn_input's arithmetic)
+LRT_D void feature_units(const ConvIn& in, int b, int H, int W, int yy, int xx, uint4& u0, uint4& u1) {
+    u0 = u1 = make_uint4(0u, 0u, 0u, 0u);
+    if (yy < 0 || yy >= H || xx < 0 || xx >= W) return;
+    const size_t q = ((size_t)b * H + yy) * W + xx;
+    const float* r = in.rad + 3 * q;
+    const float* a = in.aux + 7 * q;
+    u0 = make_uint4(bf2x(log1pf(fmaxf(r[0], 0.0f)), log1pf(fmaxf(r[1], 0.0f))), bf2x(log1pf(fmaxf(r[2], 0.0f)), a[0]),
+                    bf2x(a[1], a[2]), bf2x(a[3] * 0.5f + 0.5f, a[4] * 0.5f + 0.5f));
+    u1 = make_uint4(bf2x(a[5] * 0.5f + 0.5f, log1pf(fmaxf(a[6], 0.0f))), 0u, 0u, 0u);
+}
+
+// y = act(conv3x3(x) + bias) over the virtual input `in` (nkc chunks of 32
+// channels), output channels [16 NTW g, 16 NTW (g + 1)) of group g =
+// blockIdx.y, output written as OUT (out.mode).  Blocks (grid (n, groups),
+// 256 threads) walk the tiles of 64 x 4 output pixels t = blockIdx.x,
+// + gridDim.x, ... (persistent when n is below the tile count); per chunk
+// a tile stages its input (6 rows x 66 columns x 32 channels) and the
+// chunk's weight fragments of the group (9 taps x NTW; once per block when
+// there is one chunk) in LDS, so the tap loop reads only LDS, and the loads
+// of the next (tile, chunk) are in flight while the current one's MFMAs run.
+// The MFMA's A operand is the weight fragment and B the staged pixels, so D
+// is [channel][pixel]: a lane ends with four consecutive channels of a pixel
+// and writes them with one 8-byte store.
+template <int NTW, int M0, int M1, int OUT>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTW >= 4 ? 2 : 3))) void k_conv_fwd(ConvIn in, int B, int H, int W, int nkc, int groups,
+                                                  const bf16x8* __restrict__ wfrag, const float* __restrict__ bias,
+                                                  int cout_real, int relu, ConvOut out) {
+    using S = FwdShape<NTW>;
+    constexpr bool FEAT = M0 == kSrcFeatures;
+    __shared__ __attribute__((aligned(16))) uint16_t tile[S::IN_ELEMS];
+    __shared__ __attribute__((aligned(16))) bf16x8 wtile[S::W_UNITS];
+    const int g = blockIdx.y;
+    const int tiles_x = (W + kFwdTW - 1) / kFwdTW, tiles_y = (H + S::TR - 1) / S::TR;
+    const int ntiles = B * tiles_x * tiles_y;
+    const int NT = groups * NTW;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int i = lane & 15, kq = lane >> 4;
+    int t = blockIdx.x;
+    if (t >= ntiles) return;
+    f32x4 acc[4][NTW];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int tt = 0; tt < NTW; ++tt) acc[m][tt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    // staging registers of one (tile, chunk)
+    uint4 rin[FEAT ? 2 * S::PER_PIX : S::PER_IN];
+    bf16x8 rw[S::PER_W];
+    auto origin = [&](int tl, int& b, int& y0, int& x0) {
+        const int tx = tl % tiles_x, r = tl / tiles_x;
+        x0 = tx * kFwdTW;
+        y0 = (r % tiles_y) * S::TR;
+        b = r / tiles_y;
+    };
+    auto load = [&](int tl, int kc, bool weights) {
+        int b, y0, x0;
+        origin(tl, b, y0, x0);
+        if constexpr (FEAT) {
+#pragma unroll
+            for (int k = 0; k < S::PER_PIX; ++k) {
+                const int p = threadIdx.x + 256 * k;
+                if (p < S::PIX) {
+                    const int r = p / kFwdSW, px = p - r * kFwdSW;
+                    feature_units(in, b, H, W, y0 + r - 1, x0 + px - 1, rin[2 * k], rin[2 * k + 1]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < S::PER_IN; ++k) {
+                const int u = threadIdx.x + 256 * k;
+                if (u < S::IN_UNITS) {
+                    const int c8 = u & 3, pr = u >> 2;
+                    const int r = pr / kFwdSW, px = pr - r * kFwdSW;
+                    rin[k] = conv_in_unit<M0, M1>(in, b, H, W, y0 + r - 1, x0 + px - 1, kc * 32 + c8 * 8);
+                }
+            }
+        }
+        if (weights) {
+#pragma unroll
+            for (int k = 0; k < S::PER_W; ++k) {
+                const int u = threadIdx.x + 256 * k;
+                if (u < S::W_UNITS) {
+                    const int f = u >> 6;  // tap * NTW + tt
+                    const int tap = f / NTW, tt = f - tap * NTW;
+                    rw[k] = wfrag[((size_t)(tap * nkc + kc) * NT + g * NTW + tt) * 64 + (u & 63)];
+                }
+            }
+        }
+    };
+    auto store = [&](bool weights) {
+        if constexpr (FEAT) {
+#pragma unroll
+            for (int k = 0; k < S::PER_PIX; ++k) {
+                const int p = threadIdx.x + 256 * k;
+                if (p < S::PIX) {
+                    *(uint4*)(tile + fwd_lds(p, 0)) = rin[2 * k];
+                    *(uint4*)(tile + fwd_lds(p, 1)) = rin[2 * k + 1];
+                    *(uint4*)(tile + fwd_lds(p, 2)) = make_uint4(0u, 0u, 0u, 0u);
+                    *(uint4*)(tile + fwd_lds(p, 3)) = make_uint4(0u, 0u, 0u, 0u);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < S::PER_IN; ++k) {
+                const int u = threadIdx.x + 256 * k;
+                if (u < S::IN_UNITS) *(uint4*)(tile + fwd_lds(u >> 2, u & 3)) = rin[k];
+            }
+        }
+        if (weights) {
+#pragma unroll
+            for (int k = 0; k < S::PER_W; ++k) {
+                const int u = threadIdx.x + 256 * k;
+                if (u < S::W_UNITS) wtile[u] = rw[k];
+            }
+        }
+    };
+    const bool w_once = nkc == 1;
+    int kc = 0;
+    bool first = true;
+    load(t, 0, true);
+    for (;;) {
+        if (!first) __syncthreads();
+        store(first || !w_once);
+        __syncthreads();
+        first = false;
+        int tn = t, kcn = kc + 1;
+        if (kcn == nkc) {
+            kcn = 0;
+            tn = t + gridDim.x;
+        }
+        const bool more = tn < ntiles;
+        if (more) load(tn, kcn, !w_once);
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) {
+            const int ky = tap / 3, kx = tap - 3 * ky;
+            bf16x8 bw[NTW];
+#pragma unroll
+            for (int tt = 0; tt < NTW; ++tt) bw[tt] = wtile[(tap * NTW + tt) * 64 + lane];
+            // pixel tiles m are 16 pixels apart: same swizzle, offset 16 m pitches
+            const uint16_t* arow = tile + fwd_lds((wave + ky) * kFwdSW + i + kx, kq);
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const bf16x8 a = *(const bf16x8*)(arow + 16 * m * kFwdPitch);
+#pragma unroll
+                for (int tt = 0; tt < NTW; ++tt)
+                    acc[m][tt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bw[tt], a, acc[m][tt], 0, 0, 0);
+            }
+        }
+        if (kc == nkc - 1) {
+            int b, y0, x0;
+            origin(t, b, y0, x0);
+            const int row = y0 + wave;
+            if (row < H) {
+                const size_t rowq = ((size_t)b * H + row) * W;
+#pragma unroll
+                for (int tt = 0; tt < NTW; ++tt) {
+                    const int co0 = (g * NTW + tt) * 16 + 4 * kq;  // this lane's four channels co0 .. co0 + 3
+                    if (co0 >= cout_real) continue;
+                    float bv[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) bv[j] = (bias && co0 + j < cout_real) ? bias[co0 + j] : 0.0f;
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        const int px = x0 + 16 * m + i;
+                        if (px >= W) continue;
+                        float v[4];
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            v[j] = acc[m][tt][j] + bv[j];
+                            if (relu) v[j] = fmaxf(v[j], 0.0f);
+                        }
+                        const size_t q = rowq + px;
+                        if constexpr (OUT == kOutBf16) {
+                            uint16_t* o = out.y + q * out.ycs + out.yoff + co0;
+                            if (co0 + 3 < cout_real) {
+                                *(uint2*)o = make_uint2(bf2x(v[0], v[1]), bf2x(v[2], v[3]));
+                            } else {
+#pragma unroll
+                                for (int j = 0; j < 4; ++j)
+                                    if (co0 + j < cout_real) o[j] = f2bf(v[j]);
+                            }
+                        } else if constexpr (OUT == kOutF32) {
+#pragma unroll
+                            for (int j = 0; j < 4; ++j)
+                                if (co0 + j < cout_real) {
+                                    float* o = out.yf + q * out.yfcs + out.yfoff + co0 + j;
+                                    *o = out.yf_acc ? *o + v[j] : v[j];
+                                }
+                        } else if (co0 == 0) {  // kOutImage
+#pragma unroll
+                            for (int j = 0; j < 3; ++j) {
+                                const float l = log1pf(fmaxf(out.rad[3 * q + j], 0.0f)) + v[j];
+                                out.img[3 * q + j] = expm1f(fmaxf(l, 0.0f));
+                            }
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+                for (int tt = 0; tt < NTW; ++tt) acc[m][tt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        }
+        if (!more) break;
+        t = tn;
+        kc = kcn;
+    }
+}
+
 // dW[tap][ci][co] += sum_q X[q shifted by tap][ci] * G[q][co] and, from the
 // blocks with tap 0 and ci-tile 0, db[co] += sum_q G[q][co].
 // grid (chunks, 9, cin / 16), 256 threads; each wave takes 32-pixel k-chunks

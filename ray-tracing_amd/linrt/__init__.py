@@ -159,7 +159,7 @@ EXPORTED = [
     "lrt_photon_build", "lrt_photon_export", "lrt_photon_map_load", "lrt_photon_estimate", "lrt_render_photon",
     "lrt_generate_scene", "lrt_generated_desc", "lrt_generated_free", "lrt_philox4x32_10", "lrt_render_aov",
     "lrt_denoiser_create", "lrt_denoiser_release", "lrt_denoiser_param_count", "lrt_denoiser_params",
-    "lrt_denoiser_forward", "lrt_denoiser_loss_grad", "lrt_denoiser_adam_step", "lrt_conv3x3",
+    "lrt_denoiser_forward", "lrt_denoiser_loss_grad", "lrt_denoiser_adam_step", "lrt_conv3x3", "lrt_denoiser_timing",
     "lrt_diff_param_count", "lrt_diff_render", "lrt_diff_adjoint", "lrt_diff_render_held",
     "lrt_load_obj", "lrt_mesh_view", "lrt_mesh_free", "lrt_generate_obj_scene", "lrt_image_write",
     "lrt_image_to_bytes", "lrt_render_capture_rays", "lrt_trace_closest_noalloc", "lrt_sky_cook",
@@ -761,6 +761,15 @@ class Denoiser:
         out = np.zeros((B, H, W, 3), np.float32)
         _check(LIB.lrt_denoiser_forward(self._h, _p(rad), _p(aux), C.c_uint32(B), C.c_uint32(H), C.c_uint32(W), _p(out), 0))
         return out
+
+    def timing(self, enable=True):
+        """Per-layer milliseconds summed over the forwards since the previous
+        call (10 convolution layers) and the number of those forwards; timing
+        stays on for the following forwards if `enable`."""
+        ms = (C.c_double * 10)()
+        n = C.c_uint64()
+        _check(LIB.lrt_denoiser_timing(self._h, C.c_int(int(enable)), ms, C.byref(n)))
+        return list(ms), n.value
 
     def forward_device(self, rad_ptr, aux_ptr, B, H, W, out_ptr):
         _check(LIB.lrt_denoiser_forward(self._h, C.c_void_p(rad_ptr), C.c_void_p(aux_ptr), C.c_uint32(B), C.c_uint32(H),

@@ -143,6 +143,49 @@ def linrt_layers():
     return [10, 32, 32, 32, 32, 64, 64, 64, 64, 128, 128, 128, 192, 64, 64, 64, 96, 32, 32, 3]
 
 
+BF16_PEAK = 2.5e15  # dense bf16 MFMA flop/s of one MI355X
+HBM_PEAK = 8.0e12   # HBM bytes/s
+
+
+def layer_work(H, W):
+    """Algorithmic work of the forward's 10 convolutions at an H x W frame:
+    (name, flops, bytes) per layer.  flops = 2 * 9 * cin * cout per output
+    pixel (real channel counts); bytes = what the layer must read and write
+    once: its input sources (the pooled and upsampled ones at their own
+    resolution, the features as 10 floats, the first layer's radiance and
+    the last layer's radiance in and image out as 3 floats) and its bf16
+    output."""
+    P0, P1, P2 = H * W, (H // 2) * (W // 2), (H // 4) * (W // 4)
+    bf = 2
+    L = [  # name, cin, cout, output pixels, bytes read, bytes written
+        ("e0a features->32", 10, 32, P0, P0 * 40, P0 * 32 * bf),
+        ("e0b 32->32", 32, 32, P0, P0 * 32 * bf, P0 * 32 * bf),
+        ("e1a pool->64", 32, 64, P1, P0 * 32 * bf, P1 * 64 * bf),
+        ("e1b 64->64", 64, 64, P1, P1 * 64 * bf, P1 * 64 * bf),
+        ("e2a pool->128", 64, 128, P2, P1 * 64 * bf, P2 * 128 * bf),
+        ("e2b 128->128", 128, 128, P2, P2 * 128 * bf, P2 * 128 * bf),
+        ("d1a up+skip 192->64", 192, 64, P1, P2 * 128 * bf + P1 * 64 * bf, P1 * 64 * bf),
+        ("d1b 64->64", 64, 64, P1, P1 * 64 * bf, P1 * 64 * bf),
+        ("d0a up+skip 96->32", 96, 32, P0, P1 * 64 * bf + P0 * 32 * bf, P0 * 32 * bf),
+        ("out 32->3 image", 32, 3, P0, P0 * 32 * bf + P0 * 12, P0 * 12),
+    ]
+    return [(n, 2 * 9 * ci * co * p, rd + wr) for n, ci, co, p, rd, wr in L]
+
+
+def layer_roofline(H, W, layer_ms):
+    """Per layer: achieved flop/s, its bound min(bf16 MFMA peak, HBM peak x
+    flops/bytes) and the fraction reached, from measured milliseconds."""
+    out = []
+    for (name, fl, by), ms in zip(layer_work(H, W), layer_ms):
+        bound = min(BF16_PEAK, HBM_PEAK * fl / by)
+        ach = fl / (ms * 1e-3) if ms > 0 else 0.0
+        out.append({"layer": name, "gflop": round(fl / 1e9, 3), "mbytes": round(by / 1e6, 2),
+                    "intensity": round(fl / by, 1), "ms": round(ms, 4), "tflops": round(ach / 1e12, 1),
+                    "bound_tflops": round(bound / 1e12, 1), "bound": "mfma" if bound >= BF16_PEAK else "hbm",
+                    "frac_of_bound": round(ach / bound, 4)})
+    return out
+
+
 def c1_scene():
     gen = linrt.displaced_sphere(seed=1)
     return gen, linrt.Scene(gen.desc_ptr)
@@ -236,6 +279,23 @@ def time_to_quality(den=None, target=0.01, log=print, spp_list=(1, 2, 4), weight
         if e <= target and best is None:
             best = (spp, dt, e)
     res["denoised_best"] = None if best is None else {"spp": best[0], "seconds": best[1], "relmse": best[2]}
+    # the network alone at 1080p: per-layer times (HIP events around each
+    # convolution) against each layer's roofline
+    den.timing(True)
+    n_fw = 20
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n_fw):
+        den.forward_device(r.data_ptr(), a.data_ptr(), 1, H, W, out.data_ptr())
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / n_fw
+    ms, calls = den.timing(False)
+    ms = [m / max(calls, 1) for m in ms]
+    layers = layer_roofline(H, W, ms)
+    fl = sum(f for _, f, _ in layer_work(H, W))
+    res["network"] = {"forward_ms": wall * 1e3, "conv_ms": sum(ms), "tflop": fl / 1e12,
+                      "tflops": fl / (sum(ms) * 1e-3) / 1e12, "layers": layers}
+    log(f"  network: {wall * 1e3:.3f} ms per 1080p forward, {fl / (sum(ms) * 1e-3) / 1e12:.1f} TFLOP/s over the convs")
     # plain: relMSE ~ c / spp; measure at 64 spp, render the predicted spp for
     # the target and step up by 10% until it is reached
     torch.cuda.synchronize()
