@@ -126,6 +126,45 @@ def ray_bench(args, torch, linrt):
     print(json.dumps(out), flush=True)
 
 
+def network_bench(args, torch, linrt):
+    """The denoiser's U-Net alone: forwards of the committed network on one
+    1920x1080 frame (radiance + first-hit features of the 1080p workload at
+    1 spp, resident on the device), per-layer times from HIP events on the
+    denoiser's stream against each layer's roofline."""
+    from linrt import denoise as dn
+    gen = linrt.displaced_sphere(seed=SEED)
+    scene = linrt.Scene(gen.desc_ptr, device=0)
+    rad = dn.radiance(scene, W, H, 1, SEED, 0)
+    aux = dn.features(scene, W, H, 1, SEED, 0)
+    out = torch.zeros_like(rad)
+    den = linrt.Denoiser()
+    dn.load(den)
+    for _ in range(max(args.warmup, 1)):
+        den.forward_device(rad.data_ptr(), aux.data_ptr(), 1, H, W, out.data_ptr())
+    torch.cuda.synchronize()
+    den.timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        den.forward_device(rad.data_ptr(), aux.data_ptr(), 1, H, W, out.data_ptr())
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / args.steps
+    ms, calls = den.timing(False)
+    ms = [m / max(calls, 1) for m in ms]
+    fl = sum(f for _, f, _ in dn.layer_work(H, W))
+    ach = fl / (sum(ms) * 1e-3)
+    out_line = {"metric": "denoiser U-Net forward at 1920x1080 (TFLOP/s over its convolutions)",
+                "value": round(ach / 1e12, 1), "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps,
+                "warmup": args.warmup, "ms_per_step": round(el * 1e3, 4), "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)",
+                "data": "1-spp radiance and features of the 1080p workload",
+                "config": {"workload": "U-Net 10->32/64/128->3, 3x3 convolutions, 1920x1080, batch 1"},
+                "roofline": {"bound": "mfma", "kernel": "k_conv_fwd", "achieved": round(ach / 1e12, 1),
+                             "peak": dn.BF16_PEAK / 1e12, "unit": "TFLOP/s", "frac": round(ach / dn.BF16_PEAK, 4),
+                             "traffic": None, "flop_per_forward": fl, "conv_ms": round(sum(ms), 4)},
+                "per_layer": dn.layer_roofline(H, W, ms)}
+    print(json.dumps(out_line), flush=True)
+
+
 def photon_bench(args, torch, linrt):
     """Photon-mapper workload: 4,194,304 photons traced (max depth 8, RR from
     5) from the 0.5 x 0.5 ceiling light through the 262,144-tri displaced
@@ -205,6 +244,7 @@ def main():
     ap.add_argument("--spp", type=int, default=SPP)
     ap.add_argument("--rays", type=int, default=0, help="ray-traversal benchmark with this many rays instead")
     ap.add_argument("--photon", action="store_true", help="photon-mapper benchmark instead")
+    ap.add_argument("--network", action="store_true", help="the denoiser's network alone at 1080p instead")
     ap.add_argument("--no-denoise", action="store_true", help="skip the time-to-relMSE leg")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="only copy this many MiB device-to-device 5 times (HBM counter calibration for rocprofv3 --pmc)")
@@ -248,6 +288,8 @@ def main():
         return ray_bench(args, torch, linrt)
     if args.photon:
         return photon_bench(args, torch, linrt)
+    if args.network:
+        return network_bench(args, torch, linrt)
 
     if linrt.device_count() < 1 or not torch.cuda.is_available():
         log("no GPU visible")

@@ -8,7 +8,7 @@ high-spp frame as the target.  Frames stay in device memory (torch tensors
 are used only as allocations); every network step runs the library's BF16
 matrix-core kernels through the C ABI.
 
-  python -m linrt.denoise train [--scenes N] [--steps K] [--out path]
+  python -m linrt.denoise train [--scenes N] [--steps K] [--out path] [--evaluate]
   python -m linrt.denoise oracle --spp 65536 [--out path]   (the FP64 65,536-spp oracle of the whole frame)
   python -m linrt.denoise evaluate                            (time to relMSE <= 1e-2 over the whole frame)
 """
@@ -102,6 +102,94 @@ def make_dataset(n_scenes=160, W=1920, H=1080, band=128, spp_in=(2, 4, 8), spp_t
     return torch.stack(rad), torch.stack(aux), torch.stack(tgt), torch.tensor(idx, device="cuda:0")
 
 
+def random_sky(rng, turbidity=None):
+    """A sky of a seeded sun (elevation 10..60 degrees, any azimuth) cooked by
+    the library with the 1080p workload's turbidity 3 (or a seeded one in
+    2..6 when turbidity is "random"), ground albedo 0.3 and radiance scale."""
+    el = math.radians(rng.uniform(10.0, 60.0))
+    az = rng.uniform(0.0, 2.0 * math.pi)
+    d = [math.cos(el) * math.cos(az), math.sin(el), math.cos(el) * math.sin(az)]
+    t = float(rng.uniform(2.0, 6.0)) if turbidity == "random" else (3.0 if turbidity is None else float(turbidity))
+    return linrt.cook_sky(d, turbidity=t, ground_albedo=0.3, scale=0.05)
+
+
+def crop_camera(desc, rng, crop=256, W=1920, H=1080):
+    """A copy of the generated scene description whose camera looks at a
+    seeded crop x crop window of the W x H frame: aimed at the window's centre
+    ray, square, with the frame's angular pixel size (vertical field of view
+    scaled by crop / H), so the crop sees the scene at full-frame resolution."""
+    d = linrt.SceneDesc.from_buffer_copy(desc.contents)
+    cam = d.camera
+    o = np.array(cam.origin[:], np.float64)
+    la = np.array(cam.look_at[:], np.float64)
+    up = np.array(cam.up[:], np.float64)
+    h = math.tan(math.radians(cam.fov_deg) / 2)
+    vh, vw = 2 * h, 2 * h * (W / H)
+    w = (o - la) / np.linalg.norm(o - la)
+    u = np.cross(up, w)
+    u /= np.linalg.norm(u)
+    v = np.cross(w, u)
+    llc = o - vw / 2 * u - vh / 2 * v - w
+    cx = rng.uniform(crop / 2, W - crop / 2)
+    cy = rng.uniform(crop / 2, H - crop / 2)
+    ray = llc + cx / (W - 1) * vw * u + cy / (H - 1) * vh * v - o
+    ray /= np.linalg.norm(ray)
+    for k in range(3):
+        cam.look_at[k] = float(o[k] + ray[k])
+    cam.fov_deg = float(math.degrees(2 * math.atan(h * crop / H)))
+    cam.aspect = 1.0
+    return d
+
+
+def make_crop_dataset(n_scenes=2000, crop=256, spp_in=(1, 2, 4), spp_tgt=1024, segments=1024, rings=512, seed0=10000,
+                      builders=4, log=print):
+    """The training set: n_scenes seeded scenes of the 1080p workload's
+    generator (the displaced sphere at segments x rings, materials and mesh
+    from the seed, a sky of a seeded sun angle), each seen through a crop x
+    crop window of the 1080p frame at a seeded position; noisy frames (with
+    their first-hit features) at each spp of spp_in and a target at spp_tgt.
+    Scenes are generated and their kd trees built on `builders` host threads
+    ahead of the renders.  Device tensors rad [N * len(spp_in), crop, crop,
+    3], aux [.., 7], tgt [N, crop, crop, 3] and each input's target index."""
+    import collections
+    from concurrent.futures import ThreadPoolExecutor
+    torch = _torch()
+    n_in = len(spp_in)
+    rad = torch.empty((n_scenes * n_in, crop, crop, 3), dtype=torch.float32, device="cuda:0")
+    aux = torch.empty((n_scenes * n_in, crop, crop, 7), dtype=torch.float32, device="cuda:0")
+    tgt = torch.empty((n_scenes, crop, crop, 3), dtype=torch.float32, device="cuda:0")
+    idx = torch.arange(n_scenes, device="cuda:0").repeat_interleave(n_in)
+
+    def build(s):
+        seed = seed0 + s
+        rng = np.random.default_rng(seed)
+        gen = linrt.displaced_sphere_variant(seed=seed, segments=segments, rings=rings, aspect=1920 / 1080,
+                                             sky=random_sky(rng))
+        desc = crop_camera(gen.desc_ptr, rng, crop)
+        return seed, gen, desc, linrt.Scene(linrt.C.byref(desc))
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=builders) as ex:
+        pending = collections.deque(ex.submit(build, s) for s in range(min(n_scenes, builders + 2)))
+        nxt = len(pending)
+        for s in range(n_scenes):
+            seed, gen, desc, sc = pending.popleft().result()
+            if nxt < n_scenes:
+                pending.append(ex.submit(build, nxt))
+                nxt += 1
+            off = 0
+            for k, spp in enumerate(spp_in):
+                rad[s * n_in + k] = radiance(sc, crop, crop, spp, seed, off)
+                aux[s * n_in + k] = features(sc, crop, crop, spp, seed, off)
+                off += spp
+            tgt[s] = radiance(sc, crop, crop, spp_tgt, seed, 1 << 20)
+            sc.close()
+            gen.close()
+            if (s + 1) % 50 == 0:
+                log(f"  dataset: {s + 1}/{n_scenes} scenes, {time.perf_counter() - t0:.1f}s")
+    return rad, aux, tgt, idx
+
+
 def train(den, data, steps=1500, batch=8, crop=128, lr=2e-3, seed=0, log=print):
     """Adam on random crops (with flips) of the dataset; cosine learning rate."""
     torch = _torch()
@@ -110,6 +198,7 @@ def train(den, data, steps=1500, batch=8, crop=128, lr=2e-3, seed=0, log=print):
     g = torch.Generator(device="cpu").manual_seed(seed)
     t0 = time.perf_counter()
     hist = []
+    skipped = 0
     for it in range(steps):
         f = torch.randint(0, N, (batch,), generator=g)
         ys = torch.randint(0, H - crop + 1, (batch,), generator=g)
@@ -122,10 +211,18 @@ def train(den, data, steps=1500, batch=8, crop=128, lr=2e-3, seed=0, log=print):
             a[..., 3] = -a[..., 3]  # mirrored normals
         r, a, t = r.contiguous(), a.contiguous(), t.contiguous()
         loss = den.loss_grad_device(r.data_ptr(), a.data_ptr(), t.data_ptr(), batch, crop, crop)
-        den.adam_step(lr * 0.5 * (1 + math.cos(math.pi * it / steps)))
-        hist.append(loss)
+        # a step whose loss is non-finite or far above the recent mean (a
+        # sample of extreme fireflies) is not applied: one such update can
+        # push the ReLUs dead and end the training
+        recent = np.mean(hist[-200:]) if len(hist) >= 200 else None
+        if math.isfinite(loss) and (recent is None or loss <= 4.0 * recent):
+            den.adam_step(lr * 0.5 * (1 + math.cos(math.pi * it / steps)))
+        else:
+            skipped += 1
+        hist.append(loss if math.isfinite(loss) else (recent or 0.0))
         if (it + 1) % 250 == 0:
-            log(f"  step {it + 1}/{steps}: loss {np.mean(hist[-250:]):.4f} ({time.perf_counter() - t0:.1f}s)")
+            log(f"  step {it + 1}/{steps}: loss {np.mean(hist[-250:]):.4f} ({time.perf_counter() - t0:.1f}s, "
+                f"{skipped} steps skipped)")
     torch.cuda.synchronize()
     return hist
 
@@ -330,14 +427,19 @@ def time_to_quality(den=None, target=0.01, log=print, spp_list=(1, 2, 4), weight
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("cmd", choices=["train", "oracle", "evaluate"])
-    ap.add_argument("--scenes", type=int, default=160)
+    ap.add_argument("--scenes", type=int, default=2000)
     ap.add_argument("--steps", type=int, default=12000)
-    ap.add_argument("--spp-target", type=int, default=512)
+    ap.add_argument("--spp-target", type=int, default=1024)
     ap.add_argument("--spp", type=int, default=16384)
     ap.add_argument("--out", type=str, default=None)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--weights", type=str, default=None)
+    ap.add_argument("--dataset", choices=["crops", "bands"], default="crops",
+                    help="crops: 256x256 windows of 2,000 seeded scenes with 1,024-spp targets; "
+                         "bands: 128-row bands of 1080p frames")
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--evaluate", action="store_true", help="after training, time to relMSE with the new weights")
     args = ap.parse_args(argv)
     log = lambda *a: print(*a, file=sys.stderr, flush=True)  # noqa: E731
     _torch()
@@ -350,12 +452,21 @@ def main(argv=None):
         return
     den = linrt.Denoiser(seed=args.seed)
     t0 = time.perf_counter()
-    data = make_dataset(args.scenes, spp_tgt=args.spp_target, log=log)
+    if args.dataset == "crops":
+        data = make_crop_dataset(args.scenes, spp_tgt=args.spp_target, log=log)
+    else:
+        data = make_dataset(args.scenes, spp_tgt=args.spp_target, log=log)
     log(f"dataset {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
-    train(den, data, steps=args.steps, batch=args.batch, log=log)
+    train(den, data, steps=args.steps, batch=args.batch, lr=args.lr, log=log)
     log(f"training {time.perf_counter() - t0:.1f}s")
-    save(den, Path(args.out) if args.out else WEIGHTS)
+    out = Path(args.out) if args.out else WEIGHTS
+    save(den, out)
+    if args.evaluate:
+        import json
+        del data
+        r = time_to_quality(den=den, log=log)
+        print(json.dumps({k: v for k, v in r.items() if k != "network"}))
 
 
 if __name__ == "__main__":

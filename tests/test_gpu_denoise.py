@@ -40,6 +40,23 @@ def test_conv_forward(cin, cout, relu):
     assert err < 1e-4
 
 
+@pytest.mark.parametrize("cin,cout", [(32, 32), (10, 3)])
+def test_conv_forward_many_tiles(cin, cout):
+    """A frame of 256 x 1000 pixels (1,024 tiles of 64 x 4, more than the
+    GPU holds blocks at once) through a single-chunk layer: every block
+    walks several tiles, staging each next tile while computing the current
+    one, and the last column of tiles is partial."""
+    rng = np.random.default_rng(cin * 3 + cout)
+    x = rng.normal(size=(1, 256, 1000, cin)).astype(np.float32)
+    w = (rng.normal(size=(9, cin, cout)) / np.sqrt(9 * cin)).astype(np.float32)
+    b = rng.normal(size=cout).astype(np.float32)
+    y = linrt.conv3x3(0, w, x=x, b=b, relu=1)
+    ref = NN.conv3x3(bfr(x), bfr(w), b, True)
+    err = np.abs(y - ref).max() / np.abs(ref).max()
+    print(f"conv {cin}->{cout} over 256 x 1000: max err / scale {err:.2e}")
+    assert err < 1e-4
+
+
 @pytest.mark.parametrize("cin,cout", [(10, 32), (32, 3), (64, 128), (192, 64), (96, 32)])
 def test_conv_data_grad(cin, cout):
     rng = np.random.default_rng(cin * 11 + cout)
