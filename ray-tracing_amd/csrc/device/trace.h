@@ -88,6 +88,7 @@ LRT_D bool emitter_nonconvex(const DevScene& S, int e) {
 #define LRT_SHADE_ATTR
 #endif
 constexpr int kLdsStack = LRT_LDS_STACK;
+
 constexpr int kLeafBatch = LRT_LEAF_BATCH;
 constexpr int kMaxStack = 48;
 
@@ -273,9 +274,12 @@ LRT_D bool kd_traverse(const DevScene& S, const DevObject& ob, const Ray& r, flo
             // issue the reference and vertex loads of a batch together,
             // then test in reference order (ties go to the later triangle)
             TriRec rec[kLeafBatch];
+            // every load issued, past the leaf's end clamped to its last
+            // triangle (a record of the same leaf, never tested), so no branch
+            // separates them and the batch's loads are in flight together
 #pragma unroll
             for (int j = 0; j < kLeafBatch; ++j)
-                if (k + j < count) rec[j] = load_tri(S.leaf_tris + 9 * (size_t)(first + k + j));
+                rec[j] = load_tri(S.leaf_tris + 9 * (size_t)(first + min(k + (uint32_t)j, count - 1u)));
 #pragma unroll
             for (int j = 0; j < kLeafBatch; ++j) {
                 if (k + j < count) {
@@ -553,9 +557,10 @@ LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* 
         float limit = fminf(maxt, T.closest);
         for (uint32_t k = 0; k < count; k += kLeafBatch) {
             TriRec rec[kLeafBatch];
+            // as in kd_traverse: the batch's loads issued together, clamped to the leaf
 #pragma unroll
             for (int j = 0; j < kLeafBatch; ++j)
-                if (k + j < count) rec[j] = load_tri(S.leaf_tris + 9 * (size_t)(firstref + k + j));
+                rec[j] = load_tri(S.leaf_tris + 9 * (size_t)(firstref + min(k + (uint32_t)j, count - 1u)));
 #pragma unroll
             for (int j = 0; j < kLeafBatch; ++j) {
                 if (k + j < count) {
