@@ -25,12 +25,8 @@ namespace lrt {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
-LRT_D uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0u;
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
+// float -> bf16 rounded to nearest even (v_cvt_pk_bf16_f32; NaN stays a NaN)
+LRT_D uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 LRT_D float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 constexpr int kConvPx = 64;        // output pixels per block (one row segment)
@@ -170,45 +166,39 @@ struct FwdShape {
     static constexpr int IN_ELEMS = PIX * kFwdPitch;         // bf16 elements of the staged input
 };
 
-LRT_D uint32_t bf2x(float a, float b) { return (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16); }
+typedef __attribute__((ext_vector_type(2))) float f32x2;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+// two floats -> packed bf16 pair (a in the low half), one conversion instruction
+LRT_D uint32_t bf2x(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
 
 // 8 channels [c, c + 8) of source s at pixel (b, yy, xx) of the conv's grid
 template <int MODE>
 LRT_D uint4 conv_src_unit(const ConvIn& in, const ConvSrc& s, int b, int H, int W, int yy, int xx, int c) {
+    // element offsets in 32 bits (the host checks that every tensor has fewer than 2^31 elements)
     if (MODE == kSrcDirect)
-        return *(const uint4*)(s.p + ((size_t)((size_t)b * H + yy) * W + xx) * s.cs + s.off + c);
+        return *(const uint4*)(s.p + (uint32_t)(((b * H + yy) * W + xx) * s.cs + s.off + c));
     if (MODE == kSrcUp2) {
         const int h = H >> 1, w = W >> 1;
-        return *(const uint4*)(s.p + ((size_t)((size_t)b * h + (yy >> 1)) * w + (xx >> 1)) * s.cs + s.off + c);
+        return *(const uint4*)(s.p + (uint32_t)(((b * h + (yy >> 1)) * w + (xx >> 1)) * s.cs + s.off + c));
     }
     if (MODE == kSrcPool2) {
         const int W2 = 2 * W;
-        const size_t base = ((size_t)((size_t)b * 2 * H + 2 * yy)) * W2 + 2 * xx;
+        const uint32_t base = (uint32_t)((b * 2 * H + 2 * yy) * W2 + 2 * xx);
+        const uint32_t o0 = base * (uint32_t)s.cs + (uint32_t)(s.off + c), ocs = (uint32_t)s.cs;
         union U { uint4 v; uint16_t h[8]; };
         U a, bb, cc, d, o;
-        a.v = *(const uint4*)(s.p + base * s.cs + s.off + c);
-        bb.v = *(const uint4*)(s.p + (base + 1) * s.cs + s.off + c);
-        cc.v = *(const uint4*)(s.p + (base + W2) * s.cs + s.off + c);
-        d.v = *(const uint4*)(s.p + (base + W2 + 1) * s.cs + s.off + c);
+        a.v = *(const uint4*)(s.p + o0);
+        bb.v = *(const uint4*)(s.p + (o0 + ocs));
+        cc.v = *(const uint4*)(s.p + (o0 + (uint32_t)W2 * ocs));
+        d.v = *(const uint4*)(s.p + (o0 + ((uint32_t)W2 + 1u) * ocs));
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const float sum = bf2f(a.h[k]) + bf2f(bb.h[k]) + bf2f(cc.h[k]) + bf2f(d.h[k]);
             o.h[k] = f2bf(0.25f * sum);
         }
         return o.v;
-    }
-    // kSrcFeatures
-    const size_t q = ((size_t)b * H + yy) * W + xx;
-    if (c == 0) {
-        const float* r = in.rad + 3 * q;
-        const float* a = in.aux + 7 * q;
-        return make_uint4(bf2x(log1pf(fmaxf(r[0], 0.0f)), log1pf(fmaxf(r[1], 0.0f))),
-                          bf2x(log1pf(fmaxf(r[2], 0.0f)), a[0]), bf2x(a[1], a[2]),
-                          bf2x(a[3] * 0.5f + 0.5f, a[4] * 0.5f + 0.5f));
-    }
-    if (c == 8) {
-        const float* a = in.aux + 7 * q;
-        return make_uint4(bf2x(a[5] * 0.5f + 0.5f, log1pf(fmaxf(a[6], 0.0f))), 0u, 0u, 0u);
     }
     return make_uint4(0u, 0u, 0u, 0u);
 }
@@ -235,9 +225,9 @@ LRT_D uint4 conv_in_unit(const ConvIn& in, int b, int H, int W, int yy, int xx, 
 LRT_D void feature_units(const ConvIn& in, int b, int H, int W, int yy, int xx, uint4& u0, uint4& u1) {
     u0 = u1 = make_uint4(0u, 0u, 0u, 0u);
     if (yy < 0 || yy >= H || xx < 0 || xx >= W) return;
-    const size_t q = ((size_t)b * H + yy) * W + xx;
-    const float* r = in.rad + 3 * q;
-    const float* a = in.aux + 7 * q;
+    const uint32_t q = (uint32_t)((b * H + yy) * W + xx);
+    const float* r = in.rad + 3u * q;
+    const float* a = in.aux + 7u * q;
     u0 = make_uint4(bf2x(log1pf(fmaxf(r[0], 0.0f)), log1pf(fmaxf(r[1], 0.0f))), bf2x(log1pf(fmaxf(r[2], 0.0f)), a[0]),
                     bf2x(a[1], a[2]), bf2x(a[3] * 0.5f + 0.5f, a[4] * 0.5f + 0.5f));
     u1 = make_uint4(bf2x(a[5] * 0.5f + 0.5f, log1pf(fmaxf(a[6], 0.0f))), 0u, 0u, 0u);
@@ -245,9 +235,9 @@ LRT_D void feature_units(const ConvIn& in, int b, int H, int W, int yy, int xx, 
 
 // y = act(conv3x3(x) + bias) over the virtual input `in` (nkc chunks of 32
 // channels), output channels [16 NTW g, 16 NTW (g + 1)) of group g =
-// blockIdx.y, output written as OUT (out.mode).  Blocks (grid (n, groups),
-// 256 threads) walk the tiles of 64 x 4 output pixels t = blockIdx.x,
-// + gridDim.x, ... (persistent when n is below the tile count); per chunk
+// blockIdx.y, output written as OUT (out.mode).  Persistent blocks (grid
+// (n, groups), 256 threads) walk the tiles of 64 x 4 output pixels
+// t = blockIdx.x, + gridDim.x, ...; per chunk
 // a tile stages its input (6 rows x 66 columns x 32 channels) and the
 // chunk's weight fragments of the group (9 taps x NTW; once per block when
 // there is one chunk) in LDS, so the tap loop reads only LDS, and the loads
@@ -285,9 +275,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTW >= 4 ? 
         y0 = (r % tiles_y) * S::TR;
         b = r / tiles_y;
     };
+    int nb = 0, ny0 = 0, nx0 = 0;  // origin of the tile being loaded
     auto load = [&](int tl, int kc, bool weights) {
         int b, y0, x0;
         origin(tl, b, y0, x0);
+        nb = b;
+        ny0 = y0;
+        nx0 = x0;
         if constexpr (FEAT) {
 #pragma unroll
             for (int k = 0; k < S::PER_PIX; ++k) {
@@ -348,9 +342,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTW >= 4 ? 
         }
     };
     const bool w_once = nkc == 1;
+    // this lane's bias values (its four channels of each output tile)
+    float bias_r[NTW][4];
+#pragma unroll
+    for (int tt = 0; tt < NTW; ++tt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int co = (g * NTW + tt) * 16 + 4 * kq + j;
+            bias_r[tt][j] = (bias && co < cout_real) ? bias[co] : 0.0f;
+        }
     int kc = 0;
     bool first = true;
     load(t, 0, true);
+    int cb = nb, cy0 = ny0, cx0 = nx0;  // origin of the tile being computed
     for (;;) {
         if (!first) __syncthreads();
         store(first || !w_once);
@@ -380,18 +384,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTW >= 4 ? 
             }
         }
         if (kc == nkc - 1) {
-            int b, y0, x0;
-            origin(t, b, y0, x0);
-            const int row = y0 + wave;
-            if (row < H) {
-                const size_t rowq = ((size_t)b * H + row) * W;
+            const int row = cy0 + wave, x0 = cx0;
+            if constexpr (OUT == kOutImage) {
+                // channels 0..2 of pixel i sit in lane (i, kq = 0) of tile 0;
+                // lane (i, j), j < 3, takes channel j of that pixel, so 48
+                // lanes share the output transform instead of 16
+                if (row < H) {
+                    const uint32_t rowq = (uint32_t)((cb * H + row) * W);
+                    const float bk = (bias && kq < 3 && kq < cout_real) ? bias[kq] : 0.0f;  // channel kq's bias
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) {
+                        const float c0 = __shfl(acc[m][0][0], i), c1 = __shfl(acc[m][0][1], i),
+                                    c2 = __shfl(acc[m][0][2], i);
+                        const int px = x0 + 16 * m + i;
+                        if (g == 0 && kq < 3 && kq < cout_real && px < W) {
+                            float v = (kq == 0 ? c0 : (kq == 1 ? c1 : c2)) + bk;
+                            if (relu) v = fmaxf(v, 0.0f);
+                            const uint32_t q = rowq + (uint32_t)px;
+                            const float l = log1pf(fmaxf(out.rad[3u * q + kq], 0.0f)) + v;
+                            out.img[3u * q + kq] = expm1f(fmaxf(l, 0.0f));
+                        }
+                    }
+                }
+            } else if (row < H) {
+                const uint32_t rowq = (uint32_t)((cb * H + row) * W);
 #pragma unroll
                 for (int tt = 0; tt < NTW; ++tt) {
                     const int co0 = (g * NTW + tt) * 16 + 4 * kq;  // this lane's four channels co0 .. co0 + 3
                     if (co0 >= cout_real) continue;
-                    float bv[4];
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) bv[j] = (bias && co0 + j < cout_real) ? bias[co0 + j] : 0.0f;
 #pragma unroll
                     for (int m = 0; m < 4; ++m) {
                         const int px = x0 + 16 * m + i;
@@ -399,12 +419,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTW >= 4 ? 
                         float v[4];
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
-                            v[j] = acc[m][tt][j] + bv[j];
+                            v[j] = acc[m][tt][j] + bias_r[tt][j];
                             if (relu) v[j] = fmaxf(v[j], 0.0f);
                         }
-                        const size_t q = rowq + px;
+                        const uint32_t q = rowq + (uint32_t)px;
                         if constexpr (OUT == kOutBf16) {
-                            uint16_t* o = out.y + q * out.ycs + out.yoff + co0;
+                            uint16_t* o = out.y + (q * (uint32_t)out.ycs + (uint32_t)(out.yoff + co0));
                             if (co0 + 3 < cout_real) {
                                 *(uint2*)o = make_uint2(bf2x(v[0], v[1]), bf2x(v[2], v[3]));
                             } else {
@@ -419,12 +439,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTW >= 4 ? 
                                     float* o = out.yf + q * out.yfcs + out.yfoff + co0 + j;
                                     *o = out.yf_acc ? *o + v[j] : v[j];
                                 }
-                        } else if (co0 == 0) {  // kOutImage
-#pragma unroll
-                            for (int j = 0; j < 3; ++j) {
-                                const float l = log1pf(fmaxf(out.rad[3 * q + j], 0.0f)) + v[j];
-                                out.img[3 * q + j] = expm1f(fmaxf(l, 0.0f));
-                            }
                         }
                     }
                 }
@@ -437,6 +451,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTW >= 4 ? 
         if (!more) break;
         t = tn;
         kc = kcn;
+        cb = nb;
+        cy0 = ny0;
+        cx0 = nx0;
     }
 }
 

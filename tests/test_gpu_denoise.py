@@ -120,6 +120,33 @@ def test_unet_loss_and_gradient_vs_oracle():
     assert np.abs(out - ref).max() < 1e-2 * np.abs(ref).max()
 
 
+def test_forward_with_biases_vs_oracle():
+    """The fused forward (features read while staging, pooled / upsampled /
+    concatenated inputs, the image written by the last layer) with nonzero
+    biases in every layer, on a frame whose width is not a multiple of the
+    64-pixel tile, against the float64 oracle of the network."""
+    rng = np.random.default_rng(11)
+    den = linrt.Denoiser(seed=5)
+    p = den.params.copy()
+    off = 0
+    for cin, cout, _ in NN.NET:
+        off += 9 * cin * cout
+        p[off:off + cout] = rng.normal(scale=0.2, size=cout)
+        off += cout
+    den.params = p
+    rad, aux, _ = synthetic_batch(rng, 2, 20, 72)
+    out = den.forward(rad, aux)
+    ref = NN.denoise(p.astype(np.float64), rad.astype(np.float64), aux.astype(np.float64))
+    err = np.abs(out - ref).max() / np.abs(ref).max()
+    print(f"forward with biases: max err / scale {err:.2e}")
+    assert err < 1e-2
+    # the output layer's bias moves every channel: a zero-bias forward differs
+    q = p.copy()
+    q[off - 3:off] = 0.0
+    den.params = q
+    assert np.abs(den.forward(rad, aux) - out).max() > 10 * err * np.abs(ref).max()
+
+
 def test_training_lowers_loss():
     rng = np.random.default_rng(9)
     den = linrt.Denoiser(seed=4)
