@@ -287,7 +287,11 @@ enum { LRT_RENDER_COUNT = 1, LRT_RENDER_REFERENCE_SAMPLER = 2,
        /* lrt_diff_render / lrt_diff_adjoint only: the per-sample kernels (one thread
           traces a whole sample) instead of the wavefront; the same estimator, an
           independent implementation kept for checking */
-       LRT_DIFF_PER_SAMPLE = 4 };
+       LRT_DIFF_PER_SAMPLE = 4,
+       /* lrt_render: every bounce as a wavefront launch (the last bounces of a pass are
+          otherwise finished by one lane per path once few paths are left; the same
+          arithmetic, the same results) */
+       LRT_RENDER_WAVEFRONT_ONLY = 8 };
 
 typedef struct lrt_render_stats {
     uint64_t samples, extension_rays, shadow_rays, segments;

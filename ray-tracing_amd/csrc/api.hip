@@ -64,6 +64,22 @@ static uint32_t pass_paths() {
     return v;
 }
 
+// live paths at or below which a pass's remaining bounces run in k_tail
+// (one lane finishes a path) instead of one wavefront launch per bounce, from
+// the Russian-roulette depth on; LINRT_TAIL_PATHS overrides (0: never)
+#ifndef LRT_TAIL_PATHS
+#define LRT_TAIL_PATHS (1u << 18)
+#endif
+static uint32_t tail_paths() {
+    static const uint32_t v = [] {
+        const char* e = std::getenv("LINRT_TAIL_PATHS");
+        if (!e) return (uint32_t)LRT_TAIL_PATHS;
+        const long long x = std::atoll(e);
+        return x >= 0 && x < (1ll << 31) ? (uint32_t)x : (uint32_t)LRT_TAIL_PATHS;
+    }();
+    return v;
+}
+
 namespace lrt {
 
 struct Generated;
@@ -1158,6 +1174,11 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         const uint32_t g_trace = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc->cus * 20);
         const uint32_t g_shade = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc->cus * 8);
         const uint32_t g_bins = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc->cus * 4);
+        // the tail kernel: not with the counting kernels (they count per bounce), ray capture or
+        // LRT_RENDER_WAVEFRONT_ONLY
+        const uint32_t tail = (counting || capture || (prm->flags & LRT_RENDER_WAVEFRONT_ONLY)) ? 0u : tail_paths();
+        const uint32_t g_tail = (uint32_t)std::max<uint64_t>(
+            1, std::min<uint64_t>(((uint64_t)tail + kBlock - 1) / kBlock, (uint64_t)sc->cus * 8));
         Timer& tc = sc->timers[LRT_KERNEL_CLOSEST];
         Timer& to = sc->timers[LRT_KERNEL_OCCLUDED];
         Timer& ts = sc->timers[LRT_KERNEL_SHADE];
@@ -1182,6 +1203,11 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
                 ShadeParams sp{depth, prm->max_depth, prm->rr_depth, smp, key, sobol};
                 auto e2 = ts.next();
                 HIPCHECK(hipEventRecord(e2.first, sc->stream));
+                if (tail && depth >= prm->rr_depth) {  // few paths left: finish them in one launch
+                    hipLaunchKernelGGL(sc->mesh_lights ? k_tail<true> : k_tail<false>, dim3(g_tail), dim3(kBlock), 0,
+                                       sc->stream, sc->S, sc->Ls, sc->M, P, q, counters, Hb, sp, tail, dstats);
+                    hipLaunchKernelGGL(k_tail_done, dim3(1), dim3(64), 0, sc->stream, counters, tail);
+                }
                 hipLaunchKernelGGL(k_shade_bins, dim3(g_bins), dim3(kBlock), 0, sc->stream, sc->S, Hb, counters);
                 hipLaunchKernelGGL(k_shade_scan, dim3(1), dim3(1024), 0, sc->stream, counters);
                 hipLaunchKernelGGL(k_shade_order, dim3(g_bins), dim3(kBlock), 0, sc->stream, sc->S, Hb, counters, order);

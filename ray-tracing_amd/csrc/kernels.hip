@@ -396,9 +396,146 @@ __global__ __launch_bounds__(kBlock) void k_shade_order(const DevScene S, const 
     }
 }
 
-// one bounce of path_integrator::radiance for every queued path
+// What one bounce of a path leaves for the traversal: whether the path goes
+// on (its next ray is in its record) and its next-event shadow ray with the
+// contribution the ray adds if unoccluded.
+struct ShadeOut {
+    bool cont, shadow;
+    f3 so, sd, contrib;
+    float tmax;
+    int ignore;
+};
+
+// One bounce of path_integrator::radiance (path_integrator.h:32-166) for
+// path p at hit h: escape or emission with MIS, next-event estimation, BSDF
+// sampling and Russian roulette; the path record is updated in place.
 // MESH_LIGHTS: compiled with the mesh lights' pdf ray cast (launched only for
 // scenes that have one, so the other scenes' shading keeps its registers)
+template <bool MESH_LIGHTS>
+LRT_D void shade_path(const DevScene& S, const DevLights& Ls, const Mats& M, const PathState& P, uint32_t p,
+                      const HitRec& h, const ShadeParams& prm, ShadeOut& o) {
+    o.cont = false;
+    o.shadow = false;
+    o.tmax = 0.0f;
+    o.ignore = -1;
+    float4* pw = path_words(P, p);
+    const float4 w0 = pw[0], w1 = pw[1], w2 = pw[2], w3 = pw[3];
+    const Ray r{xyz(w0), xyz(w1)};
+    f3 T = xyz(w2);
+    f3 L = xyz(w3);
+    float eta = w0.w;
+    const float sa_pdf = w1.w;
+    const bool have = sa_pdf >= 0.0f;
+    const int obj = h.obj;
+    if (obj < 0) {  // escape: environment with MIS (path_integrator.h:32-45)
+        const f3 Le = env_eval(Ls, r.d);
+        if (!have) {
+            L = L + T * Le;
+        } else {
+            const float lp = pdf_emitter_direction<MESH_LIGHTS>(S, Ls, splat(0.0f), r.d);
+            L = L + T * power_heuristic(sa_pdf, lp) * Le;
+        }
+    } else {
+        const SurfaceHit sh = surface(S, r, obj, h.prim, h.t, h.u, h.v);
+        if (sh.emitter >= 0 && sh.front) {  // path_integrator.h:47-59
+            const f3 out = -normalize(r.d);
+            const f3 Le = dot(sh.n, out) <= 0.0f ? splat(0.0f) : ld3(Ls.l[sh.emitter].radiance);
+            if (!is_zero(Le)) {
+                if (!have) {
+                    L = L + T * Le;
+                } else {
+                    const float lp = pdf_emitter_direction<MESH_LIGHTS>(S, Ls, r.o, r.d);
+                    L = L + T * power_heuristic(sa_pdf, lp) * Le;
+                }
+            }
+        }
+        if (sh.material >= 0) {
+            const Frame fr = frame_from(sh.n);
+            SRec sr;
+            sr.wi = to_local(fr, -normalize(r.d));
+            sr.front = sh.front;
+            BsdfCtx ctx;
+            // six sampler dimensions per bounce (path_integrator.h:80-83)
+            const uint32_t pix = P.pixel0 + p % P.npix, smp = prm.sample + p / P.npix;
+            const uint32_t d0 = 2u + 6u * prm.depth;
+            float u6[6];  // light uv, lobe, BSDF uv, roulette
+            if (prm.sobol) {
+                // sampler::next_2d is vec2(next_1d(), next_1d()) (sampler.h:27-29),
+                // whose arguments the compiled reference evaluates right to
+                // left: a 2D sample is (dimension d + 1, dimension d)
+                const uint32_t ps = mix_bits(pix);
+                u6[0] = ref_sample(prm.sobol, ps, smp, d0 + 1u);
+                u6[1] = ref_sample(prm.sobol, ps, smp, d0);
+                u6[2] = ref_sample(prm.sobol, ps, smp, d0 + 2u);
+                u6[3] = ref_sample(prm.sobol, ps, smp, d0 + 4u);
+                u6[4] = ref_sample(prm.sobol, ps, smp, d0 + 3u);
+                u6[5] = ref_sample(prm.sobol, ps, smp, d0 + 5u);
+            } else {
+                const uint4 A = philox(make_uint4(pix, smp, d0 >> 2, 0u), prm.key);
+                const uint4 B = philox(make_uint4(pix, smp, (d0 >> 2) + 1u, 0u), prm.key);
+#pragma unroll
+                for (int k = 0; k < 6; ++k) {
+                    const uint32_t d = d0 + (uint32_t)k;
+                    u6[k] = u24((d >> 2) == (d0 >> 2) ? pick(A, d & 3u) : pick(B, d & 3u));
+                }
+            }
+            ctx.lobe = u6[2];
+            const uint32_t fl = bsdf_flags(M, sh.material);
+            if (fl & (F_DIFFUSE | F_GLOSSY)) {  // NEE (path_integrator.h:86-128)
+                DirectSample ds;
+                if (sample_emitter_direction(S, Ls, sh.p, u6[0], u6[1], ds)) {
+                    o.so = sh.p + sh.n * 1e-4f;
+                    o.sd = ds.dir;
+                    o.tmax = kInfF;
+                    bool target = true;
+                    if (!isinf(ds.dist)) {
+                        const f3 to = ds.point - o.so;
+                        const float dist = len(to);
+                        if (dist < 1e-6f) target = false;
+                        else { o.sd = to / dist; o.tmax = dist * (1.0f - 1e-4f); }
+                    }
+                    if (target) {
+                        sr.wo = to_local(fr, o.sd);
+                        const EP e = bsdf_ep(M, sh.material, ctx, sr, M_SOLID);
+                        if (!is_zero(e.f)) {
+                            const float lp = pdf_emitter_direction<MESH_LIGHTS>(S, Ls, sh.p, o.sd);
+                            if (lp > 0.0f) {
+                                const float w = power_heuristic(lp, e.pdf);
+                                o.contrib = T * e.f * ds.radiance * (w / lp);
+                                o.ignore = ds.emitter;
+                                o.shadow = true;
+                            }
+                        }
+                    }
+                }
+            }
+            if (prm.depth + 1 < prm.max_depth) {  // last bounce does NEE only
+                const f3 bw = bsdf_sample(M, sh.material, ctx, sr, u6[3], u6[4]);
+                if (!is_zero(bw)) {
+                    const f3 nd = to_world(fr, sr.wo);
+                    T = T * bw;
+                    eta *= sr.eta;
+                    const bool hp = ((sr.type & F_SPECULAR) == 0) && sr.pdf > 0.0f;
+                    o.cont = true;
+                    if (prm.depth >= prm.rr_depth) {  // Russian roulette (path_integrator.h:155-166)
+                        const float q = fminf(fmaxf(fmaxf(T.x, T.y), T.z) * eta * eta, 0.95f);
+                        if (u6[5] >= q) o.cont = false;
+                        else T = f3{T.x / q, T.y / q, T.z / q};
+                    }
+                    if (o.cont) {
+                        pw[0] = word(sh.p, eta);
+                        pw[1] = word(nd, hp ? sr.pdf : -1.0f);
+                        pw[2] = word(T, 0.0f);
+                    }
+                }
+            }
+        }
+    }
+    pw[3] = word(L, 0.0f);
+}
+
+// one bounce of path_integrator::radiance for every queued path, in the
+// shading order of the hit slots
 template <bool MESH_LIGHTS>
 __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_shade(const DevScene S, const DevLights Ls, const Mats M, PathState P,
                                                   const uint32_t* __restrict__ queue, uint32_t* __restrict__ next_queue,
@@ -409,142 +546,86 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_shade(const DevScene 
     // grid-stride with a wave-uniform trip count so the queue appends see full waves
     const uint32_t n_round = (n + stride - 1) / stride * stride;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_round; j += stride) {
-        bool cont = false, shadow = false;
+        ShadeOut o;
+        o.cont = false;
+        o.shadow = false;
         uint32_t p = 0;
-        f3 so, sd, contrib;
-        float tmax = 0.0f;
-        int ignore = -1;
         if (j < n) {
             const uint32_t i = order[j];  // hit slots grouped by material
             p = queue[i];
-            float4* pw = path_words(P, p);
-            const float4 w0 = pw[0], w1 = pw[1], w2 = pw[2], w3 = pw[3];
-            const Ray r{xyz(w0), xyz(w1)};
-            f3 T = xyz(w2);
-            f3 L = xyz(w3);
-            float eta = w0.w;
-            const float sa_pdf = w1.w;
-            const bool have = sa_pdf >= 0.0f;
-            const HitRec h = hit_load(Hb, i);
-            const int obj = h.obj;
-            if (obj < 0) {  // escape: environment with MIS (path_integrator.h:32-45)
-                const f3 Le = env_eval(Ls, r.d);
-                if (!have) {
-                    L = L + T * Le;
-                } else {
-                    const float lp = pdf_emitter_direction<MESH_LIGHTS>(S, Ls, splat(0.0f), r.d);
-                    L = L + T * power_heuristic(sa_pdf, lp) * Le;
-                }
-            } else {
-                const SurfaceHit sh = surface(S, r, obj, h.prim, h.t, h.u, h.v);
-                if (sh.emitter >= 0 && sh.front) {  // path_integrator.h:47-59
-                    const f3 out = -normalize(r.d);
-                    const f3 Le = dot(sh.n, out) <= 0.0f ? splat(0.0f) : ld3(Ls.l[sh.emitter].radiance);
-                    if (!is_zero(Le)) {
-                        if (!have) {
-                            L = L + T * Le;
-                        } else {
-                            const float lp = pdf_emitter_direction<MESH_LIGHTS>(S, Ls, r.o, r.d);
-                            L = L + T * power_heuristic(sa_pdf, lp) * Le;
-                        }
-                    }
-                }
-                if (sh.material >= 0) {
-                    const Frame fr = frame_from(sh.n);
-                    SRec sr;
-                    sr.wi = to_local(fr, -normalize(r.d));
-                    sr.front = sh.front;
-                    BsdfCtx ctx;
-                    // six sampler dimensions per bounce (path_integrator.h:80-83)
-                    const uint32_t pix = P.pixel0 + p % P.npix, smp = prm.sample + p / P.npix;
-                    const uint32_t d0 = 2u + 6u * prm.depth;
-                    float u6[6];  // light uv, lobe, BSDF uv, roulette
-                    if (prm.sobol) {
-                        // sampler::next_2d is vec2(next_1d(), next_1d()) (sampler.h:27-29),
-                        // whose arguments the compiled reference evaluates right to
-                        // left: a 2D sample is (dimension d + 1, dimension d)
-                        const uint32_t ps = mix_bits(pix);
-                        u6[0] = ref_sample(prm.sobol, ps, smp, d0 + 1u);
-                        u6[1] = ref_sample(prm.sobol, ps, smp, d0);
-                        u6[2] = ref_sample(prm.sobol, ps, smp, d0 + 2u);
-                        u6[3] = ref_sample(prm.sobol, ps, smp, d0 + 4u);
-                        u6[4] = ref_sample(prm.sobol, ps, smp, d0 + 3u);
-                        u6[5] = ref_sample(prm.sobol, ps, smp, d0 + 5u);
-                    } else {
-                        const uint4 A = philox(make_uint4(pix, smp, d0 >> 2, 0u), prm.key);
-                        const uint4 B = philox(make_uint4(pix, smp, (d0 >> 2) + 1u, 0u), prm.key);
-#pragma unroll
-                        for (int k = 0; k < 6; ++k) {
-                            const uint32_t d = d0 + (uint32_t)k;
-                            u6[k] = u24((d >> 2) == (d0 >> 2) ? pick(A, d & 3u) : pick(B, d & 3u));
-                        }
-                    }
-                    ctx.lobe = u6[2];
-                    const uint32_t fl = bsdf_flags(M, sh.material);
-                    if (fl & (F_DIFFUSE | F_GLOSSY)) {  // NEE (path_integrator.h:86-128)
-                        DirectSample ds;
-                        if (sample_emitter_direction(S, Ls, sh.p, u6[0], u6[1], ds)) {
-                            so = sh.p + sh.n * 1e-4f;
-                            sd = ds.dir;
-                            tmax = kInfF;
-                            bool target = true;
-                            if (!isinf(ds.dist)) {
-                                const f3 to = ds.point - so;
-                                const float dist = len(to);
-                                if (dist < 1e-6f) target = false;
-                                else { sd = to / dist; tmax = dist * (1.0f - 1e-4f); }
-                            }
-                            if (target) {
-                                sr.wo = to_local(fr, sd);
-                                const EP e = bsdf_ep(M, sh.material, ctx, sr, M_SOLID);
-                                if (!is_zero(e.f)) {
-                                    const float lp = pdf_emitter_direction<MESH_LIGHTS>(S, Ls, sh.p, sd);
-                                    if (lp > 0.0f) {
-                                        const float w = power_heuristic(lp, e.pdf);
-                                        contrib = T * e.f * ds.radiance * (w / lp);
-                                        ignore = ds.emitter;
-                                        shadow = true;
-                                    }
-                                }
-                            }
-                        }
-                    }
-                    if (prm.depth + 1 < prm.max_depth) {  // last bounce does NEE only
-                        const f3 bw = bsdf_sample(M, sh.material, ctx, sr, u6[3], u6[4]);
-                        if (!is_zero(bw)) {
-                            const f3 nd = to_world(fr, sr.wo);
-                            T = T * bw;
-                            eta *= sr.eta;
-                            const bool hp = ((sr.type & F_SPECULAR) == 0) && sr.pdf > 0.0f;
-                            cont = true;
-                            if (prm.depth >= prm.rr_depth) {  // Russian roulette (path_integrator.h:155-166)
-                                const float q = fminf(fmaxf(fmaxf(T.x, T.y), T.z) * eta * eta, 0.95f);
-                                if (u6[5] >= q) cont = false;
-                                else T = f3{T.x / q, T.y / q, T.z / q};
-                            }
-                            if (cont) {
-                                pw[0] = word(sh.p, eta);
-                                pw[1] = word(nd, hp ? sr.pdf : -1.0f);
-                                pw[2] = word(T, 0.0f);
-                            }
-                        }
-                    }
-                }
-            }
-            pw[3] = word(L, 0.0f);
+            shade_path<MESH_LIGHTS>(S, Ls, M, P, p, hit_load(Hb, i), prm, o);
         }
-        const uint32_t slot = wave_append(&counters[1], cont);
-        if (cont) next_queue[slot] = p;
-        const uint32_t ss = wave_append(&counters[2], shadow);
-        if (shadow) {
-            Sb.ox[ss] = so.x; Sb.oy[ss] = so.y; Sb.oz[ss] = so.z;
-            Sb.dx[ss] = sd.x; Sb.dy[ss] = sd.y; Sb.dz[ss] = sd.z;
-            Sb.tmax[ss] = tmax;
-            Sb.cr[ss] = contrib.x; Sb.cg[ss] = contrib.y; Sb.cb[ss] = contrib.z;
-            Sb.ignore[ss] = ignore;
+        const uint32_t slot = wave_append(&counters[1], o.cont);
+        if (o.cont) next_queue[slot] = p;
+        const uint32_t ss = wave_append(&counters[2], o.shadow);
+        if (o.shadow) {
+            Sb.ox[ss] = o.so.x; Sb.oy[ss] = o.so.y; Sb.oz[ss] = o.so.z;
+            Sb.dx[ss] = o.sd.x; Sb.dy[ss] = o.sd.y; Sb.dz[ss] = o.sd.z;
+            Sb.tmax[ss] = o.tmax;
+            Sb.cr[ss] = o.contrib.x; Sb.cg[ss] = o.contrib.y; Sb.cb[ss] = o.contrib.z;
+            Sb.ignore[ss] = o.ignore;
             Sb.path[ss] = p;
         }
     }
+}
+
+// The last bounces of a pass once few paths are left (0 < counters[0] <=
+// limit, else it returns at once): each lane takes a path at its hit and
+// finishes it, bounce after bounce (shade_path, its shadow ray, its next ray),
+// so the pass does not pay one launch tail (the slowest ray of a launch) per
+// remaining bounce.  The arithmetic and the order of the radiance additions
+// are the wavefront's (emission, then the shadow ray's contribution, per
+// bounce).  Extension and shadow rays traced are added to stats[0] / stats[1];
+// k_tail_done then empties the queue for the remaining wavefront launches.
+template <bool MESH_LIGHTS>
+__global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_tail(const DevScene S, const DevLights Ls, const Mats M, PathState P,
+                                                 const uint32_t* __restrict__ queue, const uint32_t* __restrict__ counters,
+                                                 const HitBuf Hb, ShadeParams prm, uint32_t limit,
+                                                 unsigned long long* __restrict__ stats) {
+    const uint32_t n = counters[0];
+    if (n == 0 || n > limit) return;
+    __shared__ uint2 stack[kLdsStack * kBlock];
+    uint2* lds = stack + threadIdx.x;
+    uint32_t ext = 0, shd = 0;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        const uint32_t p = queue[j];
+        HitRec h = hit_load(Hb, j);
+        ++ext;
+        for (uint32_t d = prm.depth; d < prm.max_depth; ++d) {
+            ShadeParams sp = prm;
+            sp.depth = d;
+            ShadeOut o;
+            shade_path<MESH_LIGHTS>(S, Ls, M, P, p, h, sp, o);
+            if (o.shadow) {
+                ++shd;
+                if (!trace_occluded(S, Ray{o.so, o.sd}, 0.001f, o.tmax, o.ignore, lds)) {
+                    float4* w = path_words(P, p);
+                    w[3] = word(xyz(w[3]) + o.contrib, 0.0f);
+                }
+            }
+            if (!o.cont) break;
+            const float4* w = path_words(P, p);
+            HitInfo hi;
+            Counters c{0, 0, 0};
+            trace_closest<false>(S, Ray{xyz(w[0]), xyz(w[1])}, 0.001f, kInfF, hi, lds, c);
+            h = HitRec{hi.obj, hi.prim, hi.t, hi.u, hi.v};
+            ++ext;
+        }
+    }
+    // one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        ext += __shfl_xor(ext, off);
+        shd += __shfl_xor(shd, off);
+    }
+    if ((threadIdx.x & 63u) == 0u) {
+        if (ext) atomicAdd(&stats[0], (unsigned long long)ext);
+        if (shd) atomicAdd(&stats[1], (unsigned long long)shd);
+    }
+}
+
+// after k_tail: a queue it finished is emptied
+__global__ void k_tail_done(uint32_t* counters, uint32_t limit) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && counters[0] <= limit) counters[0] = 0;
 }
 
 

@@ -93,6 +93,7 @@ class RenderParams(C.Structure):
 
 RENDER_COUNT = 1
 RENDER_REFERENCE_SAMPLER = 2
+RENDER_WAVEFRONT_ONLY = 8
 
 
 class RenderStats(C.Structure):
@@ -584,26 +585,28 @@ class Scene:
         return ms.value, nl.value, it.value
 
     def render(self, width, height, spp, max_depth, rr_depth, seed=1, spp_offset=0, rows=None, sumsq=True,
-               sum_rgb=None, sumsq_rgb=None, reference_sampler=False):
+               sum_rgb=None, sumsq_rgb=None, reference_sampler=False, wavefront_only=False):
         """Adds spp samples per pixel into (sum, sumsq) float64 buffers [H, W, 3]
         (row 0 = bottom, like src/main.cpp:122) and returns them with stats.
-        reference_sampler: the reference's Sobol sampler instead of Philox."""
+        reference_sampler: the reference's Sobol sampler instead of Philox.
+        wavefront_only: no tail kernel (every bounce a wavefront launch)."""
         if sum_rgb is None:
             sum_rgb = np.zeros((height, width, 3), np.float64)
         if sumsq and sumsq_rgb is None:
             sumsq_rgb = np.zeros((height, width, 3), np.float64)
         rb, re = rows if rows is not None else (0, 0)
         prm = RenderParams(width, height, spp, spp_offset, max_depth, rr_depth, seed, rb, re,
-                           RENDER_REFERENCE_SAMPLER if reference_sampler else 0)
+                           (RENDER_REFERENCE_SAMPLER if reference_sampler else 0) |
+                           (RENDER_WAVEFRONT_ONLY if wavefront_only else 0))
         st = RenderStats()
         _check(LIB.lrt_render(self._h, C.byref(prm), _p(sum_rgb, C.c_double),
                               _p(sumsq_rgb, C.c_double) if sumsq else None, 0, C.byref(st)))
         return sum_rgb, sumsq_rgb, st
 
     def render_device(self, width, height, spp, max_depth, rr_depth, sum_ptr, sumsq_ptr=0, seed=1, spp_offset=0,
-                      count=False):
+                      count=False, wavefront_only=False):
         prm = RenderParams(width, height, spp, spp_offset, max_depth, rr_depth, seed, 0, 0,
-                           RENDER_COUNT if count else 0)
+                           (RENDER_COUNT if count else 0) | (RENDER_WAVEFRONT_ONLY if wavefront_only else 0))
         st = RenderStats()
         _check(LIB.lrt_render(self._h, C.byref(prm), C.c_void_p(sum_ptr), C.c_void_p(sumsq_ptr) if sumsq_ptr else None,
                               1, C.byref(st)))

@@ -86,6 +86,29 @@ def test_box_icosphere_vs_reference_render():
     check_z(z, "box_icosphere vs reference render", mean_bar=0.15, var_range=(0.5, 1.1))
 
 
+@pytest.mark.parametrize("case", ["box", "sphere"])
+def test_tail_kernel_matches_wavefront(case):
+    """A pass's last bounces finished by the tail kernel (one lane per path
+    once at most 2^18 paths are left, from the Russian-roulette depth on)
+    give the same per-pixel sums, bit for bit, and the same ray counts as
+    every bounce run as a wavefront launch: the box room with all its 131,072
+    paths in the tail from depth 3, and the displaced sphere at 480x270x4
+    (518,400 paths) where the tail starts at a later bounce."""
+    if case == "box":
+        g = linrt.box_icosphere(seed=1, subdivisions=4)
+        W, H, spp, md, rr = 128, 128, 8, 8, 3
+    else:
+        g = linrt.displaced_sphere(seed=1, segments=256, rings=128, aspect=480 / 270)
+        W, H, spp, md, rr = 480, 270, 4, 16, 5
+    sc = linrt.Scene(g.desc_ptr)
+    s_t, q_t, st_t = sc.render(W, H, spp, md, rr, seed=3)
+    s_w, q_w, st_w = sc.render(W, H, spp, md, rr, seed=3, wavefront_only=True)
+    print(f"{case}: extension rays {st_t.extension_rays} / {st_w.extension_rays}, shadow {st_t.shadow_rays} / "
+          f"{st_w.shadow_rays}, {st_t.seconds * 1e3:.1f} ms with the tail, {st_w.seconds * 1e3:.1f} ms without")
+    assert st_t.extension_rays == st_w.extension_rays and st_t.shadow_rays == st_w.shadow_rays
+    assert np.array_equal(s_t, s_w) and np.array_equal(q_t, q_w)
+
+
 def test_displaced_sphere_crop():
     """Identical streams on a crop of the 1080p workload's scene (65k-triangle
     mesh, seven BSDF regions, sky)."""
