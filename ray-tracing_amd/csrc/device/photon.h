@@ -317,6 +317,48 @@ LRT_D uint64_t slots_max(const uint64_t (&key)[S]) {
     return m;
 }
 
+// The 8 S keys of a query's group sorted in descending order, in place:
+// afterwards slot s of group lane l holds the (8 s + l)-th largest key (slot
+// major, so the 8 lanes of a group hold the 8 entries of a summing batch).
+// A bitonic network over the 8 S elements, element e = 8 s + l: partners 8
+// or more apart sit in the same lane (two slots), nearer ones in two lanes
+// of the group (one exchange per slot).
+template <int S>
+LRT_D void group_sort_desc(uint64_t (&key)[S], int lane) {
+    const int gl = lane & (kGroup - 1);
+    constexpr int N = kGroup * S;
+#pragma unroll
+    for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j >= kGroup) {
+                const int js = j / kGroup;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (s & js) continue;
+                    const bool desc = ((s * kGroup + gl) & k) == 0;
+                    const uint64_t a = key[s], b = key[s | js];
+                    const bool sw = desc ? (b > a) : (a > b);
+                    key[s] = sw ? b : a;
+                    key[s | js] = sw ? a : b;
+                }
+            } else {
+                const int src = lane ^ j;
+                const bool lo = (gl & j) == 0;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    const bool desc = ((s * kGroup + gl) & k) == 0;
+                    const uint64_t o = ((uint64_t)lane_read((uint32_t)(key[s] >> 32), src) << 32) |
+                                       lane_read((uint32_t)key[s], src);
+                    const bool take_max = desc == lo;
+                    const bool other = take_max ? (o > key[s]) : (o < key[s]);
+                    key[s] = other ? o : key[s];
+                }
+            }
+        }
+    }
+}
+
 // k nearest photons within r of p (photon_map.h:117-165) and the irradiance
 // estimate over them (photon_map.h:38-70), computed by the 8 lanes of a group
 // together; every lane returns the estimate.
@@ -393,21 +435,18 @@ LRT_D f3 photon_estimate_group(const PhotonMapDev& Mp, f3 p, f3 n, int max_depth
         node = left_near ? nd.z : nd.w;
     }
     if (counter) c.tris += (uint32_t)size;
-    // the set in descending key order, 8 entries at a time: entry b + j goes
-    // to lane j, the 8 lanes load and weigh their photons together, and the
-    // terms are added in order
+    // the set in descending key order (the heap's pop order), 8 entries at a
+    // time: after the sort entry 8 s + j is slot s of lane j, the 8 lanes load
+    // and weigh their photons together, and the terms are added in order
     const float inv_r2 = 1.0f / r2;
     f3 sum = splat(0.0f);
-    for (int b = 0; b < size; b += kGroup) {
-        const int cnt = min(kGroup, size - b);
-        uint64_t mine = 0;
-        for (int j = 0; j < cnt; ++j) {
-            const uint64_t mk = group_max(slots_max(key), lane);
+    group_sort_desc(key, lane);
 #pragma unroll
-            for (int s = 0; s < S; ++s)
-                if (key[s] == mk) key[s] = 0;
-            if (gl == j) mine = mk;
-        }
+    for (int sb = 0; sb < S; ++sb) {
+        const int b = sb * kGroup;
+        if (b >= size) break;
+        const int cnt = min(kGroup, size - b);
+        const uint64_t mine = key[sb];
         f3 term = splat(0.0f);
         uint32_t use = 0;
         if (gl < cnt) {
