@@ -595,13 +595,27 @@ LRT_D bool lane_trace_round(const DevScene& S, LaneTrace& T, uint2* lds, uint2* 
     return false;
 }
 
-// Wave-level work distribution for persistent kernels: when at least kRefill
-// lanes are idle (or all are), the wave takes that many consecutive items
-// from *next with one atomic; returns the lane's new item or UINT32_MAX.
-// `more` turns false once the items are exhausted (uniform over the wave).
-LRT_D uint32_t wave_refill(bool idle_lane, uint32_t* next, uint32_t n, bool& more) {
+// Wave-level work distribution for persistent kernels: when at least
+// kRefill lanes are idle (or all are), the wave takes that many consecutive
+// items from *next with one atomic.  Returns the lane's new item or
+// UINT32_MAX; `more` turns 0 once the items are exhausted (uniform over the
+// wave).  STATIC_FIRST (the path tracer's and eye pass's queues, often short
+// or empty): the first call (more == 2, every lane idle) hands lane g of the
+// grid item g without an atomic and later items are the grid's lane count +
+// the counter's value, so a launch over an empty or short queue costs no
+// atomic per wave (20k waves on one address took ~0.2 ms per launch).  The
+// batch ray queries (the host knows their ray count) keep the plain form,
+// which needs fewer registers.
+template <bool STATIC_FIRST = true>
+LRT_D uint32_t wave_refill(bool idle_lane, uint32_t* next, uint32_t n, int& more) {
     const uint32_t none = 0xffffffffu;
     if (!more) return none;
+    const uint32_t lanes = STATIC_FIRST ? gridDim.x * blockDim.x : 0u;
+    if (STATIC_FIRST && more == 2) {
+        more = lanes < n ? 1 : 0;
+        const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+        return idle_lane && g < n ? g : none;
+    }
     const uint64_t idle = __ballot(idle_lane);
     const int nidle = __popcll(idle);
     if (nidle < kRefill && nidle < (int)__popcll(__ballot(true))) return none;
@@ -609,8 +623,8 @@ LRT_D uint32_t wave_refill(bool idle_lane, uint32_t* next, uint32_t n, bool& mor
     const int leader = __ffsll((long long)idle) - 1;
     uint32_t base = 0;
     if (lane == leader) base = atomicAdd(next, (uint32_t)nidle);
-    base = (uint32_t)__shfl((int)base, leader);
-    if ((uint64_t)base + (uint64_t)nidle >= n) more = false;
+    base = lanes + (uint32_t)__shfl((int)base, leader);  // n < 2^31: no wrap before more turns 0
+    if (base + (uint32_t)nidle >= n) more = 0;
     if (!idle_lane) return none;
     const uint32_t i = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
     return i < n ? i : none;

@@ -296,7 +296,8 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_diff_shadow(const Dev
     long long gs[3] = {0, 0, 0};
     LaneTrace T;
     uint32_t i = 0;
-    bool active = false, more = true;
+    bool active = false;
+    int more = 2;  // wave_refill: first fill pending
     for (;;) {
         const uint32_t got = wave_refill(!active, &counters[5], n, more);
         if (got != 0xffffffffu) {

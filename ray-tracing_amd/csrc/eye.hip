@@ -90,7 +90,8 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_eye_extend(const DevS
     uint2 spill[kMaxStack - kLdsStack];
     LaneTrace T;
     uint32_t i = 0;
-    bool active = false, more = true;
+    bool active = false;
+    int more = 2;  // wave_refill: first fill pending
     for (;;) {
         const uint32_t got = wave_refill(!active, next, n, more);
         if (got != 0xffffffffu) {
@@ -328,7 +329,8 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_eye_shadow(const DevS
     uint2 spill[kMaxStack - kLdsStack];
     LaneTrace T;
     uint32_t i = 0;
-    bool active = false, more = true;
+    bool active = false;
+    int more = 2;  // wave_refill: first fill pending
     for (;;) {
         const uint32_t got = wave_refill(!active, next, n, more);
         if (got != 0xffffffffu) {

@@ -41,9 +41,10 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_trace_closest(const D
     uint2 spill[kMaxStack - kLdsStack];
     LaneTrace T;
     uint32_t i = 0;
-    bool active = false, more = true;
+    bool active = false;
+    int more = 1;
     for (;;) {
-        const uint32_t got = wave_refill(!active, next, n, more);
+        const uint32_t got = wave_refill<false>(!active, next, n, more);
         if (got != 0xffffffffu) {
             i = got;
             lane_trace_start(T, Ray{f3{rays[i], rays[n + i], rays[2 * (size_t)n + i]},
@@ -96,9 +97,10 @@ __global__ __launch_bounds__(kBlock) LRT_TRACE_ATTR void k_trace_occluded(const 
     uint2 spill[kMaxStack - kLdsStack];
     LaneTrace T;
     uint32_t i = 0;
-    bool active = false, more = true;
+    bool active = false;
+    int more = 1;
     for (;;) {
-        const uint32_t got = wave_refill(!active, next, n, more);
+        const uint32_t got = wave_refill<false>(!active, next, n, more);
         if (got != 0xffffffffu) {
             i = got;
             lane_shadow_start(S, T, Ray{f3{rays[i], rays[n + i], rays[2 * (size_t)n + i]},
@@ -222,7 +224,8 @@ __global__ __launch_bounds__(kBlock) LRT_PATH_TRACE_ATTR void k_extend(const Dev
     const uint32_t n = counters[0];
     LaneTrace T;
     uint32_t i = 0;
-    bool active = false, more = true;
+    bool active = false;
+    int more = 2;  // wave_refill: first fill pending
     for (;;) {
         const uint32_t got = wave_refill(!active, &counters[4], n, more);
         if (got != 0xffffffffu) {
@@ -703,7 +706,8 @@ __global__ __launch_bounds__(kBlock) LRT_PATH_TRACE_ATTR void k_shadow(const Dev
     const uint32_t n = counters[2];
     LaneTrace T;
     uint32_t i = 0;
-    bool active = false, more = true;
+    bool active = false;
+    int more = 2;  // wave_refill: first fill pending
     for (;;) {
         const uint32_t got = wave_refill(!active, &counters[5], n, more);
         if (got != 0xffffffffu) {
@@ -743,7 +747,8 @@ __global__ __launch_bounds__(kBlock) LRT_PATH_TRACE_ATTR void k_trace_queues(con
     const uint32_t ns = counters[2], n = ns + counters[1];
     LaneTrace T;
     uint32_t i = 0;
-    bool active = false, more = true;
+    bool active = false;
+    int more = 2;  // wave_refill: first fill pending
     for (;;) {
         const uint32_t got = wave_refill(!active, &counters[5], n, more);
         if (got != 0xffffffffu) {
