@@ -487,6 +487,13 @@ int lrt_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out
  * normal (3, 0 on a miss) and hit distance (1, 0 on a miss); 7 floats per
  * pixel, rows bottom-up like the radiance.  device_ptrs: aov is device memory. */
 int lrt_render_aov(lrt_scene* scene, const lrt_render_params* params, float* aov, int device_ptrs);
+/* lrt_render and lrt_render_aov in one pass (no counterpart in the reference):
+ * the features come from the render's own camera rays, found by its first
+ * bounce, instead of a second trace of them; for Philox renders both outputs
+ * equal the two separate calls' bit for bit (with LRT_RENDER_REFERENCE_SAMPLER
+ * the features follow that sampler's camera rays) */
+int lrt_render_features(lrt_scene* scene, const lrt_render_params* params, double* sum_rgb, double* sumsq_rgb,
+                        float* aov, int device_ptrs, lrt_render_stats* stats);
 
 /* ---- differentiable rendering (no counterpart in the reference) ---------
  * The path tracer of lrt_render as one kernel thread per pixel (identical

@@ -315,6 +315,7 @@ struct lrt_scene {
     // path-tracer work buffers
     size_t cap = 0;
     DevBuf pstate, hits, shadow, queues, counters, stats, accum, accum2, totals;
+    DevBuf aov_path, aov_dev;  // lrt_render_features: per-path first-hit features, a device copy of the output
     // ray-query staging
     DevBuf q_rays, q_out0, q_out1, q_out2, q_aux;
     // photon mapping
@@ -1088,8 +1089,11 @@ struct RayCapture {
     uint32_t* fill;  // 2 counters
 };
 
+// aov_out (lrt_render_features): also the first-hit features of the camera
+// rays, 7 floats per pixel of the frame (host or device memory as the sums)
 static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum_rgb, double* sumsq_rgb,
-                        int device_ptrs, lrt_render_stats* stats, const RayCapture* capture = nullptr) {
+                        int device_ptrs, lrt_render_stats* stats, const RayCapture* capture = nullptr,
+                        float* aov_out = nullptr) {
     {
         require(sc && prm && sum_rgb, "null pointer");
         const uint32_t W = prm->width, H = prm->height;
@@ -1156,6 +1160,18 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         const unsigned long long zero[4] = {0, 0, 0, 0};
         HIPCHECK(hipMemcpyAsync(dstats, zero, sizeof(zero), hipMemcpyHostToDevice, sc->stream));
         const bool counting = (prm->flags & LRT_RENDER_COUNT) != 0;
+        float* aov_path = nullptr;
+        float* daov = aov_out;
+        const size_t aov_bytes = (size_t)W * H * 7 * 4;
+        if (aov_out) {
+            sc->aov_path.alloc((size_t)n * 7 * 4);
+            aov_path = sc->aov_path.as<float>();
+            if (!device_ptrs) {
+                sc->aov_dev.alloc(aov_bytes);
+                daov = sc->aov_dev.as<float>();
+                HIPCHECK(hipMemcpyAsync(daov, aov_out, aov_bytes, hipMemcpyHostToDevice, sc->stream));
+            }
+        }
         sc->totals.alloc(256);
         // traversal work: reference extension, reference shadow, own extension, own shadow
         const unsigned long long zero12[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -1200,7 +1216,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
             hipLaunchKernelGGL(k_extend, dim3(g_trace), dim3(kBlock), 0, sc->stream, sc->S, P, q, counters, Hb);
             HIPCHECK(hipEventRecord(e0.second, sc->stream));
             for (uint32_t depth = 0; depth < prm->max_depth; ++depth) {
-                ShadeParams sp{depth, prm->max_depth, prm->rr_depth, smp, key, sobol};
+                ShadeParams sp{depth, prm->max_depth, prm->rr_depth, smp, key, sobol, aov_path};
                 auto e2 = ts.next();
                 HIPCHECK(hipEventRecord(e2.first, sc->stream));
                 if (tail && depth >= prm->rr_depth) {  // few paths left: finish them in one launch
@@ -1234,8 +1250,12 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
                 std::swap(q, qn);
             }
             hipLaunchKernelGGL(k_accumulate, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, P, ns, dsum, dsq);
+            if (aov_path)
+                hipLaunchKernelGGL(k_aov_resolve, dim3((npix + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, P, ns,
+                                   aov_path, daov, s == 0 ? 1 : 0, s + ns >= prm->spp ? 1.0f / (float)prm->spp : 0.0f);
             HIPCHECK(hipGetLastError());
         }
+        if (aov_out && !device_ptrs) HIPCHECK(hipMemcpyAsync(aov_out, daov, aov_bytes, hipMemcpyDeviceToHost, sc->stream));
         unsigned long long hst[4] = {0, 0, 0, 0}, work[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         HIPCHECK(hipMemcpyAsync(hst, dstats, sizeof(hst), hipMemcpyDeviceToHost, sc->stream));
         HIPCHECK(hipMemcpyAsync(work, sc->totals.p, sizeof(work), hipMemcpyDeviceToHost, sc->stream));
@@ -1282,6 +1302,14 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
 int lrt_render(lrt_scene* sc, const lrt_render_params* prm, double* sum_rgb, double* sumsq_rgb, int device_ptrs,
                lrt_render_stats* stats) {
     return guarded([&] { render_impl(sc, prm, sum_rgb, sumsq_rgb, device_ptrs, stats); });
+}
+
+int lrt_render_features(lrt_scene* sc, const lrt_render_params* prm, double* sum_rgb, double* sumsq_rgb, float* aov,
+                        int device_ptrs, lrt_render_stats* stats) {
+    return guarded([&] {
+        require(aov != nullptr, "null pointer");
+        render_impl(sc, prm, sum_rgb, sumsq_rgb, device_ptrs, stats, nullptr, aov);
+    });
 }
 
 int lrt_render_capture_rays(lrt_scene* sc, const lrt_render_params* prm, float* ext_rays, uint32_t ext_cap,

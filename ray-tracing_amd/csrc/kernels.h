@@ -131,6 +131,7 @@ struct ShadeParams {
     uint32_t depth, max_depth, rr_depth, sample;
     uint2 key;
     const uint32_t* sobol;  // non-null: the reference's Sobol sampler (direction numbers), else Philox
+    float* aov_path;        // non-null: at depth 0 each path's first-hit features, 7 floats (k_aov_resolve)
 };
 
 }  // namespace lrt

@@ -399,6 +399,33 @@ __global__ __launch_bounds__(kBlock) void k_shade_order(const DevScene S, const 
     }
 }
 
+// first-hit albedo for the denoiser's features: diffuse colour, the
+// normal-incidence Fresnel reflectance of a conductor, 1 for dielectrics
+LRT_D f3 albedo_of(const Mats& M, int m) {
+    const lrt_material& A = M.m[m];
+    switch (A.type) {
+        case LRT_BSDF_DIFFUSE:
+        case LRT_BSDF_PLASTIC:
+        case LRT_BSDF_ROUGH_PLASTIC: return col(A.diffuse);
+        case LRT_BSDF_CONDUCTOR: {
+            f3 f;
+            float* o = &f.x;
+            for (int c = 0; c < 3; ++c) {
+                const float e = A.cond_eta[c], k = A.cond_k[c];
+                o[c] = ((e - 1.0f) * (e - 1.0f) + k * k) / ((e + 1.0f) * (e + 1.0f) + k * k) * A.specular_reflectance[c];
+            }
+            return f;
+        }
+        case LRT_BSDF_COATING:
+        case LRT_BSDF_ROUGH_COATING: {
+            const lrt_material& N = M.m[A.nested];
+            if (N.type == LRT_BSDF_DIFFUSE || N.type == LRT_BSDF_PLASTIC || N.type == LRT_BSDF_ROUGH_PLASTIC) return col(N.diffuse);
+            return splat(1.0f);
+        }
+        default: return splat(1.0f);
+    }
+}
+
 // What one bounce of a path leaves for the traversal: whether the path goes
 // on (its next ray is in its record) and its next-event shadow ray with the
 // contribution the ray adds if unoccluded.
@@ -430,7 +457,13 @@ LRT_D void shade_path(const DevScene& S, const DevLights& Ls, const Mats& M, con
     const float sa_pdf = w1.w;
     const bool have = sa_pdf >= 0.0f;
     const int obj = h.obj;
+    float* aov = prm.depth == 0 ? prm.aov_path : nullptr;  // first-hit features, as k_aov computes them
     if (obj < 0) {  // escape: environment with MIS (path_integrator.h:32-45)
+        if (aov) {
+            float* o = aov + 7 * (size_t)p;
+            o[0] = o[1] = o[2] = 1.0f;
+            o[3] = o[4] = o[5] = o[6] = 0.0f;
+        }
         const f3 Le = env_eval(Ls, r.d);
         if (!have) {
             L = L + T * Le;
@@ -440,6 +473,13 @@ LRT_D void shade_path(const DevScene& S, const DevLights& Ls, const Mats& M, con
         }
     } else {
         const SurfaceHit sh = surface(S, r, obj, h.prim, h.t, h.u, h.v);
+        if (aov) {
+            const f3 al = sh.material >= 0 ? albedo_of(M, sh.material) : splat(1.0f);
+            float* o = aov + 7 * (size_t)p;
+            o[0] = al.x; o[1] = al.y; o[2] = al.z;
+            o[3] = sh.n.x; o[4] = sh.n.y; o[5] = sh.n.z;
+            o[6] = h.t;
+        }
         if (sh.emitter >= 0 && sh.front) {  // path_integrator.h:47-59
             const f3 out = -normalize(r.d);
             const f3 Le = dot(sh.n, out) <= 0.0f ? splat(0.0f) : ld3(Ls.l[sh.emitter].radiance);
@@ -632,33 +672,6 @@ __global__ void k_tail_done(uint32_t* counters, uint32_t limit) {
 }
 
 
-// first-hit albedo for the denoiser's features: diffuse colour, the
-// normal-incidence Fresnel reflectance of a conductor, 1 for dielectrics
-LRT_D f3 albedo_of(const Mats& M, int m) {
-    const lrt_material& A = M.m[m];
-    switch (A.type) {
-        case LRT_BSDF_DIFFUSE:
-        case LRT_BSDF_PLASTIC:
-        case LRT_BSDF_ROUGH_PLASTIC: return col(A.diffuse);
-        case LRT_BSDF_CONDUCTOR: {
-            f3 f;
-            float* o = &f.x;
-            for (int c = 0; c < 3; ++c) {
-                const float e = A.cond_eta[c], k = A.cond_k[c];
-                o[c] = ((e - 1.0f) * (e - 1.0f) + k * k) / ((e + 1.0f) * (e + 1.0f) + k * k) * A.specular_reflectance[c];
-            }
-            return f;
-        }
-        case LRT_BSDF_COATING:
-        case LRT_BSDF_ROUGH_COATING: {
-            const lrt_material& N = M.m[A.nested];
-            if (N.type == LRT_BSDF_DIFFUSE || N.type == LRT_BSDF_PLASTIC || N.type == LRT_BSDF_ROUGH_PLASTIC) return col(N.diffuse);
-            return splat(1.0f);
-        }
-        default: return splat(1.0f);
-    }
-}
-
 // per-pixel average over the frame's camera rays of (albedo, normal, depth)
 __global__ __launch_bounds__(kBlock) void k_aov(const DevScene S, const Mats M, const CameraDev cam, uint32_t W,
                                                 uint32_t H, uint32_t row0, uint32_t n, uint32_t spp, uint32_t spp0,
@@ -695,6 +708,28 @@ __global__ __launch_bounds__(kBlock) void k_aov(const DevScene S, const Mats M, 
     }
 }
 
+
+// per pixel of a pass, its samples' first-hit features (written by shading
+// at depth 0) added in sample order to aov (from zero when `first`), and
+// scaled by `scale` when it is not zero (the last pass: 1 / spp): the sums and
+// the rounding of k_aov over the same camera rays
+__global__ __launch_bounds__(kBlock) void k_aov_resolve(const PathState P, uint32_t slots,
+                                                        const float* __restrict__ aov_path, float* __restrict__ aov,
+                                                        int first, float scale) {
+    const uint32_t ip = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ip >= P.npix) return;
+    float* o = aov + 7 * ((size_t)P.pixel0 + ip);
+    float a[7];
+#pragma unroll
+    for (int c = 0; c < 7; ++c) a[c] = first ? 0.0f : o[c];
+    for (uint32_t s = 0; s < slots; ++s) {
+        const float* v = aov_path + 7 * ((size_t)s * P.npix + ip);
+#pragma unroll
+        for (int c = 0; c < 7; ++c) a[c] += v[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 7; ++c) o[c] = scale != 0.0f ? a[c] * scale : a[c];
+}
 
 // shadow rays: unoccluded ones add their contribution to the path radiance.
 // Persistent like k_extend (rays fetched from counters[5]).

@@ -160,7 +160,7 @@ EXPORTED = [
     "lrt_photon_build", "lrt_photon_export", "lrt_photon_map_load", "lrt_photon_estimate", "lrt_render_photon",
     "lrt_generate_scene", "lrt_generated_desc", "lrt_generated_free", "lrt_philox4x32_10", "lrt_render_aov",
     "lrt_denoiser_create", "lrt_denoiser_release", "lrt_denoiser_param_count", "lrt_denoiser_params",
-    "lrt_denoiser_forward", "lrt_denoiser_loss_grad", "lrt_denoiser_adam_step", "lrt_conv3x3", "lrt_denoiser_timing",
+    "lrt_denoiser_forward", "lrt_denoiser_loss_grad", "lrt_denoiser_adam_step", "lrt_conv3x3", "lrt_denoiser_timing", "lrt_render_features",
     "lrt_diff_param_count", "lrt_diff_render", "lrt_diff_adjoint", "lrt_diff_render_held",
     "lrt_load_obj", "lrt_mesh_view", "lrt_mesh_free", "lrt_generate_obj_scene", "lrt_image_write",
     "lrt_image_to_bytes", "lrt_render_capture_rays", "lrt_trace_closest_noalloc", "lrt_sky_cook",
@@ -602,6 +602,18 @@ class Scene:
         _check(LIB.lrt_render(self._h, C.byref(prm), _p(sum_rgb, C.c_double),
                               _p(sumsq_rgb, C.c_double) if sumsq else None, 0, C.byref(st)))
         return sum_rgb, sumsq_rgb, st
+
+    def render_features_device(self, width, height, spp, max_depth, rr_depth, sum_ptr, aov_ptr, sumsq_ptr=0, seed=1,
+                               spp_offset=0, rows=None):
+        """lrt_render_features on device buffers: radiance sums [H, W, 3] float64
+        and the first-hit features [H, W, 7] float32 of the same camera rays."""
+        rb, re = rows if rows is not None else (0, 0)
+        prm = RenderParams(width, height, spp, spp_offset, max_depth, rr_depth, seed, rb, re, 0)
+        st = RenderStats()
+        _check(LIB.lrt_render_features(self._h, C.byref(prm), C.c_void_p(sum_ptr),
+                                       C.c_void_p(sumsq_ptr) if sumsq_ptr else None, C.c_void_p(aov_ptr), 1,
+                                       C.byref(st)))
+        return st
 
     def render_device(self, width, height, spp, max_depth, rr_depth, sum_ptr, sumsq_ptr=0, seed=1, spp_offset=0,
                       count=False, wavefront_only=False):

@@ -53,6 +53,18 @@ def radiance(scene, W, H, spp, seed, offset, torch_dev="cuda:0", rows=None):
     return (acc / spp).float()
 
 
+def radiance_features(scene, W, H, spp, seed, offset, torch_dev="cuda:0"):
+    """radiance() and features() of the same samples in one render pass
+    (lrt_render_features): the features come from the render's first bounce."""
+    torch = _torch()
+    acc = torch.zeros((H, W, 3), dtype=torch.float64, device=torch_dev)
+    aov = torch.zeros((H, W, 7), dtype=torch.float32, device=torch_dev)
+    scene.render_features_device(W, H, spp, MAX_DEPTH, RR_DEPTH, acc.data_ptr(), aov.data_ptr(), seed=seed,
+                                 spp_offset=offset)
+    scene.sync()
+    return (acc / spp).float(), aov
+
+
 def features(scene, W, H, spp, seed, offset, torch_dev="cuda:0", rows=None):
     torch = _torch()
     aov = torch.zeros((H, W, 7), dtype=torch.float32, device=torch_dev)
@@ -354,8 +366,7 @@ def time_to_quality(den=None, target=0.01, log=print, spp_list=(1, 2, 4), weight
         e = (x - ref) ** 2 / (ref * ref + EPS)
         return float(e[fin].mean())
 
-    r = radiance(sc, W, H, 1, 1, 0)
-    a = features(sc, W, H, 1, 1, 0)
+    r, a = radiance_features(sc, W, H, 1, 1, 0)
     den.forward_device(r.data_ptr(), a.data_ptr(), 1, H, W, out.data_ptr())
     torch.cuda.synchronize()
     res = {"oracle": {"file": oracle.name, "spp": int(d["spp"]), "accumulation": "float64",
@@ -365,8 +376,7 @@ def time_to_quality(den=None, target=0.01, log=print, spp_list=(1, 2, 4), weight
     for spp in spp_list:
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        r = radiance(sc, W, H, spp, 1, 0)
-        a = features(sc, W, H, spp, 1, 0)
+        r, a = radiance_features(sc, W, H, spp, 1, 0)
         den.forward_device(r.data_ptr(), a.data_ptr(), 1, H, W, out.data_ptr())
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0

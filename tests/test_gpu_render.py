@@ -109,6 +109,30 @@ def test_tail_kernel_matches_wavefront(case):
     assert np.array_equal(s_t, s_w) and np.array_equal(q_t, q_w)
 
 
+def test_render_features_matches_separate_calls():
+    """lrt_render_features (the first-hit features taken from the render's
+    own first bounce) equals lrt_render and lrt_render_aov called apart, bit
+    for bit: radiance sums of 3 samples per pixel and the features averaged
+    over the same 3 camera rays."""
+    import torch
+    g = linrt.displaced_sphere(seed=1, segments=256, rings=128, aspect=480 / 270)
+    sc = linrt.Scene(g.desc_ptr)
+    W, H, spp = 480, 270, 3
+    acc1 = torch.zeros((H, W, 3), dtype=torch.float64, device="cuda:0")
+    aov1 = torch.zeros((H, W, 7), dtype=torch.float32, device="cuda:0")
+    st = sc.render_features_device(W, H, spp, 16, 5, acc1.data_ptr(), aov1.data_ptr(), seed=5)
+    acc2 = torch.zeros_like(acc1)
+    aov2 = torch.zeros_like(aov1)
+    st2 = sc.render_device(W, H, spp, 16, 5, acc2.data_ptr(), seed=5)
+    sc.render_aov_device(W, H, spp, aov2.data_ptr(), seed=5)
+    sc.sync()
+    torch.cuda.synchronize()
+    print(f"fused: {st.seconds * 1e3:.1f} ms; extension rays {st.extension_rays} / {st2.extension_rays}")
+    assert st.extension_rays == st2.extension_rays
+    assert torch.equal(acc1, acc2)
+    assert torch.equal(aov1, aov2)
+
+
 def test_displaced_sphere_crop():
     """Identical streams on a crop of the 1080p workload's scene (65k-triangle
     mesh, seven BSDF regions, sky)."""
