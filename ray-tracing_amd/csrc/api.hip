@@ -1223,6 +1223,15 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
                     hipLaunchKernelGGL(sc->mesh_lights ? k_tail<true> : k_tail<false>, dim3(g_tail), dim3(kBlock), 0,
                                        sc->stream, sc->S, sc->Ls, sc->M, P, q, counters, Hb, sp, tail, dstats);
                     hipLaunchKernelGGL(k_tail_done, dim3(1), dim3(64), 0, sc->stream, counters, tail);
+                    // once the tail has finished the pass, the remaining bounces' launches would find
+                    // nothing: read the queue length back (one 4-byte copy) and end the pass
+                    uint32_t left = 0;
+                    HIPCHECK(hipMemcpyAsync(&left, counters, 4, hipMemcpyDeviceToHost, sc->stream));
+                    HIPCHECK(hipStreamSynchronize(sc->stream));
+                    if (left == 0) {
+                        HIPCHECK(hipEventRecord(e2.second, sc->stream));
+                        break;
+                    }
                 }
                 hipLaunchKernelGGL(k_shade_bins, dim3(g_bins), dim3(kBlock), 0, sc->stream, sc->S, Hb, counters);
                 hipLaunchKernelGGL(k_shade_scan, dim3(1), dim3(1024), 0, sc->stream, counters);
