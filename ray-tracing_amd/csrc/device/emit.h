@@ -29,14 +29,17 @@ struct DevLights {
     DevLight l[kMaxLights];
 };
 
-// ArHosekSkyModel.cc:291-304 and :637-651 in FP64 (published model formula)
-LRT_D double hosek(const double* c, double rad, double theta, double gamma) {
-    const double cg = cos(gamma);
+// ArHosekSkyModel.cc:291-304 and :637-651 in FP64 (published model formula),
+// one channel; cg = cos(gamma), ct = cos(theta), zenith = sqrt(ct) are the
+// same for the three channels and computed once.  The Mie term's x^1.5 is
+// x * sqrt(x) (one correctly rounded square root and one rounded product,
+// within an ulp of pow like the device pow itself; the channel's value is then
+// rounded to float)
+LRT_D double hosek(const double* c, double rad, double gamma, double cg, double ct, double zenith) {
     const double expM = exp(c[4] * gamma);
     const double rayM = cg * cg;
-    const double mieM = (1.0 + cg * cg) / pow((1.0 + c[8] * c[8] - 2.0 * c[8] * cg), 1.5);
-    const double ct = cos(theta);
-    const double zenith = sqrt(ct);
+    const double x = 1.0 + c[8] * c[8] - 2.0 * c[8] * cg;
+    const double mieM = (1.0 + cg * cg) / (x * sqrt(x));
     return (1.0 + c[0] * exp(c[1] / (ct + 0.01))) * (c[2] + c[3] * expM + c[5] * rayM + c[6] * mieM + c[7] * zenith) * rad;
 }
 
@@ -51,8 +54,9 @@ LRT_D f3 sky_eval(const DevLight& L, f3 direction) {
     // reference, and the NaN reaches the sample's radiance
     const float theta = (float)acos((double)fminf(fmaxf(dir.y, 0.0f), 1.0f));
     const float gamma = (float)acos((double)fminf(fmaxf(dot(dir, ld3(L.sun)), -1.0f), 1.0f));
-    return f3{(float)hosek(L.cfg[0], L.rad[0], theta, gamma), (float)hosek(L.cfg[1], L.rad[1], theta, gamma),
-              (float)hosek(L.cfg[2], L.rad[2], theta, gamma)} * L.scale;
+    const double g = gamma, cg = cos(g), ct = cos((double)theta), zenith = sqrt(ct);
+    return f3{(float)hosek(L.cfg[0], L.rad[0], g, cg, ct, zenith), (float)hosek(L.cfg[1], L.rad[1], g, cg, ct, zenith),
+              (float)hosek(L.cfg[2], L.rad[2], g, cg, ct, zenith)} * L.scale;
 }
 
 LRT_D f3 env_eval(const DevLights& Ls, f3 d) {
