@@ -318,6 +318,7 @@ def main():
     scene.kernel_timing(linrt.KERNEL_CLOSEST, reset=True)
     scene.kernel_timing(linrt.KERNEL_OCCLUDED, reset=True)
     scene.kernel_timing(linrt.KERNEL_SHADE, reset=True)
+    scene.kernel_timing(linrt.KERNEL_TAIL, reset=True)
 
     stats = []
     torch.cuda.synchronize()
@@ -335,11 +336,14 @@ def main():
     ms_c, n_c, _ = scene.kernel_timing(linrt.KERNEL_CLOSEST)
     ms_o, n_o, _ = scene.kernel_timing(linrt.KERNEL_OCCLUDED)
     ms_s, n_s, _ = scene.kernel_timing(linrt.KERNEL_SHADE)
+    ms_t, n_t, _ = scene.kernel_timing(linrt.KERNEL_TAIL)
     value = samples / elapsed / 1e6
 
     # roofline of the dominant kernels, the traversal launches (k_extend for
     # camera rays, then per bounce k_trace_queues over the shadow rays and the
-    # next extension rays): B_ray = 32 (ray) + 8 n_node + 4 n_ref + 36 n_tri
+    # next extension rays) and the tail launches (k_tail: once few paths are
+    # left it finishes them, tracing their remaining rays and shading them, so
+    # its whole time is priced as traversal time): B_ray = 32 (ray) + 8 n_node + 4 n_ref + 36 n_tri
     # + 16 (result) bytes, n_* from the counting pass.  `frac` prices the
     # reference's traversal work (full kd window, closest-hit shadow rays);
     # `own_work` prices what the timed kernels do (windows clipped to the mesh
@@ -353,10 +357,10 @@ def main():
     own_frame_bytes = (32 + 16) * rays + \
         8 * (st_c.own_nodes_visited + st_c.own_shadow_nodes_visited) + \
         36 * (st_c.own_tris_tested + st_c.own_shadow_tris_tested)
-    launches_per_frame = n_c / max(args.steps, 1)
+    launches_per_frame = (n_c + n_t) / max(args.steps, 1)
     bytes_per_launch = per_frame_bytes / launches_per_frame
     own_bytes_per_launch = own_frame_bytes / launches_per_frame
-    avg_ms = ms_c / max(n_c, 1)
+    avg_ms = (ms_c + ms_t) / max(n_c + n_t, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     own_achieved = own_bytes_per_launch / (avg_ms * 1e-3) / 1e9
     traffic, traffic_src = None, None
@@ -408,14 +412,16 @@ def main():
                    "triangles": int(info.n_triangles), "parallelism": "single GPU"},
         "mrays_per_s": round((ext + shadow) / elapsed / 1e6, 2),
         "rays_per_sample": round((ext + shadow) / max(samples, 1), 3),
-        "kernel_ms_per_step": {"trace": round(ms_c / args.steps, 2), "shade": round(ms_s / args.steps, 2)},
+        "kernel_ms_per_step": {"trace": round(ms_c / args.steps, 2), "tail": round(ms_t / args.steps, 2),
+                               "shade": round(ms_s / args.steps, 2)},
         "roofline": {"bound": "hbm",
-                     "kernel": "k_extend + k_trace_queues (kd traversal: extension rays closest hit, shadow rays any hit)",
+                     "kernel": "k_extend + k_trace_queues + k_tail (kd traversal: extension rays closest hit, "
+                               "shadow rays any hit; k_tail also shades the last paths)",
                      "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "traffic_source": traffic_src,
                      "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(avg_ms, 4),
-                     "launches": int(n_c),
+                     "launches": int(n_c + n_t), "tail_launches": int(n_t),
                      "frac_basis": "the reference's traversal work: full kd window, closest-hit shadow rays",
                      "own_work": {"what": "the timed kernels' own traversal: box-clipped windows, any-hit shadow rays, "
                                           "leaf triangles read in reference order",

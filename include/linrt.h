@@ -204,7 +204,12 @@ int lrt_scene_sync(lrt_scene* scene);
  * timed with HIP events on the scene's stream */
 int lrt_scene_kernel_timing(lrt_scene* scene, int kernel, int reset,
                             double* total_ms, uint64_t* launches, uint64_t* items);
-enum { LRT_KERNEL_CLOSEST = 0, LRT_KERNEL_OCCLUDED = 1, LRT_KERNEL_SHADE = 2, LRT_KERNEL_GATHER = 3, LRT_KERNEL_COUNT_ = 4 };
+/* kernel kinds: CLOSEST = the path tracer's traversal launches (k_extend, k_trace_queues) and the batch
+ * closest-hit queries; OCCLUDED = the batch shadow queries; SHADE = a bounce's binning and shading;
+ * GATHER = the photon gathers; TAIL = the path tracer's tail launches (each finishes the paths
+ * still live at a bounce at or past the roulette depth once few are left: traversal and shading) */
+enum { LRT_KERNEL_CLOSEST = 0, LRT_KERNEL_OCCLUDED = 1, LRT_KERNEL_SHADE = 2, LRT_KERNEL_GATHER = 3,
+       LRT_KERNEL_TAIL = 4, LRT_KERNEL_COUNT_ = 5 };
 
 /* ---- BSDF batch evaluation (bsdf.h:78-88) -----------------------------
  * For n tuples i: material index mat[i] into the scene's (or the given)

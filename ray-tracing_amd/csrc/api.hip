@@ -1198,6 +1198,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         Timer& tc = sc->timers[LRT_KERNEL_CLOSEST];
         Timer& to = sc->timers[LRT_KERNEL_OCCLUDED];
         Timer& ts = sc->timers[LRT_KERNEL_SHADE];
+        Timer& tt = sc->timers[LRT_KERNEL_TAIL];
         for (uint32_t s = 0; s < prm->spp; s += slots) {
             const uint32_t smp = prm->spp_offset + s;
             const uint32_t ns = std::min(slots, prm->spp - s);  // samples in this pass
@@ -1217,22 +1218,22 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
             HIPCHECK(hipEventRecord(e0.second, sc->stream));
             for (uint32_t depth = 0; depth < prm->max_depth; ++depth) {
                 ShadeParams sp{depth, prm->max_depth, prm->rr_depth, smp, key, sobol, aov_path};
-                auto e2 = ts.next();
-                HIPCHECK(hipEventRecord(e2.first, sc->stream));
                 if (tail && depth >= prm->rr_depth) {  // few paths left: finish them in one launch
+                    auto et = tt.next();
+                    HIPCHECK(hipEventRecord(et.first, sc->stream));
                     hipLaunchKernelGGL(sc->mesh_lights ? k_tail<true> : k_tail<false>, dim3(g_tail), dim3(kBlock), 0,
                                        sc->stream, sc->S, sc->Ls, sc->M, P, q, counters, Hb, sp, tail, dstats);
+                    HIPCHECK(hipEventRecord(et.second, sc->stream));
                     hipLaunchKernelGGL(k_tail_done, dim3(1), dim3(64), 0, sc->stream, counters, tail);
                     // once the tail has finished the pass, the remaining bounces' launches would find
                     // nothing: read the queue length back (one 4-byte copy) and end the pass
                     uint32_t left = 0;
                     HIPCHECK(hipMemcpyAsync(&left, counters, 4, hipMemcpyDeviceToHost, sc->stream));
                     HIPCHECK(hipStreamSynchronize(sc->stream));
-                    if (left == 0) {
-                        HIPCHECK(hipEventRecord(e2.second, sc->stream));
-                        break;
-                    }
+                    if (left == 0) break;
                 }
+                auto e2 = ts.next();
+                HIPCHECK(hipEventRecord(e2.first, sc->stream));
                 hipLaunchKernelGGL(k_shade_bins, dim3(g_bins), dim3(kBlock), 0, sc->stream, sc->S, Hb, counters);
                 hipLaunchKernelGGL(k_shade_scan, dim3(1), dim3(1024), 0, sc->stream, counters);
                 hipLaunchKernelGGL(k_shade_order, dim3(g_bins), dim3(kBlock), 0, sc->stream, sc->S, Hb, counters, order);
@@ -1279,6 +1280,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         tc.collect();
         to.collect();
         ts.collect();
+        tt.collect();
         tc.items += hst[0];
         to.items += hst[1];
         if (stats) {

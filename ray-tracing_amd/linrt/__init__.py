@@ -42,7 +42,7 @@ SHAPE_MESH, SHAPE_RECTANGLE, SHAPE_SPHERE = 0, 1, 2
 LIGHT_AREA, LIGHT_SKY = 0, 1
 GEN_BOX_ICOSPHERE, GEN_DISPLACED_SPHERE, GEN_PHOTON_BOX, GEN_DISPLACED_SPHERE_VARIANT, GEN_INVERSE_SPHERE = range(5)
 PHOTON_SURFACE, PHOTON_CAUSTIC = 0, 1
-KERNEL_CLOSEST, KERNEL_OCCLUDED, KERNEL_SHADE, KERNEL_GATHER = 0, 1, 2, 3
+KERNEL_CLOSEST, KERNEL_OCCLUDED, KERNEL_SHADE, KERNEL_GATHER, KERNEL_TAIL = 0, 1, 2, 3, 4
 
 F3 = C.c_float * 3
 
