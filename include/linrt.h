@@ -10,6 +10,11 @@
  * Threading: a lrt_scene is externally synchronised (one host thread at a time
  * per scene).  All GPU work of a scene is ordered on the scene's own HIP stream
  * and every call blocks until its results are complete, unless stated.
+ * Device pointers (device_ptrs != 0): the library's streams are non-blocking,
+ * so such a call first makes its stream wait for the work already queued on
+ * the device's null stream (torch's default stream), which may still be
+ * writing the inputs or reading earlier outputs; work on other streams is
+ * the caller's to complete before the call.
  *
  * Reference interfaces that these entry points stand in for are cited per
  * declaration as path:line of the reference repository.

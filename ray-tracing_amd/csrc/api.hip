@@ -64,6 +64,10 @@ static uint32_t pass_paths() {
     return v;
 }
 
+// k_shade's grid, in blocks per CU
+#ifndef LRT_SHADE_BLOCKS_PER_CU
+#define LRT_SHADE_BLOCKS_PER_CU 8
+#endif
 // live paths at or below which a pass's remaining bounces run in k_tail
 // (one lane finishes a path) instead of one wavefront launch per bounce, from
 // the Russian-roulette depth on; LINRT_TAIL_PATHS overrides (0: never)
@@ -102,6 +106,17 @@ struct Error : std::runtime_error {
         const hipError_t e_ = (x);                                                                      \
         if (e_ != hipSuccess) throw Error(LRT_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
+
+// The library's streams are non-blocking: they do not wait for the null
+// stream, where a caller (torch's default stream, for one) may still be
+// writing the inputs it passes by device pointer, or reading the outputs of
+// an earlier call.  A call on device pointers therefore first orders its
+// stream after the work queued on the null stream so far (an event recorded
+// there and waited for on the device; no host synchronisation).
+void after_caller(hipStream_t s, hipEvent_t ev) {
+    HIPCHECK(hipEventRecord(ev, nullptr));
+    HIPCHECK(hipStreamWaitEvent(s, ev, 0));
+}
 
 template <typename F>
 int guarded(F&& f) {
@@ -331,7 +346,9 @@ struct lrt_scene {
     std::vector<lrt_material> host_mats;
     int max_spec_components = 0;
     Timer timers[LRT_KERNEL_COUNT_];
+    hipEvent_t caller = nullptr;  // after_caller's event
     ~lrt_scene() {
+        if (caller) (void)hipEventDestroy(caller);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -640,7 +657,11 @@ T* alloc_carve(char*& p, size_t n) {
 }
 
 void stage_rays(lrt_scene& sc, const float* rays, uint32_t n, int device_ptrs, const float*& d_rays) {
-    if (device_ptrs) { d_rays = rays; return; }
+    if (device_ptrs) {
+        after_caller(sc.stream, sc.caller);
+        d_rays = rays;
+        return;
+    }
     upload(sc.q_rays, rays, (size_t)n * 8 * 4, sc.stream);
     d_rays = sc.q_rays.as<float>();
 }
@@ -757,6 +778,7 @@ int lrt_scene_create(int hip_device, const lrt_scene_desc* desc, lrt_scene** out
         sc->device = hip_device;
         sc->cus = cu_count(hip_device);
         HIPCHECK(hipStreamCreateWithFlags(&sc->stream, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreateWithFlags(&sc->caller, hipEventDisableTiming));
         build_scene(*sc, *desc);
         *out = sc.release();
     });
@@ -854,6 +876,7 @@ int lrt_trace_closest_noalloc(lrt_scene* sc, const float* rays, uint32_t n, int3
         uint32_t* next = (sc->device >= 0 && sc->device < kMaxDevices) ? g_scratch_ptr[sc->device] : nullptr;
         if (!next) throw Error(LRT_ERR_INTERNAL, "scratch of the device was not set up");
         HIPCHECK(hipSetDevice(sc->device));
+        after_caller(sc->stream, sc->caller);
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, sc->stream, next, 1u, 0u);
         hipLaunchKernelGGL(k_trace_closest, dim3(grid_for(*sc, n, 8)), dim3(kBlock), 0, sc->stream, sc->S, rays, n,
                            hit_obj_prim, hit_t, hit_uv, next);
@@ -1112,6 +1135,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         const uint32_t slots = std::max<uint32_t>(1, std::min<uint32_t>(prm->spp, pass_paths() / npix));
         const uint32_t n = npix * slots;
         HIPCHECK(hipSetDevice(sc->device));
+        if (device_ptrs) after_caller(sc->stream, sc->caller);
         const auto t0 = std::chrono::steady_clock::now();
         // work buffers: path state (64 bytes), hits (32 bytes), shadow queue (12), two queues
         if (sc->cap < n) {
@@ -1188,7 +1212,8 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         }
         const uint32_t g_all = (n + kBlock - 1) / kBlock;
         const uint32_t g_trace = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc->cus * 20);
-        const uint32_t g_shade = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc->cus * 8);
+        const uint32_t g_shade = (uint32_t)std::min<uint64_t>(((uint64_t)n + kShadeBlock - 1) / kShadeBlock,
+                                                              (uint64_t)sc->cus * LRT_SHADE_BLOCKS_PER_CU * 256 / kShadeBlock);
         const uint32_t g_bins = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc->cus * 4);
         // the tail kernel: not with the counting kernels (they count per bounce), ray capture or
         // LRT_RENDER_WAVEFRONT_ONLY
@@ -1237,7 +1262,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
                 hipLaunchKernelGGL(k_shade_bins, dim3(g_bins), dim3(kBlock), 0, sc->stream, sc->S, Hb, counters);
                 hipLaunchKernelGGL(k_shade_scan, dim3(1), dim3(1024), 0, sc->stream, counters);
                 hipLaunchKernelGGL(k_shade_order, dim3(g_bins), dim3(kBlock), 0, sc->stream, sc->S, Hb, counters, order);
-                hipLaunchKernelGGL(sc->mesh_lights ? k_shade<true> : k_shade<false>, dim3(g_shade), dim3(kBlock), 0,
+                hipLaunchKernelGGL(sc->mesh_lights ? k_shade<true> : k_shade<false>, dim3(g_shade), dim3(kShadeBlock), 0,
                                    sc->stream, sc->S, sc->Ls, sc->M, P, q, qn,
                                    counters, Hb, order, Sb, sp);
                 HIPCHECK(hipEventRecord(e2.second, sc->stream));
@@ -1370,6 +1395,7 @@ int lrt_render_aov(lrt_scene* sc, const lrt_render_params* prm, float* aov, int 
         const uint32_t re = prm->row_end > prm->row_begin ? prm->row_end : H;
         require(re <= H, "row range outside the image");
         HIPCHECK(hipSetDevice(sc->device));
+        if (device_ptrs) after_caller(sc->stream, sc->caller);
         const uint32_t n = (re - rb) * W;
         const size_t bytes = (size_t)W * H * 7 * 4;
         float* d = aov;
@@ -1562,6 +1588,7 @@ static DiffParams diff_setup(lrt_scene& sc, const lrt_diff_params& dp, const flo
 static const float* diff_theta(lrt_scene& sc, const lrt_diff_params& dp, const float* theta, int device_ptrs, float s[3]) {
     const uint32_t n = diff_count(dp);
     if (device_ptrs) {
+        after_caller(sc.stream, sc.caller);
         HIPCHECK(hipMemcpyAsync(s, theta + (n - 3), 12, hipMemcpyDeviceToHost, sc.stream));
         HIPCHECK(hipStreamSynchronize(sc.stream));
         return theta;
@@ -2116,6 +2143,7 @@ static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, doub
     require(re <= H, "row range outside the image");
     require((uint64_t)W * H < (1ull << 31), "image too large");
     HIPCHECK(hipSetDevice(sc->device));
+    if (device_ptrs) after_caller(sc->stream, sc->caller);
     const auto t0 = std::chrono::steady_clock::now();
     const size_t img = (size_t)W * H * 3;
     double* dsum = sum_rgb;

@@ -204,9 +204,11 @@ struct lrt_denoiser {
     hipEvent_t ev[11] = {};
     double layer_ms[10] = {};
     uint64_t timed_calls = 0;
+    hipEvent_t caller = nullptr;  // after_caller's event
     ~lrt_denoiser() {
         for (hipEvent_t e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (caller) (void)hipEventDestroy(caller);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -405,6 +407,7 @@ static void backward(lrt_denoiser& D) {
 static void stage_inputs(lrt_denoiser& D, const float* rad, const float* aux, const float* tgt, size_t P, int device_ptrs,
                          const float*& drad, const float*& daux, const float*& dtgt) {
     if (device_ptrs) {
+        after_caller(D.stream, D.caller);
         drad = rad;
         daux = aux;
         dtgt = tgt;
@@ -432,6 +435,7 @@ int lrt_denoiser_create(int device, uint64_t seed, lrt_denoiser** out) {
         auto D = std::make_unique<lrt_denoiser>();
         D->device = device;
         HIPCHECK(hipStreamCreateWithFlags(&D->stream, hipStreamNonBlocking));
+        HIPCHECK(hipEventCreateWithFlags(&D->caller, hipEventDisableTiming));
         hipDeviceProp_t prop;
         HIPCHECK(hipGetDeviceProperties(&prop, device));
         D->cus = prop.multiProcessorCount;

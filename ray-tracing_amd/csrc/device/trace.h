@@ -420,6 +420,30 @@ LRT_D bool trace_occluded(const DevScene& S, const Ray& r, float tmin, float tma
 // The visit order, windows and acceptance rules are those of kd_traverse and
 // trace_closest / trace_occluded.
 
+// which of the path tracer's traversal launches take their rays per XCD
+// (wave_refill_xcd): the camera rays (k_extend, a pixel-ordered queue whose
+// in-flight rays span a band of the image) and the bounce queues
+// (k_trace_queues, whose in-flight rays already start in a few shading cells)
+#ifndef LRT_XCD_FETCH_EXTEND
+#define LRT_XCD_FETCH_EXTEND 1
+#endif
+#ifndef LRT_XCD_FETCH_QUEUES
+#define LRT_XCD_FETCH_QUEUES 0
+#endif
+// k_shade appends to its two queues once per block (continuing paths grouped
+// by direction octant) instead of once per wave
+#ifndef LRT_SHADE_BLOCK_APPEND
+#define LRT_SHADE_BLOCK_APPEND 1
+#endif
+// k_shade's block size (threads)
+#ifndef LRT_SHADE_BLOCK
+#define LRT_SHADE_BLOCK 256
+#endif
+constexpr uint32_t kShadeBlock = LRT_SHADE_BLOCK;
+// the camera rays queued in 8 x 8 pixel tiles rather than in rows
+#ifndef LRT_CAMERA_TILES
+#define LRT_CAMERA_TILES 1
+#endif
 #ifndef LRT_REFILL
 #define LRT_REFILL 48
 #endif
@@ -628,6 +652,78 @@ LRT_D uint32_t wave_refill(bool idle_lane, uint32_t* next, uint32_t n, int& more
     if (!idle_lane) return none;
     const uint32_t i = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
     return i < n ? i : none;
+}
+
+// XCD-aware work distribution for the path tracer's traversal launches.
+// MI355X's 8 XCDs each have their own L2, and the dispatcher places
+// workgroups on them round-robin (block b on XCD b % 8).  The items are two
+// queues, A = [0, na) and B = [na, na + nb), each written in shading-bin
+// (Morton cell) order, so a contiguous stretch of either holds rays that start
+// in one region of the scene.  XCD x owns the x-th eighth of each queue: its
+// L2 then serves the nodes and triangles of that region instead of the whole
+// scene's.  A wave's first call hands its lanes the XCD's first items without
+// an atomic (as STATIC_FIRST above); later items come from the XCD's counter,
+// and once its range is drained the wave takes items from the other XCDs'
+// ranges, so the launch still ends with its slowest ray, not its slowest XCD.
+// ctr: kXcds counters kXcdStride words apart (one cache line each), zero at
+// launch.  `more` (uniform over the wave): 2 before the first call, 0 once
+// every range is drained, else 1 + 4 x the mask of the ranges found drained.
+constexpr uint32_t kXcds = 8, kXcdStride = 32;
+
+struct XcdSplit {
+    uint32_t na, nb;
+    LRT_D uint32_t a0(uint32_t x) const { return (uint32_t)(((uint64_t)na * x) >> 3); }
+    LRT_D uint32_t b0(uint32_t x) const { return (uint32_t)(((uint64_t)nb * x) >> 3); }
+    LRT_D uint32_t len(uint32_t x) const { return a0(x + 1) - a0(x) + b0(x + 1) - b0(x); }
+    // item j of XCD x's range: its share of A, then its share of B
+    LRT_D uint32_t item(uint32_t x, uint32_t j) const {
+        const uint32_t la = a0(x + 1) - a0(x);
+        return j < la ? a0(x) + j : na + b0(x) + (j - la);
+    }
+};
+
+LRT_D uint32_t wave_refill_xcd(bool idle_lane, uint32_t* ctr, const XcdSplit& q, int& more) {
+    const uint32_t none = 0xffffffffu;
+    if (!more) return none;
+    const uint32_t home = blockIdx.x & (kXcds - 1u);
+    if (more == 2) {
+        more = 1;
+        const uint32_t j = (blockIdx.x / kXcds) * blockDim.x + threadIdx.x;
+        return idle_lane && j < q.len(home) ? q.item(home, j) : none;
+    }
+    const uint64_t idle = __ballot(idle_lane);
+    const int nidle = __popcll(idle);
+    if (nidle < kRefill && nidle < (int)__popcll(__ballot(true))) return none;
+    const int lane = (int)(threadIdx.x & 63u);
+    const int leader = __ffsll((long long)idle) - 1;
+    uint32_t tried = (uint32_t)more >> 2, from = kXcds, start = 0u;
+    if (lane == leader) {
+        for (uint32_t t = 0; t < kXcds; ++t) {
+            const uint32_t x = (home + t) & (kXcds - 1u);
+            if ((tried >> x) & 1u) continue;
+            const uint32_t len = q.len(x);
+            // lanes of XCD x that took its first items in their first call
+            const uint32_t first = ((gridDim.x + kXcds - 1u - x) / kXcds) * blockDim.x;
+            if (len > first) {
+                // len < 2^31 and at most one overshoot of 64 per wave and range: no wrap
+                const uint32_t b = first + atomicAdd(&ctr[x * kXcdStride], (uint32_t)nidle);
+                if (b < len) {
+                    from = x;
+                    start = b;
+                    if (b + (uint32_t)nidle >= len) tried |= 1u << x;  // drained by this fetch
+                    break;
+                }
+            }
+            tried |= 1u << x;
+        }
+    }
+    tried = (uint32_t)__builtin_amdgcn_readlane((int)tried, leader);
+    from = (uint32_t)__builtin_amdgcn_readlane((int)from, leader);
+    start = (uint32_t)__builtin_amdgcn_readlane((int)start, leader);
+    more = from == kXcds || tried == (1u << kXcds) - 1u ? 0 : (int)(1u + 4u * tried);
+    if (from == kXcds || !idle_lane) return none;
+    const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+    return start + rank < q.len(from) ? q.item(from, start + rank) : none;
 }
 
 }  // namespace lrt

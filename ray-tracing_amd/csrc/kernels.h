@@ -54,7 +54,10 @@ constexpr int kCellBits = 3;
 constexpr int kShadeBins = 2 + (1 << (3 * kCellBits));
 // counters[] beyond the queue counters: per-bin counts, cursors and starts
 constexpr int kBinCount = 64, kBinCursor = kBinCount + 1024, kBinStart = kBinCursor + 1024;
-constexpr size_t kCounterWords = kBinStart + 1024;
+// then the per-XCD fetch counters of k_extend and of k_trace_queues (8 each, one
+// 128-byte line apart), zeroed by k_raygen and k_advance
+constexpr int kXcdFetchExtend = kBinStart + 1024, kXcdFetchQueues = kXcdFetchExtend + 256;
+constexpr size_t kCounterWords = kXcdFetchQueues + 256;
 static_assert(kShadeBins <= 1024, "shading bins");
 
 // Morton cell of point q in the box lo..hi (6 floats), 3 bits per axis

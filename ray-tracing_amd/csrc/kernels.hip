@@ -187,6 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_raygen(PathState P, const CameraDev 
         counters[kBinCount + i] = 0;
         counters[kBinCursor + i] = 0;
     }
+    if (i < 2 * kXcds) counters[kXcdFetchExtend + i * kXcdStride] = 0;  // both sets of XCD fetch counters
     if (i >= n) return;
     const uint32_t ip = i % P.npix, slot = i / P.npix;
     const uint32_t x = ip % W, y = row0 + ip / W;
@@ -209,7 +210,16 @@ __global__ __launch_bounds__(kBlock) void k_raygen(PathState P, const CameraDev 
     w[1] = word(d, -1.0f);  // no solid-angle pdf yet (camera ray)
     w[2] = make_float4(1.0f, 1.0f, 1.0f, 0.0f);
     w[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#if LRT_CAMERA_TILES
+    // the camera queue in 8 x 8 pixel tiles (a wave traces one tile), row-major
+    // tiles; the right and bottom tiles of a ragged frame are narrower / lower
+    const uint32_t hp = P.npix / W, ly = ip / W;
+    const uint32_t tw = min(8u, W - (x & ~7u)), th = min(8u, hp - (ly & ~7u));
+    const uint32_t rank = (ly & ~7u) * W + (x & ~7u) * th + (ly & 7u) * tw + (x & 7u);
+    queue[slot * P.npix + rank] = i;
+#else
     queue[i] = i;
+#endif
 }
 
 // extension rays of the active queue -> hit records by queue slot.
@@ -226,8 +236,15 @@ __global__ __launch_bounds__(kBlock) LRT_PATH_TRACE_ATTR void k_extend(const Dev
     uint32_t i = 0;
     bool active = false;
     int more = 2;  // wave_refill: first fill pending
+#if LRT_XCD_FETCH_EXTEND
+    const XcdSplit split{n, 0u};
+#endif
     for (;;) {
+#if LRT_XCD_FETCH_EXTEND
+        const uint32_t got = wave_refill_xcd(!active, &counters[kXcdFetchExtend], split, more);
+#else
         const uint32_t got = wave_refill(!active, &counters[4], n, more);
+#endif
         if (got != 0xffffffffu) {
             i = got;
             const float4* w = path_words(P, queue[i]);
@@ -431,6 +448,7 @@ LRT_D f3 albedo_of(const Mats& M, int m) {
 // contribution the ray adds if unoccluded.
 struct ShadeOut {
     bool cont, shadow;
+    uint32_t octant;  // of the continuing ray's direction (sign bits x, y, z)
     f3 so, sd, contrib;
     float tmax;
     int ignore;
@@ -566,6 +584,7 @@ LRT_D void shade_path(const DevScene& S, const DevLights& Ls, const Mats& M, con
                         else T = f3{T.x / q, T.y / q, T.z / q};
                     }
                     if (o.cont) {
+                        o.octant = (nd.x < 0.0f ? 1u : 0u) | (nd.y < 0.0f ? 2u : 0u) | (nd.z < 0.0f ? 4u : 0u);
                         pw[0] = word(sh.p, eta);
                         pw[1] = word(nd, hp ? sr.pdf : -1.0f);
                         pw[2] = word(T, 0.0f);
@@ -580,14 +599,19 @@ LRT_D void shade_path(const DevScene& S, const DevLights& Ls, const Mats& M, con
 // one bounce of path_integrator::radiance for every queued path, in the
 // shading order of the hit slots
 template <bool MESH_LIGHTS>
-__global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_shade(const DevScene S, const DevLights Ls, const Mats M, PathState P,
+__global__ __launch_bounds__(kShadeBlock) LRT_SHADE_ATTR void k_shade(const DevScene S, const DevLights Ls, const Mats M, PathState P,
                                                   const uint32_t* __restrict__ queue, uint32_t* __restrict__ next_queue,
                                                   uint32_t* counters, const HitBuf Hb, const uint32_t* __restrict__ order,
                                                   ShadowBuf Sb, ShadeParams prm) {
     const uint32_t n = counters[0];
     const uint32_t stride = gridDim.x * blockDim.x;
-    // grid-stride with a wave-uniform trip count so the queue appends see full waves
+    // grid-stride with a grid-uniform trip count: the queue appends see full
+    // waves and every thread of a block reaches its barriers
     const uint32_t n_round = (n + stride - 1) / stride * stride;
+#if LRT_SHADE_BLOCK_APPEND
+    constexpr uint32_t kWaves = kShadeBlock / 64;
+    __shared__ uint32_t wcnt[kWaves][9], wat[kWaves][9];  // per wave: paths per octant, shadow records; first slots
+#endif
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_round; j += stride) {
         ShadeOut o;
         o.cont = false;
@@ -598,9 +622,52 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_shade(const DevScene 
             p = queue[i];
             shade_path<MESH_LIGHTS>(S, Ls, M, P, p, hit_load(Hb, i), prm, o);
         }
+#if LRT_SHADE_BLOCK_APPEND
+        // one append per block and queue: per wave, ballots count the
+        // continuing paths of each direction octant and the shadow records;
+        // thread 0 reserves the block's slots with one atomic per queue (a
+        // wave's appends made 2 atomics on the same two counters per 64 paths)
+        // and the continuing paths are laid out octant by octant, so a
+        // traversal wave takes rays of one shading cell and one octant
+        const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+        const uint64_t lt = (1ull << lane) - 1ull;
+        uint32_t rank = 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+            const bool in = o.cont && o.octant == k;
+            const uint64_t m = __ballot(in);
+            if (lane == 0) wcnt[wv][k] = (uint32_t)__popcll(m);
+            if (in) rank = (uint32_t)__popcll(m & lt);
+        }
+        const uint64_t msh = __ballot(o.shadow);
+        if (lane == 0) wcnt[wv][8] = (uint32_t)__popcll(msh);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t te = 0u, ts = 0u;
+            for (uint32_t k = 0; k < 8; ++k)
+                for (uint32_t v = 0; v < kWaves; ++v) {
+                    wat[v][k] = te;
+                    te += wcnt[v][k];
+                }
+            for (uint32_t v = 0; v < kWaves; ++v) {
+                wat[v][8] = ts;
+                ts += wcnt[v][8];
+            }
+            const uint32_t be = te ? atomicAdd(&counters[1], te) : 0u;
+            const uint32_t bs = ts ? atomicAdd(&counters[2], ts) : 0u;
+            for (uint32_t v = 0; v < kWaves; ++v) {
+                for (uint32_t k = 0; k < 8; ++k) wat[v][k] += be;
+                wat[v][8] += bs;
+            }
+        }
+        __syncthreads();
+        if (o.cont) next_queue[wat[wv][o.octant] + rank] = p;
+        const uint32_t ss = o.shadow ? wat[wv][8] + (uint32_t)__popcll(msh & lt) : 0u;
+#else
         const uint32_t slot = wave_append(&counters[1], o.cont);
         if (o.cont) next_queue[slot] = p;
         const uint32_t ss = wave_append(&counters[2], o.shadow);
+#endif
         if (o.shadow) {
             Sb.ox[ss] = o.so.x; Sb.oy[ss] = o.so.y; Sb.oz[ss] = o.so.z;
             Sb.dx[ss] = o.sd.x; Sb.dy[ss] = o.sd.y; Sb.dz[ss] = o.sd.z;
@@ -784,8 +851,15 @@ __global__ __launch_bounds__(kBlock) LRT_PATH_TRACE_ATTR void k_trace_queues(con
     uint32_t i = 0;
     bool active = false;
     int more = 2;  // wave_refill: first fill pending
+#if LRT_XCD_FETCH_QUEUES
+    const XcdSplit split{ns, n - ns};
+#endif
     for (;;) {
+#if LRT_XCD_FETCH_QUEUES
+        const uint32_t got = wave_refill_xcd(!active, &counters[kXcdFetchQueues], split, more);
+#else
         const uint32_t got = wave_refill(!active, &counters[5], n, more);
+#endif
         if (got != 0xffffffffu) {
             i = got;
             if (i < ns) {
@@ -871,6 +945,7 @@ __global__ void k_advance(uint32_t* counters, unsigned long long* stats) {
         counters[kBinCount + b] = 0;
         counters[kBinCursor + b] = 0;
     }
+    if (threadIdx.x < 2 * kXcds) counters[kXcdFetchExtend + threadIdx.x * kXcdStride] = 0;
 }
 
 // per pixel, the pass's samples in sample order

@@ -147,6 +147,33 @@ def test_forward_with_biases_vs_oracle():
     assert np.abs(den.forward(rad, aux) - out).max() > 10 * err * np.abs(ref).max()
 
 
+def test_device_inputs_written_by_torch_just_before_the_call():
+    """Inputs that torch writes on its default stream right before a
+    device-pointer call are complete when the call's kernels read them (the
+    library's streams are non-blocking, so the call orders itself after the
+    null stream's work): a frame whose radiance is recomputed into a block
+    that last held NaNs, each time just before the forward, denoises to the
+    host-pointer forward's output every time."""
+    import torch
+    rng = np.random.default_rng(21)
+    B, H, W = 1, 540, 960
+    rad, aux, _ = synthetic_batch(rng, B, H, W)
+    den = linrt.Denoiser(seed=3)
+    ref = den.forward(rad, aux)
+    r64 = torch.from_numpy(rad.astype(np.float64)).cuda()
+    a = torch.from_numpy(aux).cuda()
+    torch.cuda.synchronize()
+    for k in range(4):
+        junk = torch.full(rad.shape, float("nan"), dtype=torch.float32, device="cuda")
+        del junk  # the caching allocator hands this block to r
+        r = (r64 * 1.0).float()
+        out = torch.full_like(r, float("nan"))
+        den.forward_device(r.data_ptr(), a.data_ptr(), B, H, W, out.data_ptr())
+        got = out.cpu().numpy()
+        assert np.isfinite(got).all(), f"call {k}: non-finite output"
+        assert np.array_equal(got, ref), f"call {k}: differs from the host-pointer forward"
+
+
 def test_training_lowers_loss():
     rng = np.random.default_rng(9)
     den = linrt.Denoiser(seed=4)
