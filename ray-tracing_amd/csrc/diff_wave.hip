@@ -75,9 +75,12 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_diff_shade(const DevS
     const uint32_t n_round = (n + stride - 1) / stride * stride;
     const float gsc = ADJ ? *prm.gscale : 0.0f;
     long long gs[3] = {0, 0, 0};
+#if LRT_SHADE_BLOCK_APPEND
+    __shared__ BlockAppend<kBlock> ba;
+#endif
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_round; j += stride) {
         bool cont = false, shadow = false;
-        uint32_t p = 0;
+        uint32_t p = 0, octant = 0;
         f3 so, sd, contrib;
         float tmax = 0.0f;
         int ignore = -1;
@@ -249,6 +252,7 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_diff_shade(const DevS
                                 else T = f3{T.x / q, T.y / q, T.z / q};
                             }
                             if (cont) {
+                                octant = dir_octant(nd);
                                 pw[0] = word(sh.p, eta);
                                 pw[1] = word(nd, hp ? sr.pdf : -1.0f);
                                 pw[2] = word(T, 0.0f);
@@ -259,9 +263,16 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_diff_shade(const DevS
             }
             pw[3] = word(L, 0.0f);
         }
+#if LRT_SHADE_BLOCK_APPEND
+        // one append per block and queue, as k_shade
+        const uint2 sl = block_append2<kBlock>(ba, cont, octant, shadow, &counters[1], &counters[2]);
+        if (cont) next_queue[sl.x] = p;
+        const uint32_t ss = sl.y;
+#else
         const uint32_t slot = wave_append(&counters[1], cont);
         if (cont) next_queue[slot] = p;
         const uint32_t ss = wave_append(&counters[2], shadow);
+#endif
         if (shadow) {
             Sb.ox[ss] = so.x; Sb.oy[ss] = so.y; Sb.oz[ss] = so.z;
             Sb.dx[ss] = sd.x; Sb.dy[ss] = sd.y; Sb.dz[ss] = sd.z;

@@ -28,6 +28,58 @@ LRT_D uint32_t wave_append(uint32_t* counter, bool want) {
     return base + rank;
 }
 
+// the octant of a direction (sign bits of x, y, z)
+LRT_D uint32_t dir_octant(const f3& d) { return (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u); }
+
+// Block-aggregated appends to two queues, for kernels whose every thread calls
+// it the same number of times: per wave, ballots count queue A's items of each
+// key (0..7) and queue B's items; thread 0 reserves the block's slots with one
+// atomic per queue.  (A wave's own appends make one atomic per queue and 64
+// items on the same counter, which the memory-side atomic unit serialises.)
+// A's items are laid out key by key, then wave by wave and lane by lane; B's
+// wave by wave.  Returns the thread's slots in A and B (where a / b).
+template <uint32_t NT>
+struct BlockAppend {
+    uint32_t cnt[NT / 64][9], at[NT / 64][9];
+};
+template <uint32_t NT>
+LRT_D uint2 block_append2(BlockAppend<NT>& sm, bool a, uint32_t key, bool b, uint32_t* ca, uint32_t* cb) {
+    constexpr uint32_t kW = NT / 64;
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t rank = 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+        const bool in = a && key == k;
+        const uint64_t m = __ballot(in);
+        if (lane == 0) sm.cnt[wv][k] = (uint32_t)__popcll(m);
+        if (in) rank = (uint32_t)__popcll(m & lt);
+    }
+    const uint64_t mb = __ballot(b);
+    if (lane == 0) sm.cnt[wv][8] = (uint32_t)__popcll(mb);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ta = 0u, tb = 0u;
+        for (uint32_t k = 0; k < 8; ++k)
+            for (uint32_t v = 0; v < kW; ++v) {
+                sm.at[v][k] = ta;
+                ta += sm.cnt[v][k];
+            }
+        for (uint32_t v = 0; v < kW; ++v) {
+            sm.at[v][8] = tb;
+            tb += sm.cnt[v][8];
+        }
+        const uint32_t ba = ta ? atomicAdd(ca, ta) : 0u;
+        const uint32_t bb = tb ? atomicAdd(cb, tb) : 0u;
+        for (uint32_t v = 0; v < kW; ++v) {
+            for (uint32_t k = 0; k < 8; ++k) sm.at[v][k] += ba;
+            sm.at[v][8] += bb;
+        }
+    }
+    __syncthreads();
+    return make_uint2(a ? sm.at[wv][key] + rank : 0u, b ? sm.at[wv][8] + (uint32_t)__popcll(mb & lt) : 0u);
+}
+
 // ---------------------------------------------------------------------------
 // ray queries
 // rays -> closest hits (SoA rays: ox, oy, oz, dx, dy, dz, tmin, tmax);
@@ -584,7 +636,7 @@ LRT_D void shade_path(const DevScene& S, const DevLights& Ls, const Mats& M, con
                         else T = f3{T.x / q, T.y / q, T.z / q};
                     }
                     if (o.cont) {
-                        o.octant = (nd.x < 0.0f ? 1u : 0u) | (nd.y < 0.0f ? 2u : 0u) | (nd.z < 0.0f ? 4u : 0u);
+                        o.octant = dir_octant(nd);
                         pw[0] = word(sh.p, eta);
                         pw[1] = word(nd, hp ? sr.pdf : -1.0f);
                         pw[2] = word(T, 0.0f);
@@ -609,8 +661,7 @@ __global__ __launch_bounds__(kShadeBlock) LRT_SHADE_ATTR void k_shade(const DevS
     // waves and every thread of a block reaches its barriers
     const uint32_t n_round = (n + stride - 1) / stride * stride;
 #if LRT_SHADE_BLOCK_APPEND
-    constexpr uint32_t kWaves = kShadeBlock / 64;
-    __shared__ uint32_t wcnt[kWaves][9], wat[kWaves][9];  // per wave: paths per octant, shadow records; first slots
+    __shared__ BlockAppend<kShadeBlock> ba;
 #endif
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_round; j += stride) {
         ShadeOut o;
@@ -623,46 +674,13 @@ __global__ __launch_bounds__(kShadeBlock) LRT_SHADE_ATTR void k_shade(const DevS
             shade_path<MESH_LIGHTS>(S, Ls, M, P, p, hit_load(Hb, i), prm, o);
         }
 #if LRT_SHADE_BLOCK_APPEND
-        // one append per block and queue: per wave, ballots count the
-        // continuing paths of each direction octant and the shadow records;
-        // thread 0 reserves the block's slots with one atomic per queue (a
-        // wave's appends made 2 atomics on the same two counters per 64 paths)
-        // and the continuing paths are laid out octant by octant, so a
-        // traversal wave takes rays of one shading cell and one octant
-        const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-        const uint64_t lt = (1ull << lane) - 1ull;
-        uint32_t rank = 0u;
-#pragma unroll
-        for (uint32_t k = 0; k < 8; ++k) {
-            const bool in = o.cont && o.octant == k;
-            const uint64_t m = __ballot(in);
-            if (lane == 0) wcnt[wv][k] = (uint32_t)__popcll(m);
-            if (in) rank = (uint32_t)__popcll(m & lt);
-        }
-        const uint64_t msh = __ballot(o.shadow);
-        if (lane == 0) wcnt[wv][8] = (uint32_t)__popcll(msh);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t te = 0u, ts = 0u;
-            for (uint32_t k = 0; k < 8; ++k)
-                for (uint32_t v = 0; v < kWaves; ++v) {
-                    wat[v][k] = te;
-                    te += wcnt[v][k];
-                }
-            for (uint32_t v = 0; v < kWaves; ++v) {
-                wat[v][8] = ts;
-                ts += wcnt[v][8];
-            }
-            const uint32_t be = te ? atomicAdd(&counters[1], te) : 0u;
-            const uint32_t bs = ts ? atomicAdd(&counters[2], ts) : 0u;
-            for (uint32_t v = 0; v < kWaves; ++v) {
-                for (uint32_t k = 0; k < 8; ++k) wat[v][k] += be;
-                wat[v][8] += bs;
-            }
-        }
-        __syncthreads();
-        if (o.cont) next_queue[wat[wv][o.octant] + rank] = p;
-        const uint32_t ss = o.shadow ? wat[wv][8] + (uint32_t)__popcll(msh & lt) : 0u;
+        // one append per block and queue, the continuing paths laid out by
+        // direction octant: a traversal wave takes rays of one shading cell
+        // and one octant
+        const uint2 sl = block_append2<kShadeBlock>(ba, o.cont, o.cont ? o.octant : 0u, o.shadow, &counters[1],
+                                                    &counters[2]);
+        if (o.cont) next_queue[sl.x] = p;
+        const uint32_t ss = sl.y;
 #else
         const uint32_t slot = wave_append(&counters[1], o.cont);
         if (o.cont) next_queue[slot] = p;
