@@ -435,6 +435,10 @@ LRT_D bool trace_occluded(const DevScene& S, const Ray& r, float tmin, float tma
 #ifndef LRT_SHADE_BLOCK_APPEND
 #define LRT_SHADE_BLOCK_APPEND 1
 #endif
+// the photon eye pass's shading kernel appends once per block as well
+#ifndef LRT_EYE_BLOCK_APPEND
+#define LRT_EYE_BLOCK_APPEND 1
+#endif
 // k_shade's block size (threads)
 #ifndef LRT_SHADE_BLOCK
 #define LRT_SHADE_BLOCK 256

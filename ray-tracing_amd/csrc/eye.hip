@@ -129,7 +129,18 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_eye_shade(const DevSc
                                                       EyeShadow Sh, EyeQuery Q, uint32_t* counters, EyeParams prm,
                                                       unsigned long long* __restrict__ acc) {
     const uint32_t stride = gridDim.x * blockDim.x;
+    // a grid-uniform trip count, and loops over samples that run while any
+    // lane of the block has one left: every thread reaches the appends'
+    // barriers (one append per block and queue, not per wave)
     const uint32_t n_round = (n + stride - 1) / stride * stride;
+#if LRT_EYE_BLOCK_APPEND
+    __shared__ BlockAppend1<kBlock> ba;
+#define LRT_EYE_APPEND(ctr, want) block_append1<kBlock>(ba, (want), (ctr))
+#define LRT_EYE_ANY(c) __syncthreads_or(c)
+#else
+#define LRT_EYE_APPEND(ctr, want) wave_append((ctr), (want))
+#define LRT_EYE_ANY(c) __any(c)
+#endif
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_round; i += stride) {
         const bool valid = i < n;
         uint32_t sid = 0, meta = 0, path = 0;
@@ -199,7 +210,7 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_eye_shade(const DevSc
         if (gather) refl = col(M.m[sh.material].diffuse) * (1.0f / kPiF);
         for (uint32_t m = 0; m < 2; ++m) {
             const bool q = gather && ((prm.maps >> m) & 1u);
-            const uint32_t qs = wave_append(&counters[2], q);
+            const uint32_t qs = LRT_EYE_APPEND(&counters[2], q);
             if (q && qs < prm.cap_query) {
                 Q.sid[qs] = sid;
                 Q.meta[qs] = m | ((prm.max_depth - depth) << 1);
@@ -217,7 +228,7 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_eye_shade(const DevSc
         }
         // specular components while depth < 4 (photon_integrator.h:126-155)
         const int ncomp = (shade && exhaustive) ? bsdf_component_count(M, sh.material) : 0;
-        for (int c = 0; __any(c < ncomp); ++c) {
+        for (int c = 0; LRT_EYE_ANY(c < ncomp); ++c) {
             bool push = false;
             SRec sr;
             f3 w = splat(0.0f);
@@ -229,7 +240,7 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_eye_shade(const DevSc
                 w = bsdf_sample(M, sh.material, ctx, sr, 0.5f, 0.5f);
                 push = !is_zero(w);
             }
-            const uint32_t s = wave_append(&counters[0], push);
+            const uint32_t s = LRT_EYE_APPEND(&counters[0], push);
             if (push && s < prm.cap_items)
                 eye_push(Out, s, sid, eye_child(path, (uint32_t)c), (depth + 1u) | kEyeEmission | kEyeLi, sh.p,
                          to_world(fr, sr.wo), T * w, splat(0.0f), 0.0f);
@@ -242,7 +253,7 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_eye_shade(const DevSc
         const int cbn = first ? (diffuse ? 16 : 32) : 1;
         const float ew = first ? 1.0f / 16.0f : 1.0f;
         const uint32_t mask = (!specular || exhaustive) ? (F_DIFFUSE | F_GLOSSY) : 0xffffffffu;
-        for (int k = 0; __any(k < ce); ++k) {
+        for (int k = 0; LRT_EYE_ANY(k < ce); ++k) {
             bool push = false;
             f3 so = splat(0.0f), sd = splat(0.0f), contrib = splat(0.0f);
             float tmax = kInfF;
@@ -280,7 +291,7 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_eye_shade(const DevSc
                     }
                 }
             }
-            const uint32_t s = wave_append(&counters[1], push);
+            const uint32_t s = LRT_EYE_APPEND(&counters[1], push);
             if (push && s < prm.cap_shadow) {
                 Sh.sid[s] = sid;
                 Sh.ox[s] = so.x; Sh.oy[s] = so.y; Sh.oz[s] = so.z;
@@ -297,7 +308,7 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_eye_shade(const DevSc
         const float bw = first ? 1.0f / (float)cbn : 1.0f;
         const uint32_t child_meta = (depth + 1u) | (sample_direct ? kEyeDirect : 0u) | (sample_indirect ? kEyeLi : 0u) |
                                     (diffuse ? kEyeParentDiffuse : 0u);
-        for (int k = 0; __any(k < cb); ++k) {
+        for (int k = 0; LRT_EYE_ANY(k < cb); ++k) {
             bool push = false;
             SRec sr;
             f3 w = splat(0.0f);
@@ -311,13 +322,16 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_eye_shade(const DevSc
                 w = bsdf_sample(M, sh.material, ctx, sr, ux, uy);
                 push = !is_zero(w) && sr.pdf > 0.0f;
             }
-            const uint32_t s = wave_append(&counters[0], push);
+            const uint32_t s = LRT_EYE_APPEND(&counters[0], push);
             if (push && s < prm.cap_items)
                 eye_push(Out, s, sid, eye_child(path, 16u + (uint32_t)k), child_meta, sh.p, to_world(fr, sr.wo),
                          (T * w) * bw, T * w, sr.pdf);
         }
     }
 }
+
+#undef LRT_EYE_APPEND
+#undef LRT_EYE_ANY
 
 // unoccluded shadow rays add their contributions; persistent (lanes
 // refilled from *next)

@@ -38,6 +38,31 @@ LRT_D uint32_t dir_octant(const f3& d) { return (d.x < 0.0f ? 1u : 0u) | (d.y < 
 // items on the same counter, which the memory-side atomic unit serialises.)
 // A's items are laid out key by key, then wave by wave and lane by lane; B's
 // wave by wave.  Returns the thread's slots in A and B (where a / b).
+// One queue, no keys: the same as block_append2's B.  Returns the thread's
+// slot, UINT32_MAX where it does not append.
+template <uint32_t NT>
+struct BlockAppend1 {
+    uint32_t cnt[NT / 64], at[NT / 64];
+};
+template <uint32_t NT>
+LRT_D uint32_t block_append1(BlockAppend1<NT>& sm, bool want, uint32_t* counter) {
+    const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint64_t m = __ballot(want);
+    if (lane == 0) sm.cnt[wv] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0u;
+        for (uint32_t v = 0; v < NT / 64; ++v) {
+            sm.at[v] = t;
+            t += sm.cnt[v];
+        }
+        const uint32_t b = t ? atomicAdd(counter, t) : 0u;
+        for (uint32_t v = 0; v < NT / 64; ++v) sm.at[v] += b;
+    }
+    __syncthreads();
+    return want ? sm.at[wv] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)) : 0xffffffffu;
+}
+
 template <uint32_t NT>
 struct BlockAppend {
     uint32_t cnt[NT / 64][9], at[NT / 64][9];
