@@ -65,15 +65,22 @@ LRT_D void eye_add(unsigned long long* acc, uint32_t sid, f3 c) {
 
 __global__ __launch_bounds__(kBlock) void k_eye_raygen(const CameraDev cam, uint32_t W, uint32_t H, uint32_t n,
                                                        EyeParams prm, EyeItems It) {
+    // item i takes the pixels in 8 x 8 tiles (whole-row chunks): a wave traces
+    // one tile, and the children it queues start in one patch of the scene.
+    // A sample's id (pixel, sample) keys its streams and its accumulator, so
+    // the order changes no result.
+    const bool tiled = LRT_CAMERA_TILES && prm.npix % W == 0u && prm.pixel0 % W == 0u;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const uint32_t pix = prm.pixel0 + i % prm.npix;
+        const uint32_t r = i % prm.npix, ip = tiled ? tile_pixel(r, W, prm.npix / W) : r;
+        const uint32_t sid = i - r + ip;
+        const uint32_t pix = prm.pixel0 + ip;
         const uint32_t x = pix % W, y = pix / W;
         const uint4 v = philox(make_uint4(pix, prm.sample + i / prm.npix, 0u, 0u), prm.key);
         const float jx = u24(v.x), jy = u24(v.y);
         const float u = ((float)x + jx) / (float)(W - 1);
         const float vv = ((float)y + jy) / (float)(H - 1);
         const f3 d = normalize(ld3(cam.llc) + u * ld3(cam.horizontal) + vv * ld3(cam.vertical) - ld3(cam.origin));
-        It.sid[i] = i;
+        It.sid[i] = sid;
         It.path[i] = 0u;
         It.meta[i] = 1u | kEyeEmission | kEyeLi;
         It.ox[i] = cam.origin[0]; It.oy[i] = cam.origin[1]; It.oz[i] = cam.origin[2];

@@ -248,6 +248,17 @@ LRT_D float dim_u(uint32_t pix, uint32_t smp, uint32_t d, uint2 key) {
     return u24(pick(v, d & 3u));
 }
 
+// the pixel at position r of a W x hp frame walked in 8 x 8 tiles, row-major
+// tiles, the right and bottom tiles of a ragged frame narrower / lower (the
+// inverse of the rank k_raygen computes)
+LRT_D uint32_t tile_pixel(uint32_t r, uint32_t W, uint32_t hp) {
+    const uint32_t ty = r / (8u * W), r2 = r - ty * 8u * W;
+    const uint32_t th = min(8u, hp - ty * 8u);
+    const uint32_t tx = r2 / (8u * th), r3 = r2 - tx * 8u * th;
+    const uint32_t tw = min(8u, W - tx * 8u);
+    return (ty * 8u + r3 / tw) * W + tx * 8u + r3 % tw;
+}
+
 __global__ __launch_bounds__(kBlock) void k_raygen(PathState P, const CameraDev cam, uint32_t W, uint32_t H, uint32_t row0,
                                                    uint32_t n, uint32_t smp, uint2 key, const uint32_t* __restrict__ sobol,
                                                    uint32_t* queue, uint32_t* counters) {
