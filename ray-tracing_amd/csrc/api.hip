@@ -64,9 +64,13 @@ static uint32_t pass_paths() {
     return v;
 }
 
-// k_shade's grid, in blocks per CU
+// k_shade's grid, in blocks per CU (at most; 2 are resident): with more
+// blocks each walks fewer strided items and the queues it writes follow the
+// shading order more closely (64-spp frame: 4 -> 430, 8 -> 432, 32 -> 437,
+// 128 -> 442 Msamples/s; one item per thread 426, the blocks' dispatch and
+// barriers then dominate)
 #ifndef LRT_SHADE_BLOCKS_PER_CU
-#define LRT_SHADE_BLOCKS_PER_CU 8
+#define LRT_SHADE_BLOCKS_PER_CU 128
 #endif
 // live paths at or below which a pass's remaining bounces run in k_tail
 // (one lane finishes a path) instead of one wavefront launch per bounce, from
