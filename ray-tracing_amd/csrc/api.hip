@@ -72,6 +72,11 @@ static uint32_t pass_paths() {
 #ifndef LRT_SHADE_BLOCKS_PER_CU
 #define LRT_SHADE_BLOCKS_PER_CU 128
 #endif
+// the photon eye pass's k_eye_shade (its generations' queues gain nothing
+// from a larger grid: 128 blocks per CU made the eye pass 0.7% slower)
+#ifndef LRT_EYE_SHADE_BLOCKS_PER_CU
+#define LRT_EYE_SHADE_BLOCKS_PER_CU 16
+#endif
 // live paths at or below which a pass's remaining bounces run in k_tail
 // (one lane finishes a path) instead of one wavefront launch per bounce, from
 // the Russian-roulette depth on; LINRT_TAIL_PATHS overrides (0: never)
@@ -1498,7 +1503,7 @@ static void diff_wave_run(lrt_scene& sc, const Mats& M, const DiffParams& D, con
     unsigned long long* dstats = sc.stats.as<unsigned long long>();
     const uint32_t g_all = (n + kBlock - 1) / kBlock;
     const uint32_t g_trace = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc.cus * 20);
-    const uint32_t g_shade = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc.cus * 8);
+    const uint32_t g_shade = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc.cus * 8);  // 128 per CU: 4% slower
     const uint32_t g_bins = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc.cus * 4);
     if (img) hipLaunchKernelGGL(k_zero_f32, dim3(grid_for(sc, (uint64_t)npix * 3, 4)), dim3(kBlock), 0, sc.stream, img,
                                 (long long)npix * 3);
@@ -2246,7 +2251,7 @@ static void render_photon_impl(lrt_scene* sc, const lrt_render_params* prm, doub
                 // [0..2] appends of k_eye_shade, [4] / [5] ray fetches of k_eye_extend / k_eye_shadow
                 const uint32_t zero8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
                 HIPCHECK(hipMemcpyAsync(counters, zero8, sizeof(zero8), hipMemcpyHostToDevice, sc->stream));
-                const uint32_t g = (uint32_t)std::min<uint64_t>((n + kBlock - 1) / kBlock, (uint64_t)sc->cus * 16);
+                const uint32_t g = (uint32_t)std::min<uint64_t>((n + kBlock - 1) / kBlock, (uint64_t)sc->cus * LRT_EYE_SHADE_BLOCKS_PER_CU);
                 auto e1 = tc.next();
                 HIPCHECK(hipEventRecord(e1.first, sc->stream));
                 hipLaunchKernelGGL(k_eye_extend, dim3(g), dim3(kBlock), 0, sc->stream, sc->S, A, n, Hb, counters + 4);
