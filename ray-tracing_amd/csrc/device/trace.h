@@ -435,6 +435,11 @@ LRT_D bool trace_occluded(const DevScene& S, const Ray& r, float tmin, float tma
 #ifndef LRT_SHADE_BLOCK_APPEND
 #define LRT_SHADE_BLOCK_APPEND 1
 #endif
+// k_shade groups the block's shadow records by direction octant as well
+// (no traversal gain on the 1080p frame, shading +0.6 ms: off)
+#ifndef LRT_SHADOW_OCTANTS
+#define LRT_SHADOW_OCTANTS 0
+#endif
 // the photon eye pass's shading kernel appends once per block as well
 #ifndef LRT_EYE_BLOCK_APPEND
 #define LRT_EYE_BLOCK_APPEND 1

@@ -265,7 +265,7 @@ __global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_diff_shade(const DevS
         }
 #if LRT_SHADE_BLOCK_APPEND
         // one append per block and queue, as k_shade
-        const uint2 sl = block_append2<kBlock>(ba, cont, octant, shadow, &counters[1], &counters[2]);
+        const uint2 sl = block_append2<kBlock>(ba, cont, octant, shadow, 0u, &counters[1], &counters[2]);
         if (cont) next_queue[sl.x] = p;
         const uint32_t ss = sl.y;
 #else

@@ -31,15 +31,11 @@ LRT_D uint32_t wave_append(uint32_t* counter, bool want) {
 // the octant of a direction (sign bits of x, y, z)
 LRT_D uint32_t dir_octant(const f3& d) { return (d.x < 0.0f ? 1u : 0u) | (d.y < 0.0f ? 2u : 0u) | (d.z < 0.0f ? 4u : 0u); }
 
-// Block-aggregated appends to two queues, for kernels whose every thread calls
-// it the same number of times: per wave, ballots count queue A's items of each
-// key (0..7) and queue B's items; thread 0 reserves the block's slots with one
-// atomic per queue.  (A wave's own appends make one atomic per queue and 64
-// items on the same counter, which the memory-side atomic unit serialises.)
-// A's items are laid out key by key, then wave by wave and lane by lane; B's
-// wave by wave.  Returns the thread's slots in A and B (where a / b).
-// One queue, no keys: the same as block_append2's B.  Returns the thread's
-// slot, UINT32_MAX where it does not append.
+// Block-aggregated append to one queue, for kernels whose every thread calls
+// it the same number of times: per wave, a ballot counts the items; thread 0
+// reserves the block's slots with one atomic.  (A wave's own append makes one
+// atomic per 64 items on the same counter, which the memory-side atomic unit
+// serialises.)  Returns the thread's slot, UINT32_MAX where it does not append.
 template <uint32_t NT>
 struct BlockAppend1 {
     uint32_t cnt[NT / 64], at[NT / 64];
@@ -63,46 +59,47 @@ LRT_D uint32_t block_append1(BlockAppend1<NT>& sm, bool want, uint32_t* counter)
     return want ? sm.at[wv] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)) : 0xffffffffu;
 }
 
+// The same for two queues A and B with a key (0..7) per item: each queue's
+// items are laid out key by key, then wave by wave and lane by lane, one
+// atomic per queue and block.  Returns the thread's slots in A and B (where
+// a / b).
 template <uint32_t NT>
 struct BlockAppend {
-    uint32_t cnt[NT / 64][9], at[NT / 64][9];
+    uint32_t cnt[NT / 64][16], at[NT / 64][16];  // [0, 8): A's keys, [8, 16): B's
 };
 template <uint32_t NT>
-LRT_D uint2 block_append2(BlockAppend<NT>& sm, bool a, uint32_t key, bool b, uint32_t* ca, uint32_t* cb) {
+LRT_D uint2 block_append2(BlockAppend<NT>& sm, bool a, uint32_t key_a, bool b, uint32_t key_b, uint32_t* ca,
+                          uint32_t* cb) {
     constexpr uint32_t kW = NT / 64;
     const uint32_t wv = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint64_t lt = (1ull << lane) - 1ull;
-    uint32_t rank = 0u;
+    uint32_t rank_a = 0u, rank_b = 0u;
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k) {
-        const bool in = a && key == k;
-        const uint64_t m = __ballot(in);
-        if (lane == 0) sm.cnt[wv][k] = (uint32_t)__popcll(m);
-        if (in) rank = (uint32_t)__popcll(m & lt);
+        const bool in_a = a && key_a == k, in_b = b && key_b == k;
+        const uint64_t ma = __ballot(in_a), mb = __ballot(in_b);
+        if (lane == 0) {
+            sm.cnt[wv][k] = (uint32_t)__popcll(ma);
+            sm.cnt[wv][8 + k] = (uint32_t)__popcll(mb);
+        }
+        if (in_a) rank_a = (uint32_t)__popcll(ma & lt);
+        if (in_b) rank_b = (uint32_t)__popcll(mb & lt);
     }
-    const uint64_t mb = __ballot(b);
-    if (lane == 0) sm.cnt[wv][8] = (uint32_t)__popcll(mb);
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t ta = 0u, tb = 0u;
-        for (uint32_t k = 0; k < 8; ++k)
+        uint32_t t[2] = {0u, 0u};
+        for (uint32_t k = 0; k < 16; ++k)
             for (uint32_t v = 0; v < kW; ++v) {
-                sm.at[v][k] = ta;
-                ta += sm.cnt[v][k];
+                sm.at[v][k] = t[k >> 3];
+                t[k >> 3] += sm.cnt[v][k];
             }
-        for (uint32_t v = 0; v < kW; ++v) {
-            sm.at[v][8] = tb;
-            tb += sm.cnt[v][8];
-        }
-        const uint32_t ba = ta ? atomicAdd(ca, ta) : 0u;
-        const uint32_t bb = tb ? atomicAdd(cb, tb) : 0u;
-        for (uint32_t v = 0; v < kW; ++v) {
-            for (uint32_t k = 0; k < 8; ++k) sm.at[v][k] += ba;
-            sm.at[v][8] += bb;
-        }
+        const uint32_t ba = t[0] ? atomicAdd(ca, t[0]) : 0u;
+        const uint32_t bb = t[1] ? atomicAdd(cb, t[1]) : 0u;
+        for (uint32_t v = 0; v < kW; ++v)
+            for (uint32_t k = 0; k < 16; ++k) sm.at[v][k] += k < 8 ? ba : bb;
     }
     __syncthreads();
-    return make_uint2(a ? sm.at[wv][key] + rank : 0u, b ? sm.at[wv][8] + (uint32_t)__popcll(mb & lt) : 0u);
+    return make_uint2(a ? sm.at[wv][key_a] + rank_a : 0u, b ? sm.at[wv][8 + key_b] + rank_b : 0u);
 }
 
 // ---------------------------------------------------------------------------
@@ -713,8 +710,9 @@ __global__ __launch_bounds__(kShadeBlock) LRT_SHADE_ATTR void k_shade(const DevS
         // one append per block and queue, the continuing paths laid out by
         // direction octant: a traversal wave takes rays of one shading cell
         // and one octant
-        const uint2 sl = block_append2<kShadeBlock>(ba, o.cont, o.cont ? o.octant : 0u, o.shadow, &counters[1],
-                                                    &counters[2]);
+        const uint2 sl = block_append2<kShadeBlock>(ba, o.cont, o.cont ? o.octant : 0u, o.shadow,
+                                                    o.shadow && LRT_SHADOW_OCTANTS ? dir_octant(o.sd) : 0u,
+                                                    &counters[1], &counters[2]);
         if (o.cont) next_queue[sl.x] = p;
         const uint32_t ss = sl.y;
 #else
