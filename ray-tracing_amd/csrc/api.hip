@@ -79,18 +79,32 @@ static uint32_t pass_paths() {
 #endif
 // live paths at or below which a pass's remaining bounces run in k_tail
 // (one lane finishes a path) instead of one wavefront launch per bounce, from
-// the Russian-roulette depth on; LINRT_TAIL_PATHS overrides (0: never)
-#ifndef LRT_TAIL_PATHS
-#define LRT_TAIL_PATHS (1u << 18)
+// the Russian-roulette depth on.  A large pass is served better by the
+// wavefront down to few paths (its late launches still hold 10^5-10^6 paths),
+// a small one by an early tail (each wavefront launch lasts at least as long
+// as its slowest ray): LRT_TAIL_SMALL for passes of at most LRT_SMALL_PASS
+// paths, else LRT_TAIL_LARGE (1080p, 64 spp = 2^27 paths: 2^17 -> 442.8,
+// 2^18 -> 440.4, 2^19 -> 433.4 Msamples/s; 1 spp = 2^21 paths: 2^17 -> 17.3,
+// 2^18 -> 15.0, 2^19 -> 14.6 ms per denoised frame, 2 and 4 spp likewise
+// faster at 2^19).  LINRT_TAIL_PATHS sets a fixed threshold (0: never).
+#ifndef LRT_TAIL_SMALL
+#define LRT_TAIL_SMALL (1u << 19)
 #endif
-static uint32_t tail_paths() {
-    static const uint32_t v = [] {
+#ifndef LRT_TAIL_LARGE
+#define LRT_TAIL_LARGE (1u << 17)
+#endif
+#ifndef LRT_SMALL_PASS
+#define LRT_SMALL_PASS (1u << 23)
+#endif
+static uint32_t tail_paths(uint32_t pass_paths) {
+    static const long long fixed = [] {
         const char* e = std::getenv("LINRT_TAIL_PATHS");
-        if (!e) return (uint32_t)LRT_TAIL_PATHS;
+        if (!e) return -1ll;
         const long long x = std::atoll(e);
-        return x >= 0 && x < (1ll << 31) ? (uint32_t)x : (uint32_t)LRT_TAIL_PATHS;
+        return x >= 0 && x < (1ll << 31) ? x : -1ll;
     }();
-    return v;
+    if (fixed >= 0) return (uint32_t)fixed;
+    return pass_paths <= LRT_SMALL_PASS ? LRT_TAIL_SMALL : LRT_TAIL_LARGE;
 }
 
 namespace lrt {
@@ -1226,9 +1240,7 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
         const uint32_t g_bins = (uint32_t)std::min<uint64_t>(g_all, (uint64_t)sc->cus * 4);
         // the tail kernel: not with the counting kernels (they count per bounce), ray capture or
         // LRT_RENDER_WAVEFRONT_ONLY
-        const uint32_t tail = (counting || capture || (prm->flags & LRT_RENDER_WAVEFRONT_ONLY)) ? 0u : tail_paths();
-        const uint32_t g_tail = (uint32_t)std::max<uint64_t>(
-            1, std::min<uint64_t>(((uint64_t)tail + kBlock - 1) / kBlock, (uint64_t)sc->cus * 8));
+        const bool tail_on = !(counting || capture || (prm->flags & LRT_RENDER_WAVEFRONT_ONLY));
         Timer& tc = sc->timers[LRT_KERNEL_CLOSEST];
         Timer& to = sc->timers[LRT_KERNEL_OCCLUDED];
         Timer& ts = sc->timers[LRT_KERNEL_SHADE];
@@ -1237,6 +1249,9 @@ static void render_impl(lrt_scene* sc, const lrt_render_params* prm, double* sum
             const uint32_t smp = prm->spp_offset + s;
             const uint32_t ns = std::min(slots, prm->spp - s);  // samples in this pass
             const uint32_t np = npix * ns;
+            const uint32_t tail = tail_on ? tail_paths(np) : 0u;
+            const uint32_t g_tail = (uint32_t)std::max<uint64_t>(
+                1, std::min<uint64_t>(((uint64_t)tail + kBlock - 1) / kBlock, (uint64_t)sc->cus * 8));
             hipLaunchKernelGGL(k_raygen, dim3((std::max(np, 1024u) + kBlock - 1) / kBlock), dim3(kBlock), 0, sc->stream, P, sc->cam, W, H, rb,
                                np, smp, key, sobol, qa, counters);
             uint32_t* q = qa;
