@@ -87,6 +87,18 @@ LRT_D bool emitter_nonconvex(const DevScene& S, int e) {
 #else
 #define LRT_SHADE_ATTR
 #endif
+// k_tail (shading and both traversals in one lane per path): more waves than
+// k_shade's 2 hide more of its dependent loads, at the price of spills
+// (denoised 1-spp 1080p frame: 2 -> 14.9, 3 -> 14.3, 4 -> 14.3, 5 -> 14.0,
+// 6 -> 14.1 ms; 2 spp 23.4 -> 21.8 ms at 5)
+#ifndef LRT_TAIL_WAVES
+#define LRT_TAIL_WAVES 5
+#endif
+#if LRT_TAIL_WAVES > 0
+#define LRT_TAIL_ATTR __attribute__((amdgpu_waves_per_eu(LRT_TAIL_WAVES)))
+#else
+#define LRT_TAIL_ATTR
+#endif
 constexpr int kLdsStack = LRT_LDS_STACK;
 
 constexpr int kLeafBatch = LRT_LEAF_BATCH;

@@ -740,7 +740,7 @@ __global__ __launch_bounds__(kShadeBlock) LRT_SHADE_ATTR void k_shade(const DevS
 // bounce).  Extension and shadow rays traced are added to stats[0] / stats[1];
 // k_tail_done then empties the queue for the remaining wavefront launches.
 template <bool MESH_LIGHTS>
-__global__ __launch_bounds__(kBlock) LRT_SHADE_ATTR void k_tail(const DevScene S, const DevLights Ls, const Mats M, PathState P,
+__global__ __launch_bounds__(kBlock) LRT_TAIL_ATTR void k_tail(const DevScene S, const DevLights Ls, const Mats M, PathState P,
                                                  const uint32_t* __restrict__ queue, const uint32_t* __restrict__ counters,
                                                  const HitBuf Hb, ShadeParams prm, uint32_t limit,
                                                  unsigned long long* __restrict__ stats) {
